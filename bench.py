@@ -1,0 +1,156 @@
+#!/usr/bin/env python3
+"""Throughput of the projection-method time step on one MI355X.
+
+Workload (the reference's Re=1000 LES cavity): unit cube, uniform 128^3
+finite-volume grid, Re = 1000, QUICK advection, Smagorinsky LES with Cs = 0.1
+and filter width (dx dy dz)^(1/3), dt = 5e-4, lid w = 1 on the north wall,
+no-slip elsewhere; rest plus seeded U(-1e-3, 1e-3) noise on u, v, w; SOR with
+omega = 1.5, zeta = 1e-10, itmax = 3000.  Synthetic data (no files).
+
+A step is one full pass of the hot path (LES viscosity, predictor, pressure
+solve, projection with direct forcing, force integral, boundary conditions)
+with the state resident in HBM.  W warm-up steps, then K timed steps
+bracketed by HIP events on the library's stream (device synchronised on both
+sides).  One JSON line on stdout.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Mcell-steps/s, FP32+refine vs FP64 plain @≤1e-6 rel err; adjoint/fwd ratio"
+
+
+def workload(n):
+    return dict(nx=n, ny=n, nz=n, Re=1000.0, dt=5e-4, Cs=0.1, LES=1, scheme=0, omega=1.5,
+                zeta=1e-10, itmax=3000.0, noise=1e-3, seed=2024)
+
+
+def initial_state(grid, w, solver):
+    from ntustcfdlab_amd.grid import noise_ic
+    u, v, ww = noise_ic(grid, w["noise"], w["seed"])
+    p = grid.halo()
+    eta = grid.halo()
+    solver.boundary_conditions(u, v, ww, p)
+    return u, v, ww, p, eta
+
+
+def cpu_baseline(grid, w, state, sweeps_per_step):
+    """The CPU restatement (one core) timed on a bounded sample of the same
+    workload: one step from the same state with the sweep cap at 4 and at 12;
+    the difference gives the time per SOR sweep, the rest the time of the
+    other kernels; the step time is extrapolated to the GPU run's mean sweep
+    count."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    from oracle import OracleRun
+    from ntustcfdlab_amd.grid import bc_codes
+
+    def one(cap):
+        r = OracleRun(grid, dt=w["dt"], nu=1.0 / w["Re"], Cs=w["Cs"], omega=w["omega"],
+                      zeta=w["zeta"], itmax=cap, LES=w["LES"], scheme=w["scheme"], sor=0,
+                      bc=bc_codes())
+        u, v, ww, p, eta = state
+        r.f["u"][...], r.f["v"][...], r.f["w"][...], r.f["p"][...] = u, v, ww, p
+        r.f["ETA"][...] = eta
+        r.apply_bc()
+        t0 = time.perf_counter()
+        ik = r.step()[0]
+        return time.perf_counter() - t0, ik
+
+    t_a, ik_a = one(4.0)
+    t_b, ik_b = one(12.0)
+    t_sweep = (t_b - t_a) / max(ik_b - ik_a, 1)
+    t_rest = max(t_a - ik_a * t_sweep, 0.0)
+    t_step = t_rest + sweeps_per_step * t_sweep
+    return {"value": grid.cells / t_step / 1e6, "unit": "Mcell-steps/s", "cores": 1,
+            "kind": "port",
+            "sample": (f"CPU restatement, 1 thread, one {grid.nx}^3 step at sweep caps 4 and 12 "
+                       f"({t_a + t_b:.1f} s): {t_sweep * 1e3:.1f} ms/SOR sweep (lexicographic), "
+                       f"{t_rest:.2f} s other kernels, extrapolated to "
+                       f"{sweeps_per_step:.0f} sweeps/step")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--size", type=int, default=128)
+    ap.add_argument("--sor", choices=["redblack", "lexicographic"], default="redblack")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported: the path runs "
+              "the whole domain on one GPU (no halo exchange, no collectives)", file=sys.stderr)
+        sys.exit(2)
+
+    from ntustcfdlab_amd import hip, hiptime
+    from ntustcfdlab_amd.grid import Grid, bc_codes
+
+    w = workload(a.size)
+    g = Grid(w["nx"], w["ny"], w["nz"])
+    s = hip.ProjectionSolver(g, bc_codes())
+    sor = hip.SOR_REDBLACK if a.sor == "redblack" else hip.SOR_LEXICOGRAPHIC
+    s.configure(dt=w["dt"], nu=1.0 / w["Re"], Cs=w["Cs"], LES=w["LES"], scheme=w["scheme"],
+                omega=w["omega"], zeta=w["zeta"], itmax=w["itmax"], sor=sor, lx=1.0,
+                force_every=1)
+    state = initial_state(g, w, s)
+    s.upload(*state)
+    print(f"# warmup {a.warmup} steps", file=sys.stderr, flush=True)
+    if a.warmup:
+        s.step(a.warmup)
+    s.prof_reset()
+    e0, e1 = hiptime.Event(), hiptime.Event()
+    hiptime.device_synchronize()
+    print(f"# timing {a.steps} steps", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    e0.record(s.stream())
+    ik, pm, cd, cl, rc = s.step(a.steps)
+    e1.record(s.stream())
+    ms = e0.elapsed_ms(e1)
+    hiptime.device_synchronize()
+    wall = time.perf_counter() - t0
+    prof_ms, launches = s.prof_read()
+    s.close()
+
+    cells = g.cells
+    value = cells * a.steps / (ms / 1e3) / 1e6
+    sweeps = float(np.mean(ik))
+    roof = None
+    if launches:
+        avg_ms = prof_ms / launches
+        bytes_per_launch = 12.0 * cells           # half-sweep: read p, b; write p (FP64)
+        achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_rb_sweep (one colour half-sweep)",
+                "avg_launch_us": round(avg_ms * 1e3, 3), "launches_timed": launches}
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Mcell-steps/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms / a.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": (f"LES cavity Re=1000 {a.size}^3 uniform, QUICK, Smagorinsky "
+                                f"Cs=0.1, dt=5e-4, SOR omega=1.5 zeta=1e-10 itmax=3000 "
+                                f"({a.sor}), FP64"),
+                   "cells": cells, "sor": a.sor, "sweeps_per_step": round(sweeps, 1),
+                   "wall_s": round(wall, 3)},
+        "roofline": roof,
+    }
+    if not a.no_cpu_baseline:
+        print("# cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(g, w, state, sweeps)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

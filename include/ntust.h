@@ -1,0 +1,179 @@
+/*
+ * ntust.h -- C ABI of libntust_hip.so, the gfx950 implementation of the
+ * per-time-step hot path of the HSUAN221/NTUSTCFDLAB projection solver.
+ *
+ * Conventions
+ *  - Every array argument is the host base address of a Fortran module array
+ *    of the solver, i.e. its first element: (-1,-1,-1) of a halo array
+ *    dimensioned (-1:nx+2,-1:ny+2,-1:nz+2), (1,1,1) of an interior array
+ *    (1:nx,1:ny,1:nz), element -1 of a metric array (-1:n+2).  Column-major,
+ *    x fastest, real*8.  From Fortran these bind as `real(c_double) :: a(*)`
+ *    and are passed without a copy (variables_module.f90:159,215-225,281-316).
+ *  - Scalars are passed by value.  istart/iend are the z-slab bounds of
+ *    Mpi_division.f90:11-24; on one GPU they are 1 and nz.
+ *  - Every call returns an int status: 0 ok, > 0 a warning (NTUST_WARN_*),
+ *    < 0 an error (NTUST_E_*; ntust_last_error() has the message).  The
+ *    reference has no error convention: its callers ignore ierr and a
+ *    non-converged pressure solve continues silently (GS.f90:11), which is
+ *    what ignoring a warning does.
+ *  - Not re-entrant: one host thread, outside any OpenMP parallel region,
+ *    as main.f90:139-355 calls the hot path.  Calls are stream ordered and
+ *    synchronous at return.
+ *  - Drop-in calls (ntust_<subroutine>) copy the subroutine's read set to
+ *    the device, run it and copy its write set back, so each replaces one
+ *    reference call without changing the caller.  The device-resident calls
+ *    (ntust_state_*, ntust_step) keep the state in HBM across steps.
+ */
+#ifndef NTUST_H
+#define NTUST_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    NTUST_OK = 0,
+    NTUST_WARN_ITMAX = 1,        /* SOR stopped at itmax before max|dp| <= zeta */
+    NTUST_E_NOT_INIT = -1,
+    NTUST_E_ARG = -2,
+    NTUST_E_HIP = -3,
+    NTUST_E_ALLOC = -4
+};
+
+/* pressure-solver orderings (ntust_configure sor_mode) */
+enum {
+    NTUST_SOR_LEXICOGRAPHIC = 0,  /* GS.f90 order, bit-identical, one workgroup */
+    NTUST_SOR_REDBLACK = 1,       /* red-black, FP64 */
+    NTUST_SOR_MIXED = 2           /* red-black, FP32 sweeps inside FP64 defect correction */
+};
+
+/* advection schemes (ntust_configure scheme) */
+enum {
+    NTUST_QUICK_CENTRE = 0,       /* discretisation_QUICK_centre.f90 */
+    NTUST_UPWIND = 1              /* discretisation_upwind.f90 */
+};
+
+/* ---- lifecycle ------------------------------------------------------ */
+
+/* Sets up the device for an nx*ny*nz grid and copies the six metric
+ * arrays the gridder built (gridder.f90:267-327; variables_module.f90:306-316).
+ * Replaces the device-side part of the initialisation main.f90:77-92.
+ * Calling it again re-initialises (finalize first). */
+int ntust_init(int nx, int ny, int nz, const double *iDx, const double *Dxs,
+               const double *iDy, const double *Dys, const double *iDz,
+               const double *Dzs);
+
+/* Frees all device memory.  Called before MPI_FINALIZE (main.f90:362). */
+int ntust_finalize(void);
+
+/* Text of the last error. */
+const char *ntust_last_error(void);
+
+/* The 18 boundary-condition strings of variables_module.f90:110-132 as
+ * integer codes (0 'no-slip', 1 'Neumann', 2 'Dirichlet', else no rule),
+ * order W_u W_v W_w E_u E_v E_w S_u S_v S_w N_u N_v N_w B_u B_v B_w F_u F_v F_w. */
+int ntust_set_bc(const int *codes);
+
+/* ---- drop-in replacements, one per reference subroutine -------------- */
+
+/* smagorinsky.f90:42-87 GradientPhiGauss(Phi,dPhidX,Dxm,Dym,Dzm).  Phi is
+ * read through the (nx,ny,nz) shape the reference declares; prebase (nx*ny
+ * words, may be NULL = zeros) stands for the memory before Phi that the
+ * k=1 plane reads.  dPhidX is (nx,ny,nz,3). */
+int ntust_gradient_phi_gauss(const double *Phi, const double *prebase, double *dPhidX,
+                             const double *Dxm, const double *Dym, const double *Dzm,
+                             int istart, int iend);
+
+/* smagorinsky.f90:1-37 CalculateSmagorinskyViscosity.  Cs is the model
+ * constant (variables_module.f90:277).  dudx/dvdx/dwdx (nx,ny,nz,3) may be
+ * NULL when the caller does not keep them.  pre_u/pre_v/pre_w: as prebase. */
+int ntust_smagorinsky(const double *u, const double *v, const double *w,
+                      const double *pre_u, const double *pre_v, const double *pre_w,
+                      double Cs, int istart, int iend, double *dudx, double *dvdx,
+                      double *dwdx, double *Viseff);
+
+/* discretisation_QUICK_centre.f90:1-345.  LES is the integer switch of
+ * main.f90:60.  Writes u_star, v_star, w_star over 1..nx,1..ny,istart..iend. */
+int ntust_discretisation_quick_centre(const double *u, const double *v, const double *w,
+                                      const double *Viseff, double dt, double nu, int LES,
+                                      int istart, int iend, double *u_star, double *v_star,
+                                      double *w_star);
+
+/* discretisation_upwind.f90:1-275 with its undeclared switch Q1 = 0. */
+int ntust_discretisation_upwind(const double *u, const double *v, const double *w,
+                                double dt, double nu, double *u_star, double *v_star,
+                                double *w_star);
+
+/* GS.f90:1-106 gauss_seidel.  itmax is real*8 as in variables_module.f90:233.
+ * Uses the ordering set by ntust_configure (default lexicographic).  Writes
+ * p (interior), pre, *ik and *pChangeMax.  Returns NTUST_WARN_ITMAX when the
+ * sweep cap ended the solve. */
+int ntust_gauss_seidel(const double *u_star, const double *v_star, const double *w_star,
+                       double *p, double *pre, double dt, double omega, double zeta,
+                       double itmax, int *ik, double *pChangeMax);
+
+/* calcul_new_velocity.f90:1-184.  u2, v2, w2 are read on their lowest ghost
+ * layer (the centring of :171-181) and written on the interior. */
+int ntust_calcul_new_velocity(const double *u_star, const double *v_star,
+                              const double *w_star, const double *p, const double *ETA,
+                              double u_solid, double v_solid, double w_solid, double dt,
+                              int istart, int iend, double *u1, double *v1, double *w1,
+                              double *u2, double *v2, double *w2, double *FX, double *FY,
+                              double *FZ, double *uc, double *vc, double *wc);
+
+/* calcul_new_velocity.f90:195-243 Updating_velocity: u = u2 on the interior. */
+int ntust_updating_velocity(const double *u2, const double *v2, const double *w2,
+                            int istart, int iend, double *u, double *v, double *w);
+
+/* virtualForceIntegrator.f90:3-85.  lx is the span (variables_module.f90:19). */
+int ntust_virtual_force_integrator(const double *FY, const double *FZ, double lx,
+                                   double *cDrag, double *cLift);
+
+/* boundary_conditions.f90:1-376 with the codes of ntust_set_bc: velocity
+ * faces, whole-array copies into u1/u2/u_star (v, w alike; any target may
+ * be NULL) and the Neumann pressure ghosts. */
+int ntust_boundary_conditions(double *u, double *v, double *w, double *p, double *u1,
+                              double *v1, double *w1, double *u2, double *v2, double *w2,
+                              double *u_star, double *v_star, double *w_star);
+
+/* ---- device-resident time stepping ------------------------------------ */
+
+/* Step parameters (variables_module.f90:9-23,277; main.f90:44-60):
+ * dt, nu = 1/Re, Cs, LES switch, advection scheme, SOR omega/zeta/itmax and
+ * ordering, span lx for the force coefficients, force_every: evaluate the
+ * force integral every force_every steps (0 = never; the reference: 1). */
+int ntust_configure(double dt, double nu, double Cs, int LES, int scheme, double omega,
+                    double zeta, double itmax, int sor_mode, double lx, int force_every);
+
+/* Copies the state u, v, w, p (halo arrays, after boundary_conditions) and
+ * ETA to the device. */
+int ntust_state_upload(const double *u, const double *v, const double *w, const double *p,
+                       const double *ETA);
+
+/* Copies the state back; uc/vc/wc (cell-centred velocity of the last step)
+ * and pre may be NULL. */
+int ntust_state_download(double *u, double *v, double *w, double *p, double *pre,
+                         double *uc, double *vc, double *wc);
+
+/* Runs nsteps time steps in the order of main.f90:209-259 without leaving
+ * the device.  Per-step outputs (host arrays of nsteps entries, each may be
+ * NULL): SOR sweep count, last max|dp|, cDrag, cLift.  Returns
+ * NTUST_WARN_ITMAX if any step hit the sweep cap. */
+int ntust_step(int nsteps, int *ik, double *pchange, double *cdrag, double *clift);
+
+/* ---- measurement -------------------------------------------------------- */
+
+/* The library's HIP stream (hipStream_t), for event timing by the caller. */
+void *ntust_stream(void);
+
+/* Timing of the dominant kernel, the SOR half-sweep: batches of sweeps in
+ * which every sweep ran are bracketed with HIP events on the library
+ * stream.  ntust_prof_reset zeroes the counters; ntust_prof_read returns the
+ * summed milliseconds and the number of half-sweep launches they cover. */
+int ntust_prof_reset(void);
+int ntust_prof_read(double *ms, long long *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,254 @@
+// k_sor.hip -- pressure Poisson solve by SOR (gfx950).
+//
+// Restates GS.f90:1-106: point SOR on
+//   sum_faces A_f (p_nb - p_P)/d_f = (1/dt) sum_faces A_f u*_f . n
+// with the stopping rule "sweep while max|pNew - p| > zeta and ik < itmax"
+// (GS.f90:11, the change measured before relaxation, :53) and the ghost
+// values held fixed during the solve.
+//
+// Two orderings:
+//  * lexicographic (exact): the reference's k, j, i order.  Cell (i,j,k)
+//    reads the new values of its three lower neighbours and the old values
+//    of its three upper ones, so all cells on one hyperplane i+j+k = s
+//    depend only on plane s-1; sweeping the planes in increasing s inside
+//    one workgroup reproduces the in-place lexicographic sweep bit for bit.
+//    The whole do-while loop runs in a single kernel launch.
+//  * red-black (fast): colour (i+j+k)&1, red first, on colour-split copies
+//    of p and of the right-hand side b = mChange/dt, so each half-sweep
+//    streams one colour plane of p and b and reads the other colour through
+//    the caches.  Each half-sweep kernel decides by itself whether its sweep
+//    runs, from the previous sweep's max|pNew - p| left in device memory, so
+//    the host can queue many sweeps without a round trip and the sweep count
+//    still stops exactly where GS.f90:11 stops.
+#include "ntust_kernels.h"
+
+namespace ntust {
+
+// max over a 64-wide wavefront, then over the block; non-negative inputs
+__device__ __forceinline__ double wave_max(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        double t = __shfl_xor(v, o, 64);
+        v = t > v ? t : v;
+    }
+    return v;
+}
+
+// ---------------------------------------------------------------------
+// exact lexicographic SOR
+// ---------------------------------------------------------------------
+
+__global__ void __launch_bounds__(1024)
+k_sor_lexicographic(KGrid g, const double *__restrict__ us, const double *__restrict__ vs,
+                    const double *__restrict__ ws, double *p, double dt, double omega,
+                    double zeta, double itmax, SorControl *ctl, double *pmax_out)
+{
+    __shared__ double red[16];
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int tid = threadIdx.x;
+    double last = 1.0;   // pChangeMax_ before the first sweep (GS.f90:9)
+    int ik = 0;
+    while (last > zeta && (double)ik < itmax) {
+        ik++;
+        double lmax = 0.0;
+        for (int s = 3; s <= nx + ny + nz; s++) {
+            const int klo = max(1, s - nx - ny), khi = min(nz, s - 2);
+            const int cnt = ny * (khi - klo + 1);
+            for (int q = tid; q < cnt; q += blockDim.x) {
+                const int k = q / ny + klo;
+                const int j = q - (k - klo) * ny + 1;
+                const int i = s - j - k;
+                if (i < 1 || i > nx) continue;
+                const double ax = g.iDx[i], ay = g.iDy[j], az = g.iDz[k];
+                const long o = hix(g, i, j, k);
+                const double mChange = (us[o] - us[o - 1]) * ay * az
+                                     + (vs[o] - vs[o - g.sx]) * ax * az
+                                     + (ws[o] - ws[o - g.sxy]) * ax * ay;
+                const double pNew = (-p[o + 1] * ay * az / g.Dxs[i]
+                                     - p[o - 1] * ay * az / g.Dxs[i - 1]
+                                     - p[o + g.sx] * ax * az / g.Dys[j]
+                                     - p[o - g.sx] * ax * az / g.Dys[j - 1]
+                                     - p[o + g.sxy] * ax * ay / g.Dzs[k]
+                                     - p[o - g.sxy] * ax * ay / g.Dzs[k - 1]
+                                     + mChange / dt)
+                                  / (-ay * az / g.Dxs[i] - ay * az / g.Dxs[i - 1]
+                                     - ax * az / g.Dys[j] - ax * az / g.Dys[j - 1]
+                                     - ax * ay / g.Dzs[k] - ax * ay / g.Dzs[k - 1]);
+                const double pc = fabs(pNew - p[o]);
+                p[o] = p[o] + (omega * (pNew - p[o]));
+                if (pc > lmax) lmax = pc;
+            }
+            __syncthreads();
+        }
+        lmax = wave_max(lmax);
+        if ((tid & 63) == 0) red[tid >> 6] = lmax;
+        __syncthreads();
+        double m = 0.0;
+        for (int q = 0; q < (int)(blockDim.x >> 6); q++) m = red[q] > m ? red[q] : m;
+        __syncthreads();
+        last = m;
+    }
+    if (tid == 0) {
+        ctl->ik = ik;
+        *pmax_out = last;
+    }
+}
+
+void launch_sor_lexicographic(const KGrid &g, const double *us, const double *vs,
+                              const double *ws, double *p, double dt, double omega,
+                              double zeta, double itmax, SorControl *ctl, double *pmax,
+                              hipStream_t st)
+{
+    hipLaunchKernelGGL(k_sor_lexicographic, dim3(1), dim3(1024), 0, st, g, us, vs, ws, p, dt,
+                       omega, zeta, itmax, ctl, pmax);
+}
+
+// ---------------------------------------------------------------------
+// red-black SOR on colour-split storage
+// ---------------------------------------------------------------------
+
+// p (halo) -> colour planes, cells 0..n+1 on every axis
+__global__ void k_rb_gather(KGrid g, SplitGrid s, const double *__restrict__ p,
+                            double *__restrict__ p0, double *__restrict__ p1)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i > g.nx + 1 || j > g.ny + 1) return;
+    const double val = p[hix(g, i, j, k)];
+    if ((i + j + k) & 1) p1[six(s, i, j, k)] = val;
+    else p0[six(s, i, j, k)] = val;
+}
+
+// b = mChange/dt on the interior, GS.f90:39-41 and the mChange/dt of :49
+__global__ void k_rb_rhs(KGrid g, SplitGrid s, const double *__restrict__ us,
+                         const double *__restrict__ vs, const double *__restrict__ ws,
+                         double dt, double *__restrict__ b0, double *__restrict__ b1)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.nx || j > g.ny) return;
+    const double ax = g.iDx[i], ay = g.iDy[j], az = g.iDz[k];
+    const long o = hix(g, i, j, k);
+    const double mChange = (us[o] - us[o - 1]) * ay * az
+                         + (vs[o] - vs[o - g.sx]) * ax * az
+                         + (ws[o] - ws[o - g.sxy]) * ax * ay;
+    const double b = mChange / dt;
+    if ((i + j + k) & 1) b1[six(s, i, j, k)] = b;
+    else b0[six(s, i, j, k)] = b;
+}
+
+// colour planes -> p interior and pre (GS.f90:86-94)
+__global__ void k_rb_scatter(KGrid g, SplitGrid s, const double *__restrict__ p0,
+                             const double *__restrict__ p1, double *__restrict__ p,
+                             double *__restrict__ pre)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.nx || j > g.ny) return;
+    const double val = ((i + j + k) & 1) ? p1[six(s, i, j, k)] : p0[six(s, i, j, k)];
+    p[hix(g, i, j, k)] = val;
+    if (pre) pre[iix(g, i, j, k)] = val;
+}
+
+// One half-sweep of colour c, sweep t (1-based).  slots[t-1] holds the
+// previous sweep's max|pNew-p| (slots[0] = 1.0, as GS.f90:9 starts), the
+// kernel adds its own maximum into slots[t].  Point update, bit-identical
+// to the CPU red-black restatement:
+//   aE = (iDy*iDz)*rx(i) ... ; s = fma(aE,pE, fma(aW,pW, ... aB*pB))
+//   aP = aE+aW+aN+aS+aF+aB;  pNew = (s-b)/aP;  p = fma(omega, pNew-p, p)
+template <int C>
+__global__ void __launch_bounds__(256)
+k_rb_sweep(KGrid g, SplitGrid s, const double *__restrict__ rx,
+           const double *__restrict__ ry, const double *__restrict__ rz,
+           double *__restrict__ pc_own, const double *__restrict__ pc_oth,
+           const double *__restrict__ b_own, double omega, double zeta, double itmax, int t,
+           double *slots, SorControl *ctl)
+{
+    __shared__ double red[4];
+    if (ctl->stop) return;
+    const bool active = ((double)(t - 1) < itmax) && (slots[t - 1] > zeta);
+    if (!active) {
+        if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0 &&
+            threadIdx.y == 0)
+            ctl->stop = 1;
+        return;
+    }
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    const int sh = (C + 1 + j + k) & 1;
+    const int i = 2 * q + 1 + sh;
+    double pcm = 0.0;
+    if (i <= g.nx && j <= g.ny) {
+        const int m = q + sh;
+        const long row = (long)s.mp * ((long)j + (long)s.ry * (long)k);
+        const long o = row + m;
+        const double ax = g.iDx[i], ay = g.iDy[j], az = g.iDz[k];
+        const double ayz = ay * az, axz = ax * az, axy = ax * ay;
+        const double aE = ayz * rx[i], aW = ayz * rx[i - 1];
+        const double aN = axz * ry[j], aS = axz * ry[j - 1];
+        const double aF = axy * rz[k], aB = axy * rz[k - 1];
+        const double pE = pc_oth[row + m + 1 - sh], pW = pc_oth[row + m - sh];
+        const double pN = pc_oth[o + s.mp], pS = pc_oth[o - s.mp];
+        const long pl = (long)s.mp * s.ry;
+        const double pF = pc_oth[o + pl], pB = pc_oth[o - pl];
+        const double sum = fma(aE, pE, fma(aW, pW, fma(aN, pN, fma(aS, pS, fma(aF, pF, aB * pB)))));
+        const double aP = ((((aE + aW) + aN) + aS) + aF) + aB;
+        const double pNew = (sum - b_own[o]) / aP;
+        const double pold = pc_own[o];
+        const double d = pNew - pold;
+        pcm = fabs(d);
+        pc_own[o] = fma(omega, d, pold);
+        if (!(pcm > 0.0)) pcm = 0.0;   // a NaN change never raises the max (GS.f90:61)
+    }
+    pcm = wave_max(pcm);
+    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+    if ((tid & 63) == 0) red[tid >> 6] = pcm;
+    __syncthreads();
+    if (tid == 0) {
+        double mm = red[0];
+        for (int w = 1; w < (int)((blockDim.x * blockDim.y) >> 6); w++) mm = red[w] > mm ? red[w] : mm;
+        if (mm > 0.0)
+            atomicMax((unsigned long long *)&slots[t], (unsigned long long)__double_as_longlong(mm));
+        if (C == 1 && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0) ctl->ik = t;
+    }
+}
+
+void launch_rb_gather(const KGrid &g, const SplitGrid &s, const double *p, double *p0,
+                      double *p1, hipStream_t st)
+{
+    dim3 blk(64, 4), grd((g.nx + 2 + 63) / 64, (g.ny + 2 + 3) / 4, g.nz + 2);
+    hipLaunchKernelGGL(k_rb_gather, grd, blk, 0, st, g, s, p, p0, p1);
+}
+
+void launch_rb_rhs(const KGrid &g, const SplitGrid &s, const double *us, const double *vs,
+                   const double *ws, double dt, double *b0, double *b1, hipStream_t st)
+{
+    dim3 blk(64, 4), grd((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz);
+    hipLaunchKernelGGL(k_rb_rhs, grd, blk, 0, st, g, s, us, vs, ws, dt, b0, b1);
+}
+
+void launch_rb_scatter(const KGrid &g, const SplitGrid &s, const double *p0, const double *p1,
+                       double *p, double *pre, hipStream_t st)
+{
+    dim3 blk(64, 4), grd((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz);
+    hipLaunchKernelGGL(k_rb_scatter, grd, blk, 0, st, g, s, p0, p1, p, pre);
+}
+
+void launch_rb_sweep(const KGrid &g, const SplitGrid &s, const double *rx, const double *ry,
+                     const double *rz, double *p0, double *p1, const double *b0,
+                     const double *b1, double omega, double zeta, double itmax, int t,
+                     double *slots, SorControl *ctl, hipStream_t st)
+{
+    const int nq = (g.nx + 1) / 2;
+    dim3 blk(64, 4), grd((nq + 63) / 64, (g.ny + 3) / 4, g.nz);
+    hipLaunchKernelGGL(k_rb_sweep<0>, grd, blk, 0, st, g, s, rx, ry, rz, p0, p1, b0, omega,
+                       zeta, itmax, t, slots, ctl);
+    hipLaunchKernelGGL(k_rb_sweep<1>, grd, blk, 0, st, g, s, rx, ry, rz, p1, p0, b1, omega,
+                       zeta, itmax, t, slots, ctl);
+}
+
+}  // namespace ntust

@@ -1,0 +1,775 @@
+// ntust_api.hip -- host side of libntust_hip.so: the C ABI declared in
+// include/ntust.h, device memory, the stream, and the time-step driver.
+//
+// Device memory is allocated lazily, by name, the first time a call needs a
+// buffer, and kept until ntust_finalize.  Drop-in calls copy whole arrays in
+// and out, so untouched elements of an output keep the caller's values, as
+// the reference's in-place module arrays do.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/ntust.h"
+#include "ntust_kernels.h"
+
+using namespace ntust;
+
+namespace {
+
+char g_err[512] = "";
+
+int fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(call)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail(NTUST_E_HIP, "%s:%d %s: %s", __FILE__, __LINE__, #call,             \
+                        hipGetErrorString(e_));                                            \
+    } while (0)
+
+struct Params {
+    double dt = 1.0e-3, nu = 0.1, Cs = 0.18, omega = 1.5, zeta = 1.0e-4, itmax = 3000.0,
+           lx = 1.0;
+    int les = 0, scheme = NTUST_QUICK_CENTRE, sor = NTUST_SOR_LEXICOGRAPHIC, force_every = 1;
+};
+
+struct Ctx {
+    bool ready = false;
+    int nx = 0, ny = 0, nz = 0;
+    long nh = 0, ni = 0;
+    hipStream_t st = nullptr;
+    KGrid g{};
+    SplitGrid sg{};
+    std::vector<double> h_metric[6];   // iDx Dxs iDy Dys iDz Dzs (host copies)
+    double *metric[6] = {};            // device, base (element -1)
+    double *recip[3] = {};             // 1/Dxs, 1/Dys, 1/Dzs, base
+    BcTable bc{};
+    Params prm;
+    // named device buffers
+    struct Buf { std::string name; double *ptr; size_t n; };
+    std::vector<Buf> bufs;
+    double filt_cs = NAN;              // Cs the filter table was built for
+    SorControl *ctl = nullptr;
+    double *pmax = nullptr;
+    int ik_hint = 0;
+    // measurement of the red-black half-sweep
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double prof_ms = 0.0;
+    long long prof_launches = 0;
+};
+
+Ctx C;
+
+// A buffer asked for again with a larger size is reallocated (its contents
+// are not kept); only the sweep-history buffer of the SOR grows that way.
+double *buf(const char *name, size_t n, bool zero = false)
+{
+    for (auto &b : C.bufs) {
+        if (b.name != name) continue;
+        if (b.n >= n) return b.ptr;
+        if (hipFree(b.ptr) != hipSuccess) return nullptr;
+        b.ptr = nullptr;
+        b.n = 0;
+        if (hipMalloc(&b.ptr, n * sizeof(double)) != hipSuccess) return nullptr;
+        if (zero && hipMemset(b.ptr, 0, n * sizeof(double)) != hipSuccess) return nullptr;
+        b.n = n;
+        return b.ptr;
+    }
+    double *p = nullptr;
+    if (hipMalloc(&p, n * sizeof(double)) != hipSuccess) return nullptr;
+    if (zero && hipMemset(p, 0, n * sizeof(double)) != hipSuccess) return nullptr;
+    C.bufs.push_back({name, p, n});
+    return p;
+}
+
+#define NEED(var, name, n, zero)                                                           \
+    double *var = buf(name, (n), (zero));                                                  \
+    if (!var) return fail(NTUST_E_ALLOC, "device allocation of %s failed", name)
+
+#define NEED_INIT()                                                                        \
+    if (!C.ready) return fail(NTUST_E_NOT_INIT, "ntust_init has not been called")
+
+int h2d(double *d, const double *h, size_t n)
+{
+    HIPCHK(hipMemcpyAsync(d, h, n * sizeof(double), hipMemcpyHostToDevice, C.st));
+    return 0;
+}
+
+int d2h(double *h, const double *d, size_t n)
+{
+    HIPCHK(hipMemcpyAsync(h, d, n * sizeof(double), hipMemcpyDeviceToHost, C.st));
+    return 0;
+}
+
+#define TRY(x)                                                                             \
+    do {                                                                                   \
+        int r_ = (x);                                                                      \
+        if (r_ < 0) return r_;                                                             \
+    } while (0)
+
+int sync() { HIPCHK(hipStreamSynchronize(C.st)); HIPCHK(hipGetLastError()); return 0; }
+
+// (Cs*delta)**2 with delta = (iDx*iDy*iDz)**(1./3.), smagorinsky.f90:20-22:
+// libm pow on the host, the exponent the single-precision literal 1./3.
+int filter_table(double Cs)
+{
+    NEED(filt, "filt", (size_t)C.ni, false);
+    if (C.filt_cs == Cs) return 0;
+    const double third = (double)(1.0f / 3.0f);
+    const std::vector<double> &ix = C.h_metric[0], &iy = C.h_metric[2], &iz = C.h_metric[4];
+    std::vector<double> t((size_t)C.ni);
+    size_t o = 0;
+    for (int k = 1; k <= C.nz; k++)
+        for (int j = 1; j <= C.ny; j++)
+            for (int i = 1; i <= C.nx; i++) {
+                const double d = pow(ix[i + 1] * iy[j + 1] * iz[k + 1], third);
+                const double cd = Cs * d;
+                t[o++] = cd * cd;
+            }
+    HIPCHK(hipMemcpy(filt, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+    C.filt_cs = Cs;
+    return 0;
+}
+
+bool kslab_ok(int istart, int iend)
+{
+    return istart >= 1 && iend <= C.nz && istart <= iend + 1;
+}
+
+// ---- pressure solve on device arrays us, vs, ws -> p, pre --------------
+
+int solve_pressure(const double *us, const double *vs, const double *ws, double *p,
+                   double *pre, double dt, double omega, double zeta, double itmax, int mode,
+                   int *ik_out, double *pmax_out)
+{
+    const KGrid &g = C.g;
+    if (mode == NTUST_SOR_LEXICOGRAPHIC) {
+        HIPCHK(hipMemsetAsync(C.ctl, 0, sizeof(SorControl), C.st));
+        launch_sor_lexicographic(g, us, vs, ws, p, dt, omega, zeta, itmax, C.ctl, C.pmax, C.st);
+        HIPCHK(hipGetLastError());
+        SorControl h;
+        HIPCHK(hipMemcpyAsync(&h, C.ctl, sizeof h, hipMemcpyDeviceToHost, C.st));
+        HIPCHK(hipMemcpyAsync(pmax_out, C.pmax, sizeof(double), hipMemcpyDeviceToHost, C.st));
+        // pre = p on the interior (GS.f90:86-94)
+        if (pre) launch_pack_interior(g, p, pre, C.st);
+        HIPCHK(hipGetLastError());
+        TRY(sync());
+        *ik_out = h.ik;
+        return 0;
+    }
+    if (mode != NTUST_SOR_REDBLACK) return fail(NTUST_E_ARG, "unsupported SOR mode %d", mode);
+
+    const size_t np = (size_t)C.sg.plane;
+    NEED(P0, "sor_p0", np, true);
+    NEED(P1, "sor_p1", np, true);
+    NEED(B0, "sor_b0", np, true);
+    NEED(B1, "sor_b1", np, true);
+    const long tmax = (long)fmin(itmax, 1.0e8);
+    NEED(slots, "sor_slots", (size_t)(tmax > 0 ? tmax : 0) + 2, false);
+    launch_rb_gather(g, C.sg, p, P0, P1, C.st);
+    launch_rb_rhs(g, C.sg, us, vs, ws, dt, B0, B1, C.st);
+    // slots[0] = 1.0 (pChangeMax_ before the first sweep), slots[1..] = 0
+    HIPCHK(hipMemsetAsync(slots, 0, ((size_t)tmax + 2) * sizeof(double), C.st));
+    const double one = 1.0;
+    HIPCHK(hipMemcpyAsync(slots, &one, sizeof one, hipMemcpyHostToDevice, C.st));
+    HIPCHK(hipMemsetAsync(C.ctl, 0, sizeof(SorControl), C.st));
+    const double *rx = C.recip[0] + 1, *ry = C.recip[1] + 1, *rz = C.recip[2] + 1;
+    SorControl h{};
+    long t = 1;
+    long batch = C.ik_hint > 16 ? C.ik_hint - 8 : 16;
+    while (t <= tmax) {
+        const long nb = batch < tmax - t + 1 ? batch : tmax - t + 1;
+        HIPCHK(hipEventRecord(C.ev0, C.st));
+        for (long b = 0; b < nb; b++)
+            launch_rb_sweep(g, C.sg, rx, ry, rz, P0, P1, B0, B1, omega, zeta, itmax,
+                            (int)(t + b), slots, C.ctl, C.st);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(C.ev1, C.st));
+        HIPCHK(hipMemcpyAsync(&h, C.ctl, sizeof h, hipMemcpyDeviceToHost, C.st));
+        TRY(sync());
+        if (!h.stop && h.ik == t + nb - 1) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, C.ev0, C.ev1));
+            C.prof_ms += ms;
+            C.prof_launches += 2 * nb;
+        }
+        if (h.stop) break;
+        t += nb;
+        batch = 16;
+    }
+    C.ik_hint = h.ik;
+    HIPCHK(hipMemcpyAsync(pmax_out, slots + h.ik, sizeof(double), hipMemcpyDeviceToHost, C.st));
+    launch_rb_scatter(g, C.sg, P0, P1, p, pre, C.st);
+    HIPCHK(hipGetLastError());
+    TRY(sync());
+    *ik_out = h.ik;
+    return 0;
+}
+
+}  // namespace
+
+// =====================================================================
+// C ABI
+// =====================================================================
+
+extern "C" {
+
+const char *ntust_last_error(void) { return g_err; }
+
+int ntust_finalize(void)
+{
+    // every release is attempted; the first failure is reported
+    hipError_t first = hipSuccess;
+    auto keep = [&first](hipError_t e) { if (first == hipSuccess) first = e; };
+    if (C.st) keep(hipStreamSynchronize(C.st));
+    for (auto &b : C.bufs) keep(hipFree(b.ptr));
+    C.bufs.clear();
+    for (int m = 0; m < 6; m++)
+        if (C.metric[m]) keep(hipFree(C.metric[m]));
+    for (int m = 0; m < 3; m++)
+        if (C.recip[m]) keep(hipFree(C.recip[m]));
+    if (C.ctl) keep(hipFree(C.ctl));
+    if (C.pmax) keep(hipFree(C.pmax));
+    if (C.ev0) keep(hipEventDestroy(C.ev0));
+    if (C.ev1) keep(hipEventDestroy(C.ev1));
+    if (C.st) keep(hipStreamDestroy(C.st));
+    C = Ctx();
+    if (first != hipSuccess)
+        return fail(NTUST_E_HIP, "ntust_finalize: %s", hipGetErrorString(first));
+    return 0;
+}
+
+int ntust_init(int nx, int ny, int nz, const double *iDx, const double *Dxs,
+               const double *iDy, const double *Dys, const double *iDz, const double *Dzs)
+{
+    if (nx < 2 || ny < 2 || nz < 2)
+        return fail(NTUST_E_ARG, "grid %dx%dx%d: every extent must be >= 2", nx, ny, nz);
+    if (!iDx || !Dxs || !iDy || !Dys || !iDz || !Dzs)
+        return fail(NTUST_E_ARG, "metric array is NULL");
+    if (C.ready) ntust_finalize();
+    C.nx = nx; C.ny = ny; C.nz = nz;
+    C.nh = (long)(nx + 4) * (ny + 4) * (nz + 4);
+    C.ni = (long)nx * ny * nz;
+    HIPCHK(hipStreamCreateWithFlags(&C.st, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&C.ev0));
+    HIPCHK(hipEventCreate(&C.ev1));
+    const double *src[6] = {iDx, Dxs, iDy, Dys, iDz, Dzs};
+    const int len[6] = {nx + 4, nx + 4, ny + 4, ny + 4, nz + 4, nz + 4};
+    for (int m = 0; m < 6; m++) {
+        C.h_metric[m].assign(src[m], src[m] + len[m]);
+        HIPCHK(hipMalloc(&C.metric[m], len[m] * sizeof(double)));
+        HIPCHK(hipMemcpy(C.metric[m], src[m], len[m] * sizeof(double), hipMemcpyHostToDevice));
+    }
+    for (int a = 0; a < 3; a++) {
+        const std::vector<double> &D = C.h_metric[2 * a + 1];
+        std::vector<double> r(D.size());
+        for (size_t m = 0; m < D.size(); m++) r[m] = 1.0 / D[m];
+        HIPCHK(hipMalloc(&C.recip[a], r.size() * sizeof(double)));
+        HIPCHK(hipMemcpy(C.recip[a], r.data(), r.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMalloc(&C.ctl, sizeof(SorControl)));
+    HIPCHK(hipMalloc(&C.pmax, sizeof(double)));
+    KGrid &g = C.g;
+    g.nx = nx; g.ny = ny; g.nz = nz;
+    g.sx = nx + 4; g.sy = ny + 4; g.sxy = (long)g.sx * g.sy;
+    g.iDx = C.metric[0] + 1; g.Dxs = C.metric[1] + 1;
+    g.iDy = C.metric[2] + 1; g.Dys = C.metric[3] + 1;
+    g.iDz = C.metric[4] + 1; g.Dzs = C.metric[5] + 1;
+    SplitGrid &s = C.sg;
+    s.mp = ((nx + 3) / 2 + 7) / 8 * 8;
+    s.ry = ny + 2;
+    s.plane = (long)s.mp * s.ry * (nz + 2);
+    // the reference's cavity: no-slip everywhere, lid w = 1 on the north wall
+    for (int q = 0; q < 18; q++) C.bc.c[q] = BC_NOSLIP;
+    C.bc.c[11] = BC_DIRICHLET;
+    C.ready = true;
+    return 0;
+}
+
+int ntust_set_bc(const int *codes)
+{
+    NEED_INIT();
+    if (!codes) return fail(NTUST_E_ARG, "codes is NULL");
+    for (int q = 0; q < 18; q++) C.bc.c[q] = codes[q];
+    return 0;
+}
+
+int ntust_gradient_phi_gauss(const double *Phi, const double *prebase, double *dPhidX,
+                             const double *Dxm, const double *Dym, const double *Dzm,
+                             int istart, int iend)
+{
+    NEED_INIT();
+    if (!Phi || !dPhidX || !Dxm || !Dym || !Dzm) return fail(NTUST_E_ARG, "NULL array");
+    if (!kslab_ok(istart, iend)) return fail(NTUST_E_ARG, "istart/iend out of 1..nz");
+    NEED(phi, "grad_phi", C.nh, false);
+    NEED(d, "grad_out", 3 * C.ni, false);
+    NEED(pre, "grad_pre", (size_t)C.nx * C.ny, false);
+    // the three metric arguments are (-1:nx+2) in the reference declaration
+    // (smagorinsky.f90:51) and are read at i, j, k only
+    const int mlen = 4 + (C.nx > C.ny ? (C.nx > C.nz ? C.nx : C.nz) : (C.ny > C.nz ? C.ny : C.nz));
+    NEED(mx, "grad_mx", mlen, false);
+    NEED(my, "grad_my", mlen, false);
+    NEED(mz, "grad_mz", mlen, false);
+    TRY(h2d(phi, Phi, C.nh));
+    TRY(h2d(d, dPhidX, 3 * C.ni));
+    TRY(h2d(mx, Dxm, C.nx + 4));
+    TRY(h2d(my, Dym, C.ny + 4));
+    TRY(h2d(mz, Dzm, C.nz + 4));
+    if (prebase) TRY(h2d(pre, prebase, (size_t)C.nx * C.ny));
+    launch_gradient_flat(C.g, phi, prebase ? pre : nullptr, d, mx + 1, my + 1, mz + 1, istart,
+                         iend, C.st);
+    HIPCHK(hipGetLastError());
+    TRY(d2h(dPhidX, d, 3 * C.ni));
+    return sync();
+}
+
+int ntust_smagorinsky(const double *u, const double *v, const double *w,
+                      const double *pre_u, const double *pre_v, const double *pre_w,
+                      double Cs, int istart, int iend, double *dudx, double *dvdx,
+                      double *dwdx, double *Viseff)
+{
+    NEED_INIT();
+    if (!u || !v || !w || !Viseff) return fail(NTUST_E_ARG, "NULL array");
+    if ((dudx || dvdx || dwdx) && !(dudx && dvdx && dwdx))
+        return fail(NTUST_E_ARG, "dudx, dvdx, dwdx: give all three or none");
+    if (!kslab_ok(istart, iend)) return fail(NTUST_E_ARG, "istart/iend out of 1..nz");
+    NEED(du, "u", C.nh, true);
+    NEED(dv, "v", C.nh, true);
+    NEED(dw, "w", C.nh, true);
+    NEED(vis, "visc", C.ni, true);
+    TRY(filter_table(Cs));
+    NEED(filt, "filt", C.ni, false);
+    TRY(h2d(du, u, C.nh));
+    TRY(h2d(dv, v, C.nh));
+    TRY(h2d(dw, w, C.nh));
+    TRY(h2d(vis, Viseff, C.ni));
+    const size_t nxy = (size_t)C.nx * C.ny;
+    double *pu = nullptr, *pv = nullptr, *pw = nullptr;
+    if (pre_u || pre_v || pre_w) {
+        NEED(a, "pre_u", nxy, true);
+        NEED(b, "pre_v", nxy, true);
+        NEED(c, "pre_w", nxy, true);
+        if (pre_u) { TRY(h2d(a, pre_u, nxy)); pu = a; }
+        if (pre_v) { TRY(h2d(b, pre_v, nxy)); pv = b; }
+        if (pre_w) { TRY(h2d(c, pre_w, nxy)); pw = c; }
+    }
+    double *gu = nullptr, *gv = nullptr, *gw = nullptr;
+    if (dudx) {
+        NEED(a, "dudx", 3 * C.ni, false);
+        NEED(b, "dvdx", 3 * C.ni, false);
+        NEED(c, "dwdx", 3 * C.ni, false);
+        TRY(h2d(a, dudx, 3 * C.ni));
+        TRY(h2d(b, dvdx, 3 * C.ni));
+        TRY(h2d(c, dwdx, 3 * C.ni));
+        gu = a; gv = b; gw = c;
+    }
+    launch_smagorinsky(C.g, du, dv, dw, pu, pv, pw, filt, vis, gu, gv, gw, istart, iend, C.st);
+    HIPCHK(hipGetLastError());
+    TRY(d2h(Viseff, vis, C.ni));
+    if (dudx) {
+        TRY(d2h(dudx, gu, 3 * C.ni));
+        TRY(d2h(dvdx, gv, 3 * C.ni));
+        TRY(d2h(dwdx, gw, 3 * C.ni));
+    }
+    return sync();
+}
+
+int ntust_discretisation_quick_centre(const double *u, const double *v, const double *w,
+                                      const double *Viseff, double dt, double nu, int LES,
+                                      int istart, int iend, double *u_star, double *v_star,
+                                      double *w_star)
+{
+    NEED_INIT();
+    if (!u || !v || !w || !Viseff || !u_star || !v_star || !w_star)
+        return fail(NTUST_E_ARG, "NULL array");
+    if (!kslab_ok(istart, iend)) return fail(NTUST_E_ARG, "istart/iend out of 1..nz");
+    NEED(du, "u", C.nh, true);
+    NEED(dv, "v", C.nh, true);
+    NEED(dw, "w", C.nh, true);
+    NEED(vis, "visc", C.ni, true);
+    NEED(us, "us", C.nh, true);
+    NEED(vs, "vs", C.nh, true);
+    NEED(ws, "ws", C.nh, true);
+    TRY(h2d(du, u, C.nh));
+    TRY(h2d(dv, v, C.nh));
+    TRY(h2d(dw, w, C.nh));
+    TRY(h2d(vis, Viseff, C.ni));
+    TRY(h2d(us, u_star, C.nh));
+    TRY(h2d(vs, v_star, C.nh));
+    TRY(h2d(ws, w_star, C.nh));
+    launch_predict_quick(C.g, du, dv, dw, vis, dt, nu, LES, us, vs, ws, istart, iend, C.st);
+    HIPCHK(hipGetLastError());
+    TRY(d2h(u_star, us, C.nh));
+    TRY(d2h(v_star, vs, C.nh));
+    TRY(d2h(w_star, ws, C.nh));
+    return sync();
+}
+
+int ntust_discretisation_upwind(const double *u, const double *v, const double *w,
+                                double dt, double nu, double *u_star, double *v_star,
+                                double *w_star)
+{
+    NEED_INIT();
+    if (!u || !v || !w || !u_star || !v_star || !w_star) return fail(NTUST_E_ARG, "NULL array");
+    NEED(du, "u", C.nh, true);
+    NEED(dv, "v", C.nh, true);
+    NEED(dw, "w", C.nh, true);
+    NEED(us, "us", C.nh, true);
+    NEED(vs, "vs", C.nh, true);
+    NEED(ws, "ws", C.nh, true);
+    TRY(h2d(du, u, C.nh));
+    TRY(h2d(dv, v, C.nh));
+    TRY(h2d(dw, w, C.nh));
+    TRY(h2d(us, u_star, C.nh));
+    TRY(h2d(vs, v_star, C.nh));
+    TRY(h2d(ws, w_star, C.nh));
+    launch_predict_upwind(C.g, du, dv, dw, dt, nu, us, vs, ws, 0, C.st);
+    HIPCHK(hipGetLastError());
+    TRY(d2h(u_star, us, C.nh));
+    TRY(d2h(v_star, vs, C.nh));
+    TRY(d2h(w_star, ws, C.nh));
+    return sync();
+}
+
+int ntust_gauss_seidel(const double *u_star, const double *v_star, const double *w_star,
+                       double *p, double *pre, double dt, double omega, double zeta,
+                       double itmax, int *ik, double *pChangeMax)
+{
+    NEED_INIT();
+    if (!u_star || !v_star || !w_star || !p) return fail(NTUST_E_ARG, "NULL array");
+    NEED(us, "us", C.nh, true);
+    NEED(vs, "vs", C.nh, true);
+    NEED(ws, "ws", C.nh, true);
+    NEED(dp, "p", C.nh, true);
+    NEED(dpre, "pre", C.ni, true);
+    TRY(h2d(us, u_star, C.nh));
+    TRY(h2d(vs, v_star, C.nh));
+    TRY(h2d(ws, w_star, C.nh));
+    TRY(h2d(dp, p, C.nh));
+    int n = 0;
+    double pm = 0.0;
+    TRY(solve_pressure(us, vs, ws, dp, dpre, dt, omega, zeta, itmax, C.prm.sor, &n, &pm));
+    TRY(d2h(p, dp, C.nh));
+    TRY(sync());
+    if (pre) {
+        // pre = p on the interior (GS.f90:86-94)
+        const int nx = C.nx, ny = C.ny, nz = C.nz;
+        const long sx = nx + 4, sxy = (long)sx * (ny + 4);
+        for (int k = 1; k <= nz; k++)
+            for (int j = 1; j <= ny; j++)
+                memcpy(pre + (size_t)nx * ((j - 1) + (size_t)ny * (k - 1)),
+                       p + 2 + sx * (j + 1) + sxy * (k + 1), nx * sizeof(double));
+    }
+    if (ik) *ik = n;
+    if (pChangeMax) *pChangeMax = pm;
+    return ((double)n >= itmax && !(pm <= zeta)) ? NTUST_WARN_ITMAX : 0;
+}
+
+int ntust_calcul_new_velocity(const double *u_star, const double *v_star,
+                              const double *w_star, const double *p, const double *ETA,
+                              double u_solid, double v_solid, double w_solid, double dt,
+                              int istart, int iend, double *u1, double *v1, double *w1,
+                              double *u2, double *v2, double *w2, double *FX, double *FY,
+                              double *FZ, double *uc, double *vc, double *wc)
+{
+    NEED_INIT();
+    const double *in[5] = {u_star, v_star, w_star, p, ETA};
+    double *out[12] = {u1, v1, w1, u2, v2, w2, FX, FY, FZ, uc, vc, wc};
+    for (auto a : in) if (!a) return fail(NTUST_E_ARG, "NULL array");
+    for (auto a : out) if (!a) return fail(NTUST_E_ARG, "NULL array");
+    if (!kslab_ok(istart, iend)) return fail(NTUST_E_ARG, "istart/iend out of 1..nz");
+    static const char *hn[14] = {"us", "vs", "ws", "p", "eta", "u1", "v1", "w1",
+                                 "u2", "v2", "w2", "FX", "FY", "FZ"};
+    double *d[14];
+    for (int q = 0; q < 14; q++) {
+        d[q] = buf(hn[q], C.nh, true);
+        if (!d[q]) return fail(NTUST_E_ALLOC, "device allocation of %s failed", hn[q]);
+    }
+    NEED(duc, "uc", C.ni, true);
+    NEED(dvc, "vc", C.ni, true);
+    NEED(dwc, "wc", C.ni, true);
+    for (int q = 0; q < 5; q++) TRY(h2d(d[q], in[q], C.nh));
+    for (int q = 0; q < 9; q++) TRY(h2d(d[5 + q], out[q], C.nh));
+    launch_project(C.g, d[0], d[1], d[2], d[3], d[4], u_solid, v_solid, w_solid, dt, d[5], d[6],
+                   d[7], d[8], d[9], d[10], d[11], d[12], d[13], duc, dvc, dwc, nullptr,
+                   nullptr, nullptr, d[8], d[9], d[10], istart, iend, C.st);
+    HIPCHK(hipGetLastError());
+    for (int q = 0; q < 9; q++) TRY(d2h(out[q], d[5 + q], C.nh));
+    // uc, vc, wc are written on istart..iend only; keep the rest
+    const size_t nxy = (size_t)C.nx * C.ny;
+    const size_t off = nxy * (istart - 1), cnt = nxy * (iend - istart + 1);
+    TRY(d2h(uc + off, duc + off, cnt));
+    TRY(d2h(vc + off, dvc + off, cnt));
+    TRY(d2h(wc + off, dwc + off, cnt));
+    return sync();
+}
+
+int ntust_updating_velocity(const double *u2, const double *v2, const double *w2,
+                            int istart, int iend, double *u, double *v, double *w)
+{
+    NEED_INIT();
+    if (!u2 || !v2 || !w2 || !u || !v || !w) return fail(NTUST_E_ARG, "NULL array");
+    if (!kslab_ok(istart, iend)) return fail(NTUST_E_ARG, "istart/iend out of 1..nz");
+    NEED(a, "u2", C.nh, true);
+    NEED(b, "v2", C.nh, true);
+    NEED(c, "w2", C.nh, true);
+    NEED(x, "u", C.nh, true);
+    NEED(y, "v", C.nh, true);
+    NEED(z, "w", C.nh, true);
+    TRY(h2d(a, u2, C.nh));
+    TRY(h2d(b, v2, C.nh));
+    TRY(h2d(c, w2, C.nh));
+    TRY(h2d(x, u, C.nh));
+    TRY(h2d(y, v, C.nh));
+    TRY(h2d(z, w, C.nh));
+    launch_copy_interior(C.g, a, b, c, x, y, z, istart, iend, C.st);
+    HIPCHK(hipGetLastError());
+    TRY(d2h(u, x, C.nh));
+    TRY(d2h(v, y, C.nh));
+    TRY(d2h(w, z, C.nh));
+    return sync();
+}
+
+int ntust_virtual_force_integrator(const double *FY, const double *FZ, double lx,
+                                   double *cDrag, double *cLift)
+{
+    NEED_INIT();
+    if (!FY || !FZ || !cDrag || !cLift) return fail(NTUST_E_ARG, "NULL argument");
+    NEED(fy, "FY", C.nh, true);
+    NEED(fz, "FZ", C.nh, true);
+    NEED(fxz, "force_xz", (size_t)(C.nx + 2) * (C.ny + 2), true);
+    NEED(fyz, "force_yz", (size_t)(C.nx + 2) * (C.ny + 2), true);
+    NEED(fxy, "force_xy", (size_t)(C.nx + 2), true);
+    NEED(fyy, "force_yy", (size_t)(C.nx + 2), true);
+    NEED(coef, "force_coef", 2, true);
+    TRY(h2d(fy, FY, C.nh));
+    TRY(h2d(fz, FZ, C.nh));
+    launch_force_integral(C.g, fy, fz, lx, fxz, fyz, fxy, fyy, coef, C.st);
+    HIPCHK(hipGetLastError());
+    double h[2];
+    TRY(d2h(h, coef, 2));
+    TRY(sync());
+    *cDrag = h[0];
+    *cLift = h[1];
+    return 0;
+}
+
+int ntust_boundary_conditions(double *u, double *v, double *w, double *p, double *u1,
+                              double *v1, double *w1, double *u2, double *v2, double *w2,
+                              double *u_star, double *v_star, double *w_star)
+{
+    NEED_INIT();
+    if (!u || !v || !w) return fail(NTUST_E_ARG, "NULL velocity array");
+    NEED(du, "u", C.nh, true);
+    NEED(dv, "v", C.nh, true);
+    NEED(dw, "w", C.nh, true);
+    TRY(h2d(du, u, C.nh));
+    TRY(h2d(dv, v, C.nh));
+    TRY(h2d(dw, w, C.nh));
+    double *dp = nullptr;
+    if (p) {
+        NEED(a, "p", C.nh, true);
+        TRY(h2d(a, p, C.nh));
+        dp = a;
+    }
+    launch_boundary_conditions(C.g, C.bc, du, dv, dw, dp, C.st);
+    HIPCHK(hipGetLastError());
+    TRY(d2h(u, du, C.nh));
+    TRY(d2h(v, dv, C.nh));
+    TRY(d2h(w, dw, C.nh));
+    if (p) TRY(d2h(p, dp, C.nh));
+    TRY(sync());
+    // boundary_conditions.f90:305-324: whole-array copies (host side, the
+    // arrays are the caller's)
+    const size_t bytes = C.nh * sizeof(double);
+    double *tu[3] = {u1, u2, u_star}, *tv[3] = {v1, v2, v_star}, *tw[3] = {w1, w2, w_star};
+    for (int q = 0; q < 3; q++) {
+        if (tu[q]) memcpy(tu[q], u, bytes);
+        if (tv[q]) memcpy(tv[q], v, bytes);
+        if (tw[q]) memcpy(tw[q], w, bytes);
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------
+// device-resident stepping
+// ---------------------------------------------------------------------
+
+int ntust_configure(double dt, double nu, double Cs, int LES, int scheme, double omega,
+                    double zeta, double itmax, int sor_mode, double lx, int force_every)
+{
+    NEED_INIT();
+    if (scheme != NTUST_QUICK_CENTRE && scheme != NTUST_UPWIND)
+        return fail(NTUST_E_ARG, "unknown scheme %d", scheme);
+    if (sor_mode != NTUST_SOR_LEXICOGRAPHIC && sor_mode != NTUST_SOR_REDBLACK &&
+        sor_mode != NTUST_SOR_MIXED)
+        return fail(NTUST_E_ARG, "unknown SOR mode %d", sor_mode);
+    if (!(itmax >= 0.0)) return fail(NTUST_E_ARG, "itmax must be >= 0");
+    Params &q = C.prm;
+    q.dt = dt; q.nu = nu; q.Cs = Cs; q.les = LES; q.scheme = scheme; q.omega = omega;
+    q.zeta = zeta; q.itmax = itmax; q.sor = sor_mode; q.lx = lx; q.force_every = force_every;
+    return 0;
+}
+
+int ntust_state_upload(const double *u, const double *v, const double *w, const double *p,
+                       const double *ETA)
+{
+    NEED_INIT();
+    if (!u || !v || !w || !p || !ETA) return fail(NTUST_E_ARG, "NULL array");
+    NEED(du, "u", C.nh, true);
+    NEED(dv, "v", C.nh, true);
+    NEED(dw, "w", C.nh, true);
+    NEED(dp, "p", C.nh, true);
+    NEED(de, "eta", C.nh, true);
+    NEED(us, "us", C.nh, true);
+    NEED(vs, "vs", C.nh, true);
+    NEED(ws, "ws", C.nh, true);
+    NEED(u2, "u2", C.nh, true);
+    NEED(v2, "v2", C.nh, true);
+    NEED(w2, "w2", C.nh, true);
+    TRY(h2d(du, u, C.nh));
+    TRY(h2d(dv, v, C.nh));
+    TRY(h2d(dw, w, C.nh));
+    TRY(h2d(dp, p, C.nh));
+    TRY(h2d(de, ETA, C.nh));
+    launch_copy_shell(C.g, du, dv, dw, us, vs, ws, u2, v2, w2, C.st);
+    HIPCHK(hipGetLastError());
+    return sync();
+}
+
+int ntust_state_download(double *u, double *v, double *w, double *p, double *pre, double *uc,
+                         double *vc, double *wc)
+{
+    NEED_INIT();
+    NEED(du, "u", C.nh, true);
+    NEED(dv, "v", C.nh, true);
+    NEED(dw, "w", C.nh, true);
+    NEED(dp, "p", C.nh, true);
+    if (u) TRY(d2h(u, du, C.nh));
+    if (v) TRY(d2h(v, dv, C.nh));
+    if (w) TRY(d2h(w, dw, C.nh));
+    if (p) TRY(d2h(p, dp, C.nh));
+    if (pre) { NEED(a, "pre", C.ni, true); TRY(d2h(pre, a, C.ni)); }
+    if (uc) { NEED(a, "uc", C.ni, true); TRY(d2h(uc, a, C.ni)); }
+    if (vc) { NEED(a, "vc", C.ni, true); TRY(d2h(vc, a, C.ni)); }
+    if (wc) { NEED(a, "wc", C.ni, true); TRY(d2h(wc, a, C.ni)); }
+    return sync();
+}
+
+int ntust_step(int nsteps, int *ik, double *pchange, double *cdrag, double *clift)
+{
+    NEED_INIT();
+    if (nsteps < 0) return fail(NTUST_E_ARG, "nsteps < 0");
+    const Params &q = C.prm;
+    const KGrid &g = C.g;
+    NEED(u, "u", C.nh, true);
+    NEED(v, "v", C.nh, true);
+    NEED(w, "w", C.nh, true);
+    NEED(p, "p", C.nh, true);
+    NEED(eta, "eta", C.nh, true);
+    NEED(us, "us", C.nh, true);
+    NEED(vs, "vs", C.nh, true);
+    NEED(ws, "ws", C.nh, true);
+    NEED(u2, "u2", C.nh, true);
+    NEED(v2, "v2", C.nh, true);
+    NEED(w2, "w2", C.nh, true);
+    NEED(pre, "pre", C.ni, true);
+    NEED(uc, "uc", C.ni, true);
+    NEED(vc, "vc", C.ni, true);
+    NEED(wc, "wc", C.ni, true);
+    NEED(vis, "visc", C.ni, true);
+    double *FY = nullptr, *FZ = nullptr, *fxz = nullptr, *fyz = nullptr, *fxy = nullptr,
+           *fyy = nullptr, *coef = nullptr;
+    if (q.force_every > 0) {
+        NEED(a, "FY", C.nh, true);
+        NEED(b, "FZ", C.nh, true);
+        NEED(c, "force_xz", (size_t)(C.nx + 2) * (C.ny + 2), true);
+        NEED(d, "force_yz", (size_t)(C.nx + 2) * (C.ny + 2), true);
+        NEED(e, "force_xy", (size_t)(C.nx + 2), true);
+        NEED(f, "force_yy", (size_t)(C.nx + 2), true);
+        NEED(h, "force_coef", 2, true);
+        FY = a; FZ = b; fxz = c; fyz = d; fxy = e; fyy = f; coef = h;
+    }
+    if (q.les) TRY(filter_table(q.Cs));
+    NEED(filt, "filt", C.ni, false);
+    int warn = 0;
+    for (int s = 0; s < nsteps; s++) {
+        const bool last = (s == nsteps - 1);
+        // main.f90:209 -- the reference evaluates nu_t every step and scales
+        // it by LES; with LES = 0 the product is nu + 0 for any finite nu_t,
+        // so the evaluation is skipped.
+        if (q.les)
+            launch_smagorinsky(g, u, v, w, nullptr, nullptr, nullptr, filt, vis, nullptr,
+                               nullptr, nullptr, 1, C.nz, C.st);
+        // main.f90:213
+        if (q.scheme == NTUST_QUICK_CENTRE)
+            launch_predict_quick(g, u, v, w, q.les ? vis : nullptr, q.dt, q.nu, q.les, us, vs,
+                                 ws, 1, C.nz, C.st);
+        else
+            launch_predict_upwind(g, u, v, w, q.dt, q.nu, us, vs, ws, 1, C.st);
+        HIPCHK(hipGetLastError());
+        // main.f90:231
+        int n = 0;
+        double pm = 0.0;
+        TRY(solve_pressure(us, vs, ws, p, pre, q.dt, q.omega, q.zeta, q.itmax, q.sor, &n, &pm));
+        if ((double)n >= q.itmax && !(pm <= q.zeta)) warn = NTUST_WARN_ITMAX;
+        if (ik) ik[s] = n;
+        if (pchange) pchange[s] = pm;
+        // main.f90:247,255 -- projection, forcing and the update u = u2
+        const bool force = q.force_every > 0 && ((s + 1) % q.force_every == 0);
+        launch_project(g, us, vs, ws, p, eta, 0.0, 0.0, 0.0, q.dt, nullptr, nullptr, nullptr,
+                       nullptr, nullptr, nullptr, nullptr, force ? FY : nullptr,
+                       force ? FZ : nullptr, last ? uc : nullptr, last ? vc : nullptr,
+                       last ? wc : nullptr, u, v, w, u2, v2, w2, 1, C.nz, C.st);
+        // main.f90:251
+        if (force) {
+            launch_force_integral(g, FY, FZ, q.lx, fxz, fyz, fxy, fyy, coef, C.st);
+            double h[2];
+            HIPCHK(hipMemcpyAsync(h, coef, sizeof h, hipMemcpyDeviceToHost, C.st));
+            TRY(sync());
+            if (cdrag) cdrag[s] = h[0];
+            if (clift) clift[s] = h[1];
+        } else {
+            if (cdrag) cdrag[s] = NAN;
+            if (clift) clift[s] = NAN;
+        }
+        // main.f90:259
+        launch_boundary_conditions(g, C.bc, u, v, w, p, C.st);
+        launch_copy_shell(g, u, v, w, us, vs, ws, u2, v2, w2, C.st);
+        HIPCHK(hipGetLastError());
+    }
+    TRY(sync());
+    return warn;
+}
+
+void *ntust_stream(void) { return (void *)C.st; }
+
+int ntust_prof_reset(void)
+{
+    C.prof_ms = 0.0;
+    C.prof_launches = 0;
+    return 0;
+}
+
+int ntust_prof_read(double *ms, long long *launches)
+{
+    if (ms) *ms = C.prof_ms;
+    if (launches) *launches = C.prof_launches;
+    return 0;
+}
+
+}  // extern "C"

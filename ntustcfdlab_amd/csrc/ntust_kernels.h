@@ -1,0 +1,122 @@
+// ntust_kernels.h -- shared definitions of the gfx950 kernels.
+//
+// Storage follows the Fortran module arrays of the solver (column-major,
+// x fastest, one 64-bit word per value):
+//   halo arrays (-1:nx+2,-1:ny+2,-1:nz+2): (i,j,k) at (i+1)+sx*((j+1)+sy*(k+1))
+//   interior    (1:nx,1:ny,1:nz):          (i,j,k) at (i-1)+nx*((j-1)+ny*(k-1))
+//   metrics     (-1:n+2): the device pointers below are offset by one
+//               element, so m[i] addresses element i for i = -1 .. n+2.
+//
+// Every exact-mode kernel is compiled with -ffp-contract=off and writes its
+// expressions in the reference's operation order, so its results equal the
+// CPU restatement bit for bit.  Fast-mode kernels spell out every fused
+// multiply-add with fma().
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace ntust {
+
+struct KGrid {
+    int nx, ny, nz;
+    int sx, sy;          // halo strides: sx = nx+4, sy = ny+4
+    long sxy;            // sx*sy
+    const double *iDx, *Dxs, *iDy, *Dys, *iDz, *Dzs;   // offset by +1
+};
+
+__host__ __device__ inline long hix(const KGrid &g, int i, int j, int k)
+{
+    return (long)(i + 1) + (long)g.sx * (long)(j + 1) + g.sxy * (long)(k + 1);
+}
+
+__host__ __device__ inline long iix(const KGrid &g, int i, int j, int k)
+{
+    return (long)(i - 1) + (long)g.nx * ((long)(j - 1) + (long)g.ny * (long)(k - 1));
+}
+
+// Colour-split pressure storage of the red-black solver.  Cell (i,j,k),
+// i in 0..nx+1, j in 0..ny+1, k in 0..nz+1, has colour (i+j+k)&1 and lives
+// in plane[colour] at (i>>1) + mp*(j + (ny+2)*k).
+struct SplitGrid {
+    int mp;              // padded half-row length >= (nx+3)/2
+    int ry;              // ny+2
+    long plane;          // mp*(ny+2)*(nz+2)
+};
+
+__host__ __device__ inline long six(const SplitGrid &s, int i, int j, int k)
+{
+    return (long)(i >> 1) + (long)s.mp * ((long)j + (long)s.ry * (long)k);
+}
+
+// Boundary-condition codes, one per (face, component) in the order
+//   W_u W_v W_w  E_u E_v E_w  S_u S_v S_w  N_u N_v N_w  B_u B_v B_w  F_u F_v F_w
+enum BcCode { BC_NOSLIP = 0, BC_NEUMANN = 1, BC_DIRICHLET = 2 };
+struct BcTable { int c[18]; };
+
+// ---- launchers (one translation unit per kernel family) ----------------
+
+// LES: gradient of a flat (nx,ny,nz) view and the fused eddy viscosity
+void launch_gradient_flat(const KGrid &g, const double *phi, const double *prebase,
+                          double *dphi, const double *Dxm, const double *Dym,
+                          const double *Dzm, int k0, int k1, hipStream_t st);
+void launch_smagorinsky(const KGrid &g, const double *u, const double *v, const double *w,
+                        const double *pre_u, const double *pre_v, const double *pre_w,
+                        const double *filt, double *visc, double *dudx, double *dvdx,
+                        double *dwdx, int k0, int k1, hipStream_t st);
+
+// predictor
+void launch_predict_quick(const KGrid &g, const double *u, const double *v, const double *w,
+                          const double *visc, double dt, double nu, int les, double *us,
+                          double *vs, double *ws, int k0, int k1, hipStream_t st);
+void launch_predict_upwind(const KGrid &g, const double *u, const double *v, const double *w,
+                           double dt, double nu, double *us, double *vs, double *ws,
+                           int wall_copy, hipStream_t st);
+
+// pressure: lexicographic SOR (exact) and red-black SOR (fast)
+struct SorControl {
+    int stop;            // set by the first sweep that does not run
+    int ik;              // sweeps run
+    int pad[2];
+};
+void launch_sor_lexicographic(const KGrid &g, const double *us, const double *vs,
+                              const double *ws, double *p, double dt, double omega,
+                              double zeta, double itmax, SorControl *ctl, double *pmax,
+                              hipStream_t st);
+void launch_rb_rhs(const KGrid &g, const SplitGrid &s, const double *us, const double *vs,
+                   const double *ws, double dt, double *b0, double *b1, hipStream_t st);
+void launch_rb_gather(const KGrid &g, const SplitGrid &s, const double *p, double *p0,
+                      double *p1, hipStream_t st);
+void launch_rb_scatter(const KGrid &g, const SplitGrid &s, const double *p0, const double *p1,
+                       double *p, double *pre, hipStream_t st);
+void launch_rb_sweep(const KGrid &g, const SplitGrid &s, const double *rx, const double *ry,
+                     const double *rz, double *p0, double *p1, const double *b0,
+                     const double *b1, double omega, double zeta, double itmax, int t,
+                     double *slots, SorControl *ctl, hipStream_t st);
+
+// projection, direct forcing, centring, update
+void launch_project(const KGrid &g, const double *us, const double *vs, const double *ws,
+                    const double *p, const double *eta, double usol, double vsol,
+                    double wsol, double dt, double *u1, double *v1, double *w1,
+                    double *u2, double *v2, double *w2, double *FX, double *FY,
+                    double *FZ, double *uc, double *vc, double *wc, double *uout,
+                    double *vout, double *wout, const double *u2g, const double *v2g,
+                    const double *w2g, int k0, int k1, hipStream_t st);
+void launch_copy_interior(const KGrid &g, const double *a, const double *b, const double *c,
+                          double *x, double *y, double *z, int k0, int k1, hipStream_t st);
+void launch_pack_interior(const KGrid &g, const double *a, double *x, hipStream_t st);
+void launch_force_integral(const KGrid &g, const double *FY, const double *FZ, double lx,
+                           double *FXz, double *FYz, double *FXy, double *FYy,
+                           double *coef_out, hipStream_t st);
+
+// boundary conditions
+void launch_boundary_conditions(const KGrid &g, const BcTable &bc, double *u, double *v,
+                                double *w, double *p, hipStream_t st);
+void launch_copy_full(long n, const double *src, double *dst0, double *dst1, double *dst2,
+                      hipStream_t st);
+void launch_copy_shell(const KGrid &g, const double *u, const double *v, const double *w,
+                       double *a0, double *a1, double *a2, double *b0, double *b1,
+                       double *b2, hipStream_t st);
+
+}  // namespace ntust

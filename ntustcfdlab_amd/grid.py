@@ -1,0 +1,157 @@
+"""Host-side set-up of a run: grid metrics, boundary-condition codes, initial
+fields.  These are the driver-side steps of the reference (reading_data.f90,
+gridder.f90, initial_conditions.f90, the BC strings of variables_module.f90)
+restated for the Python host; the Fortran driver in fortran/ has its own.
+
+Arrays use the reference's Fortran layout: halo arrays are numpy arrays of
+shape (nx+4, ny+4, nz+4) in Fortran order, element (i,j,k) at
+[i+1, j+1, k+1]; interior arrays (nx, ny, nz) at [i-1, j-1, k-1]; metric
+arrays (n+4,) at [m+1].
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# BC codes in the order the C ABI expects
+FACES = ("West", "East", "South", "North", "Back", "Front")
+BC_CODE = {"no-slip": 0, "Neumann": 1, "Dirichlet": 2}
+
+
+def metrics_from_nodes(X: np.ndarray, n: int) -> tuple[np.ndarray, np.ndarray]:
+    """Cell widths iD (X(i+1)-X(i)) and centre spacings Ds ((X(i+2)-X(i))/2)
+    with the ghost rules of gridder.f90:267-327.  X holds nodes -1..n+3 at
+    X[m+1].  Returns (iD, Ds), each (n+4,) indexed [m+1], m = -1..n+2."""
+    x = lambda m: X[m + 1]  # noqa: E731
+    iD = np.zeros(n + 4)
+    Ds = np.zeros(n + 4)
+    for m in range(1, n):
+        iD[m + 1] = x(m + 1) - x(m)
+        Ds[m + 1] = (x(m + 2) - x(m)) / 2.0
+    iD[0 + 1] = iD[1 + 1]
+    iD[-1 + 1] = iD[1 + 1]
+    iD[n + 1] = x(n + 1) - x(n)
+    iD[n + 1 + 1] = iD[n + 1]
+    iD[n + 2 + 1] = iD[n + 1]
+    Ds[0 + 1] = Ds[1 + 1]
+    Ds[-1 + 1] = Ds[1 + 1]
+    Ds[n + 1] = Ds[n - 1 + 1]
+    Ds[n + 1 + 1] = Ds[n - 1 + 1]
+    Ds[n + 2 + 1] = Ds[n - 1 + 1]
+    return iD, Ds
+
+
+def uniform_nodes(n: int, length: float) -> np.ndarray:
+    """Nodes of gridder_equal (gridder.f90:235-263): X(1)=0, X(0)=-dx,
+    X(-1)=-2dx, X(i)=X(i-1)+dx accumulated, dx = lx/nx (reading_data.f90:25)."""
+    d = float(length) / n
+    X = np.zeros(n + 5)
+    X[1 + 1] = 0.0
+    X[0 + 1] = X[1 + 1] - d
+    X[-1 + 1] = X[0 + 1] - d
+    for m in range(2, n + 4):
+        X[m + 1] = X[m] + d
+    return X
+
+
+def tanh_nodes(n: int, length: float, centre: float, beta: float) -> np.ndarray:
+    """Two-sided tanh stretching clustered around `centre` (0<centre<length):
+    x(s) = c + A*tanh(beta*(s - s0))/tanh(beta) type mapping of a uniform
+    parameter s in [0,1], with the end nodes at 0 and length and uniform
+    ghost spacing copied from the end cells.  The reference has only uniform
+    and piecewise-uniform grids (gridder.f90:1-223); this generator serves
+    the stretched airfoil grids."""
+    s = np.linspace(0.0, 1.0, n + 1)
+    c = centre / length
+    # map s -> xi in [0,1] with clustering at xi = c
+    a = np.arctanh(np.tanh(beta) * (0.0 - c) / max(c, 1 - c)) / beta
+    b = np.arctanh(np.tanh(beta) * (1.0 - c) / max(c, 1 - c)) / beta
+    t = a + (b - a) * s
+    xi = c + max(c, 1 - c) * np.tanh(beta * t) / np.tanh(beta)
+    xi[0], xi[-1] = 0.0, 1.0
+    X = np.zeros(n + 5)
+    X[2:n + 3] = xi * length           # nodes 1..n+1
+    h0 = X[3] - X[2]
+    h1 = X[n + 2] - X[n + 1]
+    X[1] = X[2] - h0                   # node 0
+    X[0] = X[1] - h0                   # node -1
+    X[n + 3] = X[n + 2] + h1           # node n+2
+    X[n + 4] = X[n + 3] + h1           # node n+3
+    return X
+
+
+@dataclass
+class Grid:
+    nx: int
+    ny: int
+    nz: int
+    lx: float = 1.0
+    ly: float = 1.0
+    lz: float = 1.0
+    X: np.ndarray = field(default=None, repr=False)
+    Y: np.ndarray = field(default=None, repr=False)
+    Z: np.ndarray = field(default=None, repr=False)
+
+    def __post_init__(self):
+        if self.X is None:
+            self.X = uniform_nodes(self.nx, self.lx)
+        if self.Y is None:
+            self.Y = uniform_nodes(self.ny, self.ly)
+        if self.Z is None:
+            self.Z = uniform_nodes(self.nz, self.lz)
+        self.iDx, self.Dxs = metrics_from_nodes(self.X, self.nx)
+        self.iDy, self.Dys = metrics_from_nodes(self.Y, self.ny)
+        self.iDz, self.Dzs = metrics_from_nodes(self.Z, self.nz)
+
+    @property
+    def halo_shape(self):
+        return (self.nx + 4, self.ny + 4, self.nz + 4)
+
+    @property
+    def shape(self):
+        return (self.nx, self.ny, self.nz)
+
+    @property
+    def cells(self) -> int:
+        return self.nx * self.ny * self.nz
+
+    def halo(self) -> np.ndarray:
+        return np.zeros(self.halo_shape, order="F")
+
+    def interior(self) -> np.ndarray:
+        return np.zeros(self.shape, order="F")
+
+    def metrics(self):
+        return (self.iDx, self.Dxs, self.iDy, self.Dys, self.iDz, self.Dzs)
+
+
+def bc_codes(**over) -> np.ndarray:
+    """The 18 BC codes, default = the reference's cavity
+    (variables_module.f90:110-132: no-slip everywhere, NorthWall_w Dirichlet).
+    Override with e.g. NorthWall_w='no-slip', BackWall_w='Dirichlet'."""
+    names = {}
+    for f in FACES:
+        for c in "uvw":
+            names[f"{f}Wall_{c}"] = "no-slip"
+    names["NorthWall_w"] = "Dirichlet"
+    for k, v in over.items():
+        if k not in names:
+            raise KeyError(k)
+        names[k] = v
+    return np.array([BC_CODE.get(names[f"{f}Wall_{c}"], 99) for f in FACES for c in "uvw"],
+                    dtype=np.int32)
+
+
+def noise_ic(grid: Grid, amp: float, seed: int):
+    """u, v, w = seeded U(-amp, amp) on the interior faces 1..n of each
+    axis (a rest state plus noise); ghosts zero.  Apply the boundary
+    conditions afterwards.  numpy's generator is used (fast, seeded); the
+    values are data, fixed by the seed."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(3):
+        a = grid.halo()
+        a[2:grid.nx + 2, 2:grid.ny + 2, 2:grid.nz + 2] = rng.uniform(-amp, amp, grid.shape)
+        out.append(a)
+    return out

@@ -1,0 +1,215 @@
+"""Python host mirror of the reference's hot-path interface, over the C ABI
+of libntust_hip.so (include/ntust.h).
+
+`ProjectionSolver` exposes one method per reference subroutine, with the
+reference's name and argument meaning, operating in place on numpy arrays in
+the reference's Fortran layout (see grid.py) -- the same contract the
+Fortran shims in fortran/hotpath.f90 give the Fortran driver.  The
+device-resident stepping (configure / upload / step / download) keeps the
+state in HBM across steps.
+
+There is no fallback: if libntust_hip.so is missing or cannot be loaded,
+constructing a solver raises.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+from pathlib import Path
+
+import numpy as np
+
+from .grid import Grid, bc_codes
+
+LIB = Path(__file__).resolve().parent / "libntust_hip.so"
+
+SOR_LEXICOGRAPHIC, SOR_REDBLACK, SOR_MIXED = 0, 1, 2
+QUICK_CENTRE, UPWIND = 0, 1
+WARN_ITMAX = 1
+
+_D = ct.POINTER(ct.c_double)
+_I = ct.POINTER(ct.c_int)
+_lib = None
+
+
+class NtustError(RuntimeError):
+    pass
+
+
+def load_library() -> ct.CDLL:
+    """Load libntust_hip.so (built by ntustcfdlab_amd.build / __graft_entry__.build)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB.exists():
+        raise NtustError(f"{LIB} is not built; run __graft_entry__.build()")
+    lib = ct.CDLL(str(LIB))
+    d, i, vp = ct.c_double, ct.c_int, ct.c_void_p
+    sig = {
+        "ntust_init": [i, i, i, _D, _D, _D, _D, _D, _D],
+        "ntust_finalize": [],
+        "ntust_set_bc": [_I],
+        "ntust_gradient_phi_gauss": [_D, _D, _D, _D, _D, _D, i, i],
+        "ntust_smagorinsky": [_D, _D, _D, _D, _D, _D, d, i, i, _D, _D, _D, _D],
+        "ntust_discretisation_quick_centre": [_D, _D, _D, _D, d, d, i, i, i, _D, _D, _D],
+        "ntust_discretisation_upwind": [_D, _D, _D, d, d, _D, _D, _D],
+        "ntust_gauss_seidel": [_D, _D, _D, _D, _D, d, d, d, d, _I, _D],
+        "ntust_calcul_new_velocity": [_D, _D, _D, _D, _D, d, d, d, d, i, i] + [_D] * 12,
+        "ntust_updating_velocity": [_D, _D, _D, i, i, _D, _D, _D],
+        "ntust_virtual_force_integrator": [_D, _D, d, _D, _D],
+        "ntust_boundary_conditions": [_D] * 13,
+        "ntust_configure": [d, d, d, i, i, d, d, d, i, d, i],
+        "ntust_state_upload": [_D] * 5,
+        "ntust_state_download": [_D] * 8,
+        "ntust_step": [i, _I, _D, _D, _D],
+        "ntust_prof_reset": [],
+        "ntust_prof_read": [_D, ct.POINTER(ct.c_longlong)],
+    }
+    for name, args in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = ct.c_int
+    lib.ntust_last_error.restype = ct.c_char_p
+    lib.ntust_last_error.argtypes = []
+    lib.ntust_stream.restype = vp
+    lib.ntust_stream.argtypes = []
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    if a.dtype != np.float64:
+        raise TypeError("arrays must be float64")
+    if not (a.flags.f_contiguous or a.ndim == 1):
+        raise TypeError("arrays must be Fortran-contiguous")
+    return a.ctypes.data_as(_D)
+
+
+def _check(rc: int) -> int:
+    if rc < 0:
+        raise NtustError(f"ntust error {rc}: {_lib.ntust_last_error().decode()}")
+    return rc
+
+
+class ProjectionSolver:
+    """One grid on the GPU.  Methods mirror the reference subroutines."""
+
+    def __init__(self, grid: Grid, bc=None):
+        self.lib = load_library()
+        self.grid = grid
+        self._m = [np.ascontiguousarray(m, dtype=np.float64) for m in grid.metrics()]
+        _check(self.lib.ntust_init(grid.nx, grid.ny, grid.nz, *[_p(m) for m in self._m]))
+        self.set_bc(bc_codes() if bc is None else bc)
+
+    def close(self):
+        _check(self.lib.ntust_finalize())
+
+    def set_bc(self, codes):
+        self.bc = np.ascontiguousarray(codes, dtype=np.int32)
+        _check(self.lib.ntust_set_bc(self.bc.ctypes.data_as(_I)))
+
+    # ---- drop-in subroutines ------------------------------------------
+
+    def GradientPhiGauss(self, Phi, dPhidX, Dxm, Dym, Dzm, istart=1, iend=None, prebase=None):
+        """smagorinsky.f90:42-87"""
+        iend = self.grid.nz if iend is None else iend
+        _check(self.lib.ntust_gradient_phi_gauss(_p(Phi), _p(prebase), _p(dPhidX), _p(Dxm),
+                                                 _p(Dym), _p(Dzm), istart, iend))
+
+    def CalculateSmagorinskyViscosity(self, u, v, w, Viseff, Cs=0.18, dudx=None, dvdx=None,
+                                      dwdx=None, istart=1, iend=None, prebase=(None,) * 3):
+        """smagorinsky.f90:1-37"""
+        iend = self.grid.nz if iend is None else iend
+        _check(self.lib.ntust_smagorinsky(_p(u), _p(v), _p(w), _p(prebase[0]), _p(prebase[1]),
+                                          _p(prebase[2]), Cs, istart, iend, _p(dudx), _p(dvdx),
+                                          _p(dwdx), _p(Viseff)))
+
+    def discretisation_QUICK_centre(self, u, v, w, Viseff, u_star, v_star, w_star, dt, nu,
+                                    LES, istart=1, iend=None):
+        """discretisation_QUICK_centre.f90:1-345"""
+        iend = self.grid.nz if iend is None else iend
+        _check(self.lib.ntust_discretisation_quick_centre(_p(u), _p(v), _p(w), _p(Viseff), dt,
+                                                          nu, int(LES), istart, iend,
+                                                          _p(u_star), _p(v_star), _p(w_star)))
+
+    def discretisation_upwind(self, u, v, w, u_star, v_star, w_star, dt, nu):
+        """discretisation_upwind.f90:1-275 (Q1 = 0)"""
+        _check(self.lib.ntust_discretisation_upwind(_p(u), _p(v), _p(w), dt, nu, _p(u_star),
+                                                    _p(v_star), _p(w_star)))
+
+    def gauss_seidel(self, u_star, v_star, w_star, p, pre, dt, omega, zeta, itmax,
+                     ordering=SOR_LEXICOGRAPHIC):
+        """GS.f90:1-106.  Returns (ik, pChangeMax, warn)."""
+        self.configure(dt=dt, nu=0.0, omega=omega, zeta=zeta, itmax=itmax, sor=ordering)
+        ik = ct.c_int()
+        pm = ct.c_double()
+        rc = _check(self.lib.ntust_gauss_seidel(_p(u_star), _p(v_star), _p(w_star), _p(p),
+                                                _p(pre), dt, omega, zeta, float(itmax),
+                                                ct.byref(ik), ct.byref(pm)))
+        return ik.value, pm.value, rc
+
+    def calcul_new_velocity(self, u_star, v_star, w_star, p, ETA, u1, v1, w1, u2, v2, w2, FX,
+                            FY, FZ, uc, vc, wc, dt, solid=(0.0, 0.0, 0.0), istart=1,
+                            iend=None):
+        """calcul_new_velocity.f90:1-184"""
+        iend = self.grid.nz if iend is None else iend
+        _check(self.lib.ntust_calcul_new_velocity(
+            _p(u_star), _p(v_star), _p(w_star), _p(p), _p(ETA), solid[0], solid[1], solid[2],
+            dt, istart, iend, _p(u1), _p(v1), _p(w1), _p(u2), _p(v2), _p(w2), _p(FX), _p(FY),
+            _p(FZ), _p(uc), _p(vc), _p(wc)))
+
+    def Updating_velocity(self, u2, v2, w2, u, v, w, istart=1, iend=None):
+        """calcul_new_velocity.f90:195-243"""
+        iend = self.grid.nz if iend is None else iend
+        _check(self.lib.ntust_updating_velocity(_p(u2), _p(v2), _p(w2), istart, iend, _p(u),
+                                                _p(v), _p(w)))
+
+    def virtualForceIntegrator(self, FY, FZ, lx):
+        """virtualForceIntegrator.f90:3-85.  Returns (cDrag, cLift)."""
+        cd, cl = ct.c_double(), ct.c_double()
+        _check(self.lib.ntust_virtual_force_integrator(_p(FY), _p(FZ), lx, ct.byref(cd),
+                                                       ct.byref(cl)))
+        return cd.value, cl.value
+
+    def boundary_conditions(self, u, v, w, p=None, u1=None, v1=None, w1=None, u2=None,
+                            v2=None, w2=None, u_star=None, v_star=None, w_star=None):
+        """boundary_conditions.f90:1-376"""
+        _check(self.lib.ntust_boundary_conditions(_p(u), _p(v), _p(w), _p(p), _p(u1), _p(v1),
+                                                  _p(w1), _p(u2), _p(v2), _p(w2), _p(u_star),
+                                                  _p(v_star), _p(w_star)))
+
+    # ---- device-resident stepping --------------------------------------
+
+    def configure(self, *, dt, nu, Cs=0.18, LES=0, scheme=QUICK_CENTRE, omega=1.5, zeta=1e-4,
+                  itmax=3000.0, sor=SOR_LEXICOGRAPHIC, lx=1.0, force_every=1):
+        _check(self.lib.ntust_configure(dt, nu, Cs, int(LES), int(scheme), omega, zeta,
+                                        float(itmax), int(sor), lx, int(force_every)))
+
+    def upload(self, u, v, w, p, ETA):
+        _check(self.lib.ntust_state_upload(_p(u), _p(v), _p(w), _p(p), _p(ETA)))
+
+    def download(self, u=None, v=None, w=None, p=None, pre=None, uc=None, vc=None, wc=None):
+        _check(self.lib.ntust_state_download(_p(u), _p(v), _p(w), _p(p), _p(pre), _p(uc),
+                                             _p(vc), _p(wc)))
+
+    def step(self, nsteps: int):
+        """Returns per-step (ik, pChangeMax, cDrag, cLift) arrays and the status."""
+        ik = np.zeros(nsteps, dtype=np.int32)
+        pm = np.zeros(nsteps)
+        cd = np.zeros(nsteps)
+        cl = np.zeros(nsteps)
+        rc = _check(self.lib.ntust_step(nsteps, ik.ctypes.data_as(_I), _p(pm), _p(cd), _p(cl)))
+        return ik, pm, cd, cl, rc
+
+    def stream(self) -> int:
+        return self.lib.ntust_stream()
+
+    def prof_reset(self):
+        self.lib.ntust_prof_reset()
+
+    def prof_read(self):
+        ms = ct.c_double()
+        n = ct.c_longlong()
+        self.lib.ntust_prof_read(ct.byref(ms), ct.byref(n))
+        return ms.value, n.value

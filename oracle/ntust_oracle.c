@@ -1,0 +1,961 @@
+/*
+ * ntust_oracle.c -- CPU restatement of the projection-method time step.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP library, never part of
+ * the product path.  Built with -O2 -ffp-contract=off (no FMA contraction,
+ * IEEE division, correctly rounded sqrt); glibc pow for the LES filter.
+ *
+ * Each function cites the reference subroutine (file:line) it restates.
+ * Expressions keep the reference's textual operation order: C and Fortran
+ * both associate * / + - left to right, and a leading unary minus that C
+ * binds tighter than * yields the same bits because IEEE rounding is sign
+ * symmetric.
+ */
+#include "ntust_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define HX(g) ((g)->nx + 4)
+#define HY(g) ((g)->ny + 4)
+#define HZ(g) ((g)->nz + 4)
+/* halo element (i,j,k), bounds -1..n+2 */
+#define H(i, j, k) ((size_t)((i) + 1) + (size_t)sx * ((size_t)((j) + 1) + (size_t)sy * (size_t)((k) + 1)))
+/* interior element (i,j,k), bounds 1..n */
+#define I3(i, j, k) ((size_t)((i) - 1) + (size_t)nx * ((size_t)((j) - 1) + (size_t)ny * (size_t)((k) - 1)))
+/* metric element m, bounds -1..n+2 */
+#define M(a, m) ((a)[(m) + 1])
+
+#define GRID_LOCALS(g)                    \
+    const int nx = (g)->nx, ny = (g)->ny, nz = (g)->nz; \
+    const int sx = nx + 4, sy = ny + 4;   \
+    (void)nz; (void)sx; (void)sy
+
+/* ------------------------------------------------------------------ */
+/* boundary_conditions.f90                                             */
+/* ------------------------------------------------------------------ */
+
+/* tangential component ghost (the mirror rules, e.g. :31-37) */
+static double bc_tangent(int code, double interior, double old)
+{
+    if (code == OR_NOSLIP) return 0.0 - interior;   /* 2*0-v(1)   */
+    if (code == OR_NEUMANN) return interior;        /* v(1)       */
+    if (code == OR_DIRICHLET) return 2.0 - interior; /* 2*1-v(1)  */
+    return old;                                     /* no match   */
+}
+
+/* wall-normal component on the wall face itself (e.g. :18-24) */
+static double bc_normal(int code, double interior, double old)
+{
+    if (code == OR_NOSLIP) return 0.0;
+    if (code == OR_NEUMANN) return interior;
+    if (code == OR_DIRICHLET) return 1.0;
+    return old;
+}
+
+void or_boundary_conditions(const or_grid *g, const int bc[18],
+                            double *u, double *v, double *w, double *p,
+                            double *u1, double *v1, double *w1,
+                            double *u2, double *v2, double *w2,
+                            double *us, double *vs, double *ws)
+{
+    GRID_LOCALS(g);
+    int i, j, k;
+
+    /* West, boundary_conditions.f90:14-57 */
+    for (k = -1; k <= nz + 2; k++)
+        for (j = -1; j <= ny + 2; j++) {
+            u[H(0, j, k)] = bc_normal(bc[0], u[H(1, j, k)], u[H(0, j, k)]);
+            u[H(-1, j, k)] = u[H(0, j, k)];
+            v[H(0, j, k)] = bc_tangent(bc[1], v[H(1, j, k)], v[H(0, j, k)]);
+            v[H(-1, j, k)] = v[H(0, j, k)];
+            w[H(0, j, k)] = bc_tangent(bc[2], w[H(1, j, k)], w[H(0, j, k)]);
+            w[H(-1, j, k)] = w[H(0, j, k)];
+        }
+    /* East, :60-105 */
+    for (k = -1; k <= nz + 2; k++)
+        for (j = -1; j <= ny + 2; j++) {
+            u[H(nx, j, k)] = bc_normal(bc[3], u[H(nx - 1, j, k)], u[H(nx, j, k)]);
+            u[H(nx + 1, j, k)] = u[H(nx, j, k)];
+            u[H(nx + 2, j, k)] = u[H(nx + 1, j, k)];
+            v[H(nx + 1, j, k)] = bc_tangent(bc[4], v[H(nx, j, k)], v[H(nx + 1, j, k)]);
+            v[H(nx + 2, j, k)] = v[H(nx + 1, j, k)];
+            w[H(nx + 1, j, k)] = bc_tangent(bc[5], w[H(nx, j, k)], w[H(nx + 1, j, k)]);
+            w[H(nx + 2, j, k)] = w[H(nx + 1, j, k)];
+        }
+    /* South, :108-153 */
+    for (k = -1; k <= nz + 2; k++)
+        for (i = -1; i <= nx + 2; i++) {
+            u[H(i, 0, k)] = bc_tangent(bc[6], u[H(i, 1, k)], u[H(i, 0, k)]);
+            u[H(i, -1, k)] = u[H(i, 0, k)];
+            v[H(i, 0, k)] = bc_normal(bc[7], v[H(i, 1, k)], v[H(i, 0, k)]);
+            v[H(i, -1, k)] = v[H(i, 0, k)];
+            w[H(i, 0, k)] = bc_tangent(bc[8], w[H(i, 1, k)], w[H(i, 0, k)]);
+            w[H(i, -1, k)] = w[H(i, 0, k)];
+        }
+    /* North, :156-202 */
+    for (k = -1; k <= nz + 2; k++)
+        for (i = -1; i <= nx + 2; i++) {
+            u[H(i, ny + 1, k)] = bc_tangent(bc[9], u[H(i, ny, k)], u[H(i, ny + 1, k)]);
+            u[H(i, ny + 2, k)] = u[H(i, ny + 1, k)];
+            v[H(i, ny, k)] = bc_normal(bc[10], v[H(i, ny - 1, k)], v[H(i, ny, k)]);
+            v[H(i, ny + 1, k)] = v[H(i, ny, k)];
+            v[H(i, ny + 2, k)] = v[H(i, ny + 1, k)];
+            w[H(i, ny + 1, k)] = bc_tangent(bc[11], w[H(i, ny, k)], w[H(i, ny + 1, k)]);
+            w[H(i, ny + 2, k)] = w[H(i, ny + 1, k)];
+        }
+    /* Back (inlet), :207-252.  The Neumann rule for v reads v(i,j,0)
+     * itself (:228), so it leaves the ghost unchanged. */
+    for (j = -1; j <= ny + 2; j++)
+        for (i = -1; i <= nx + 2; i++) {
+            u[H(i, j, 0)] = bc_tangent(bc[12], u[H(i, j, 1)], u[H(i, j, 0)]);
+            u[H(i, j, -1)] = u[H(i, j, 0)];
+            if (bc[13] == OR_NEUMANN)
+                v[H(i, j, 0)] = v[H(i, j, 0)];
+            else
+                v[H(i, j, 0)] = bc_tangent(bc[13], v[H(i, j, 1)], v[H(i, j, 0)]);
+            v[H(i, j, -1)] = v[H(i, j, 0)];
+            w[H(i, j, 0)] = bc_normal(bc[14], w[H(i, j, 1)], w[H(i, j, 0)]);
+            w[H(i, j, -1)] = w[H(i, j, 0)];
+        }
+    /* Front (outlet), :257-301 */
+    for (j = -1; j <= ny + 2; j++)
+        for (i = -1; i <= nx + 2; i++) {
+            u[H(i, j, nz + 1)] = bc_tangent(bc[15], u[H(i, j, nz)], u[H(i, j, nz + 1)]);
+            u[H(i, j, nz + 2)] = u[H(i, j, nz + 1)];
+            v[H(i, j, nz + 1)] = bc_tangent(bc[16], v[H(i, j, nz)], v[H(i, j, nz + 1)]);
+            v[H(i, j, nz + 2)] = v[H(i, j, nz + 1)];
+            w[H(i, j, nz)] = bc_normal(bc[17], w[H(i, j, nz - 1)], w[H(i, j, nz)]);
+            w[H(i, j, nz + 1)] = w[H(i, j, nz)];
+            w[H(i, j, nz + 2)] = w[H(i, j, nz + 1)];
+        }
+
+    /* whole-array copies, :305-324 */
+    {
+        size_t n = (size_t)HX(g) * HY(g) * HZ(g);
+        size_t b = n * sizeof(double);
+        if (u1) memcpy(u1, u, b);
+        if (v1) memcpy(v1, v, b);
+        if (w1) memcpy(w1, w, b);
+        if (u2) memcpy(u2, u, b);
+        if (v2) memcpy(v2, v, b);
+        if (w2) memcpy(w2, w, b);
+        if (us) memcpy(us, u, b);
+        if (vs) memcpy(vs, v, b);
+        if (ws) memcpy(ws, w, b);
+    }
+
+    /* Neumann pressure ghosts, :329-374 */
+    if (p) {
+        for (k = -1; k <= nz + 2; k++)
+            for (j = -1; j <= ny + 2; j++) {
+                p[H(0, j, k)] = p[H(1, j, k)];
+                p[H(-1, j, k)] = p[H(0, j, k)];
+            }
+        for (k = -1; k <= nz + 2; k++)
+            for (j = -1; j <= ny + 2; j++) {
+                p[H(nx + 1, j, k)] = p[H(nx, j, k)];
+                p[H(nx + 2, j, k)] = p[H(nx + 1, j, k)];
+            }
+        for (k = -1; k <= nz + 2; k++)
+            for (i = -1; i <= nx + 2; i++) {
+                p[H(i, 0, k)] = p[H(i, 1, k)];
+                p[H(i, -1, k)] = p[H(i, 0, k)];
+            }
+        for (k = -1; k <= nz + 2; k++)
+            for (i = -1; i <= nx + 2; i++) {
+                p[H(i, ny + 1, k)] = p[H(i, ny, k)];
+                p[H(i, ny + 2, k)] = p[H(i, ny + 1, k)];
+            }
+        for (j = -1; j <= ny + 2; j++)
+            for (i = -1; i <= nx + 2; i++) {
+                p[H(i, j, 0)] = p[H(i, j, 1)];
+                p[H(i, j, -1)] = p[H(i, j, 0)];
+            }
+        for (j = -1; j <= ny + 2; j++)
+            for (i = -1; i <= nx + 2; i++) {
+                p[H(i, j, nz + 1)] = p[H(i, j, nz)];
+                p[H(i, j, nz + 2)] = p[H(i, j, nz + 1)];
+            }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* smagorinsky.f90                                                      */
+/* ------------------------------------------------------------------ */
+
+/* Phi(i,j,k) of the (nx,ny,nz) view = flat element (i-1)+nx(j-1)+nx*ny(k-1)
+ * counted from the base address */
+static double flat_read(const double *Phi, const double *prebase, long nxy,
+                        long off)
+{
+    if (off >= 0) return Phi[off];
+    if (prebase) return prebase[nxy + off];
+    return 0.0;
+}
+
+void or_gradient_phi_gauss(const or_grid *g, const double *Phi,
+                           const double *prebase, double *dPhidX,
+                           const double *Dxm, const double *Dym,
+                           const double *Dzm, int kstart, int kend)
+{
+    GRID_LOCALS(g);
+    const long nxy = (long)nx * ny;
+    const size_t n3 = (size_t)nx * ny * nz;
+    for (int k = kstart; k <= kend; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                long f = (long)(i - 1) + (long)nx * (j - 1) + nxy * (k - 1);
+                double c = flat_read(Phi, prebase, nxy, f);
+                /* smagorinsky.f90:60-67 */
+                double pe = 0.5 * (flat_read(Phi, prebase, nxy, f + 1) + c);
+                double pw = 0.5 * (flat_read(Phi, prebase, nxy, f - 1) + c);
+                double pn = 0.5 * (flat_read(Phi, prebase, nxy, f + nx) + c);
+                double ps = 0.5 * (flat_read(Phi, prebase, nxy, f - nx) + c);
+                double pf = 0.5 * (flat_read(Phi, prebase, nxy, f + nxy) + c);
+                double pb = 0.5 * (flat_read(Phi, prebase, nxy, f - nxy) + c);
+                double dxm = M(Dxm, i), dym = M(Dym, j), dzm = M(Dzm, k);
+                /* :69-73 */
+                double gx = pw * (-1.0) * (dym * dzm) + pe * (1.0) * (dym * dzm);
+                double gy = ps * (-1.0) * (dxm * dzm) + pn * (1.0) * (dxm * dzm);
+                double gz = pb * (-1.0) * (dxm * dym) + pf * (1.0) * (dxm * dym);
+                /* :75-79 */
+                double fact = 1.0 / (dxm * dym * dzm);
+                size_t o = I3(i, j, k);
+                dPhidX[o] = fact * gx;
+                dPhidX[o + n3] = fact * gy;
+                dPhidX[o + 2 * n3] = fact * gz;
+            }
+}
+
+/* delta = (iDx*iDy*iDz)**(1./3.) with the single-precision exponent
+ * (smagorinsky.f90:20); (Cs*delta)**2 is (Cs*delta)*(Cs*delta) */
+void or_smag_filter_table(const or_grid *g, double Cs, double *tab)
+{
+    GRID_LOCALS(g);
+    const double third = (double)(1.0f / 3.0f);
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                double delta = pow(M(g->iDx, i) * M(g->iDy, j) * M(g->iDz, k), third);
+                double cd = Cs * delta;
+                tab[I3(i, j, k)] = cd * cd;
+            }
+}
+
+void or_smagorinsky(const or_grid *g, const double *u, const double *v,
+                    const double *w, const double *pre_u, const double *pre_v,
+                    const double *pre_w, double Cs, int kstart, int kend,
+                    double *dudx, double *dvdx, double *dwdx, double *Viseff)
+{
+    GRID_LOCALS(g);
+    const size_t n3 = (size_t)nx * ny * nz;
+    double *tu = dudx, *tv = dvdx, *tw = dwdx, *tab;
+    if (!tu) tu = malloc(3 * n3 * sizeof(double));
+    if (!tv) tv = malloc(3 * n3 * sizeof(double));
+    if (!tw) tw = malloc(3 * n3 * sizeof(double));
+    tab = malloc(n3 * sizeof(double));
+    /* smagorinsky.f90:6-8 */
+    or_gradient_phi_gauss(g, u, pre_u, tu, g->Dxs, g->iDy, g->iDz, kstart, kend);
+    or_gradient_phi_gauss(g, v, pre_v, tv, g->iDx, g->Dys, g->iDz, kstart, kend);
+    or_gradient_phi_gauss(g, w, pre_w, tw, g->iDx, g->iDy, g->Dzs, kstart, kend);
+    or_smag_filter_table(g, Cs, tab);
+    for (int k = kstart; k <= kend; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                size_t o = I3(i, j, k);
+                double ux = tu[o], uy = tu[o + n3], uz = tu[o + 2 * n3];
+                double vx = tv[o], vy = tv[o + n3], vz = tv[o + 2 * n3];
+                double wx = tw[o], wy = tw[o + n3], wz = tw[o + 2 * n3];
+                /* :14-18 */
+                double m = fabs(2.0 * (ux * ux) + 2.0 * (vy * vy) + 2.0 * (wz * wz) +
+                                (uy + vx) * (uy + vx) + (uz + wx) * (uz + wx) +
+                                (vz + wy) * (vz + wy));
+                /* :20-24 */
+                Viseff[o] = tab[o] * sqrt(m);
+            }
+    if (!dudx) free(tu);
+    if (!dvdx) free(tv);
+    if (!dwdx) free(tw);
+    free(tab);
+}
+
+/* ------------------------------------------------------------------ */
+/* discretisation_QUICK_centre.f90                                      */
+/* ------------------------------------------------------------------ */
+
+/* one QUICK face value in the reference's form
+ *   0.5*(a+b) - 0.125*h*h/den*( (x1-x2)/d1 - (x3-x4)/d2 ) */
+static inline double qface(double a, double b, double h, double den,
+                           double x1, double x2, double d1,
+                           double x3, double x4, double d2)
+{
+    return 0.5 * (a + b) - 0.125 * h * h / den * ((x1 - x2) / d1 - (x3 - x4) / d2);
+}
+
+void or_predictor_quick(const or_grid *g, const double *u, const double *v,
+                        const double *w, const double *Viseff, double dt,
+                        double nu, int LES, int kstart, int kend,
+                        double *us, double *vs, double *ws)
+{
+    GRID_LOCALS(g);
+    const double *iDx = g->iDx, *Dxs = g->Dxs, *iDy = g->iDy, *Dys = g->Dys;
+    const double *iDz = g->iDz, *Dzs = g->Dzs;
+    const double les = (double)LES;
+    int i, j, k;
+#define U(a, b, c) u[H(a, b, c)]
+#define V(a, b, c) v[H(a, b, c)]
+#define W(a, b, c) w[H(a, b, c)]
+
+    /* u_star, :7-117 */
+    for (k = kstart; k <= kend; k++)
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx; i++) {
+                double utx1 = 0.5 * (U(i + 1, j, k) + U(i, j, k));
+                double utx2 = 0.5 * (U(i - 1, j, k) + U(i, j, k));
+                double vtx1 = 0.5 * (V(i, j, k) + V(i + 1, j, k));
+                double vtx2 = 0.5 * (V(i, j - 1, k) + V(i + 1, j - 1, k));
+                double wtx1 = 0.5 * (W(i + 1, j, k) + W(i, j, k));
+                double wtx2 = 0.5 * (W(i, j, k - 1) + W(i + 1, j, k - 1));
+                double ue, uw, un, usx, uf, ub;
+                if (utx1 > 0)
+                    ue = qface(U(i, j, k), U(i + 1, j, k), M(iDx, i + 1), M(Dxs, i),
+                               U(i + 1, j, k), U(i, j, k), M(iDx, i + 1),
+                               U(i, j, k), U(i - 1, j, k), M(iDx, i));
+                else
+                    ue = qface(U(i, j, k), U(i + 1, j, k), M(iDx, i + 1), M(Dxs, i + 1),
+                               U(i + 2, j, k), U(i + 1, j, k), M(iDx, i + 2),
+                               U(i + 1, j, k), U(i, j, k), M(iDx, i + 1));
+                if (utx2 > 0)
+                    uw = qface(U(i - 1, j, k), U(i, j, k), M(iDx, i), M(Dxs, i - 1),
+                               U(i, j, k), U(i - 1, j, k), M(iDx, i),
+                               U(i - 1, j, k), U(i - 2, j, k), M(iDx, i - 1));
+                else
+                    uw = qface(U(i - 1, j, k), U(i, j, k), M(iDx, i), M(Dxs, i),
+                               U(i + 1, j, k), U(i, j, k), M(iDx, i + 1),
+                               U(i, j, k), U(i - 1, j, k), M(iDx, i));
+                if (vtx1 > 0)
+                    un = qface(U(i, j, k), U(i, j + 1, k), M(Dys, j), M(iDy, j),
+                               U(i, j + 1, k), U(i, j, k), M(Dys, j),
+                               U(i, j, k), U(i, j - 1, k), M(Dys, j - 1));
+                else
+                    un = qface(U(i, j, k), U(i, j + 1, k), M(Dys, j), M(iDy, j + 1),
+                               U(i, j + 2, k), U(i, j + 1, k), M(Dys, j + 1),
+                               U(i, j + 1, k), U(i, j, k), M(Dys, j));
+                if (vtx2 > 0)
+                    usx = qface(U(i, j - 1, k), U(i, j, k), M(Dys, j - 1), M(iDy, j - 1),
+                                U(i, j, k), U(i, j - 1, k), M(Dys, j - 1),
+                                U(i, j - 1, k), U(i, j - 2, k), M(Dys, j - 2));
+                else
+                    usx = qface(U(i, j - 1, k), U(i, j, k), M(Dys, j - 1), M(iDy, j),
+                                U(i, j + 1, k), U(i, j, k), M(Dys, j),
+                                U(i, j, k), U(i, j - 1, k), M(Dys, j - 1));
+                if (wtx1 > 0)
+                    uf = qface(U(i, j, k), U(i, j, k + 1), M(Dzs, k), M(iDz, k),
+                               U(i, j, k + 1), U(i, j, k), M(Dzs, k),
+                               U(i, j, k), U(i, j, k - 1), M(Dzs, k - 1));
+                else
+                    uf = qface(U(i, j, k), U(i, j, k + 1), M(Dzs, k), M(iDz, k + 1),
+                               U(i, j, k + 2), U(i, j, k + 1), M(Dzs, k + 1),
+                               U(i, j, k + 1), U(i, j, k), M(Dzs, k));
+                if (wtx2 > 0)
+                    ub = qface(U(i, j, k - 1), U(i, j, k), M(Dzs, k - 1), M(iDz, k - 1),
+                               U(i, j, k), U(i, j, k - 1), M(Dzs, k - 1),
+                               U(i, j, k - 1), U(i, j, k - 2), M(Dzs, k - 2));
+                else
+                    ub = qface(U(i, j, k - 1), U(i, j, k), M(Dzs, k - 1), M(iDz, k),
+                               U(i, j, k + 1), U(i, j, k), M(Dzs, k),
+                               U(i, j, k), U(i, j, k - 1), M(Dzs, k - 1));
+                double nut = Viseff[I3(i, j, k)];
+                double c = nu + nut * les;
+                double u0 = U(i, j, k);
+                /* :106-112 */
+                us[H(i, j, k)] = u0 - dt * (utx1 * ue - utx2 * uw) / M(Dxs, i)
+                                 - dt * (vtx1 * un - vtx2 * usx) / M(iDy, j)
+                                 - dt * (wtx1 * uf - wtx2 * ub) / M(iDz, k)
+                                 + c * dt * ((U(i + 1, j, k) - u0) / M(iDx, i + 1) - (u0 - U(i - 1, j, k)) / M(iDx, i)) / M(Dxs, i)
+                                 + c * dt * ((U(i, j + 1, k) - u0) / M(Dys, j) - (u0 - U(i, j - 1, k)) / M(Dys, j - 1)) / M(iDy, j)
+                                 + c * dt * ((U(i, j, k + 1) - u0) / M(Dzs, k) - (u0 - U(i, j, k - 1)) / M(Dzs, k - 1)) / M(iDz, k);
+            }
+
+    /* v_star, :123-230 */
+    for (k = kstart; k <= kend; k++)
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx; i++) {
+                double uty1 = 0.5 * (U(i, j, k) + U(i, j + 1, k));
+                double uty2 = 0.5 * (U(i - 1, j, k) + U(i - 1, j + 1, k));
+                double vty1 = 0.5 * (V(i, j, k) + V(i, j + 1, k));
+                double vty2 = 0.5 * (V(i, j - 1, k) + V(i, j, k));
+                double wty1 = 0.5 * (W(i, j, k) + W(i, j + 1, k));
+                double wty2 = 0.5 * (W(i, j, k - 1) + W(i, j + 1, k - 1));
+                double ve, vw, vn, vsx, vf, vb;
+                if (uty1 > 0)
+                    ve = qface(V(i, j, k), V(i + 1, j, k), M(Dxs, i), M(iDx, i),
+                               V(i + 1, j, k), V(i, j, k), M(Dxs, i),
+                               V(i, j, k), V(i - 1, j, k), M(Dxs, i - 1));
+                else
+                    ve = qface(V(i, j, k), V(i + 1, j, k), M(Dxs, i), M(iDx, i + 1),
+                               V(i + 2, j, k), V(i + 1, j, k), M(Dxs, i + 1),
+                               V(i + 1, j, k), V(i, j, k), M(Dxs, i));
+                if (uty2 > 0)
+                    vw = qface(V(i - 1, j, k), V(i, j, k), M(Dxs, i - 1), M(iDx, i - 1),
+                               V(i, j, k), V(i - 1, j, k), M(Dxs, i - 1),
+                               V(i - 1, j, k), V(i - 2, j, k), M(Dxs, i - 2));
+                else
+                    vw = qface(V(i - 1, j, k), V(i, j, k), M(Dxs, i - 1), M(iDx, i),
+                               V(i + 1, j, k), V(i, j, k), M(Dxs, i),
+                               V(i, j, k), V(i - 1, j, k), M(Dxs, i - 1));
+                if (vty1 > 0)
+                    vn = qface(V(i, j, k), V(i, j + 1, k), M(iDy, j + 1), M(Dys, j),
+                               V(i, j + 1, k), V(i, j, k), M(iDy, j + 1),
+                               V(i, j, k), V(i, j - 1, k), M(iDy, j));
+                else
+                    vn = qface(V(i, j, k), V(i, j + 1, k), M(iDy, j + 1), M(Dys, j + 1),
+                               V(i, j + 2, k), V(i, j + 1, k), M(iDy, j + 2),
+                               V(i, j + 1, k), V(i, j, k), M(iDy, j + 1));
+                if (vty2 > 0)
+                    vsx = qface(V(i, j - 1, k), V(i, j, k), M(iDy, j), M(Dys, j - 1),
+                                V(i, j, k), V(i, j - 1, k), M(iDy, j),
+                                V(i, j - 1, k), V(i, j - 2, k), M(iDy, j - 1));
+                else
+                    vsx = qface(V(i, j - 1, k), V(i, j, k), M(iDy, j), M(Dys, j),
+                                V(i, j + 1, k), V(i, j, k), M(iDy, j + 1),
+                                V(i, j, k), V(i, j - 1, k), M(iDy, j));
+                if (wty1 > 0)
+                    vf = qface(V(i, j, k), V(i, j, k + 1), M(Dzs, k), M(iDz, k),
+                               V(i, j, k + 1), V(i, j, k), M(Dzs, k),
+                               V(i, j, k), V(i, j, k - 1), M(Dzs, k - 1));
+                else
+                    vf = qface(V(i, j, k), V(i, j, k + 1), M(Dzs, k), M(iDz, k + 1),
+                               V(i, j, k + 2), V(i, j, k + 1), M(Dzs, k + 1),
+                               V(i, j, k + 1), V(i, j, k), M(Dzs, k));
+                if (wty2 > 0)
+                    vb = qface(V(i, j, k - 1), V(i, j, k), M(Dzs, k - 1), M(iDz, k - 1),
+                               V(i, j, k), V(i, j, k - 1), M(Dzs, k - 1),
+                               V(i, j, k - 1), V(i, j, k - 2), M(Dzs, k - 2));
+                else
+                    vb = qface(V(i, j, k - 1), V(i, j, k), M(Dzs, k - 1), M(iDz, k),
+                               V(i, j, k + 1), V(i, j, k), M(Dzs, k),
+                               V(i, j, k), V(i, j, k - 1), M(Dzs, k - 1));
+                double nut = Viseff[I3(i, j, k)];
+                double c = nu + nut * les;
+                double v0 = V(i, j, k);
+                /* :220-226 */
+                vs[H(i, j, k)] = v0 - dt * (uty1 * ve - uty2 * vw) / M(iDx, i)
+                                 - dt * (vty1 * vn - vty2 * vsx) / M(Dys, j)
+                                 - dt * (wty1 * vf - wty2 * vb) / M(iDz, k)
+                                 + c * dt * ((V(i + 1, j, k) - v0) / M(Dxs, i) - (v0 - V(i - 1, j, k)) / M(Dxs, i - 1)) / M(iDx, i)
+                                 + c * dt * ((V(i, j + 1, k) - v0) / M(iDy, j + 1) - (v0 - V(i, j - 1, k)) / M(iDy, j)) / M(Dys, j)
+                                 + c * dt * ((V(i, j, k + 1) - v0) / M(Dzs, k) - (v0 - V(i, j, k - 1)) / M(Dzs, k - 1)) / M(iDz, k);
+            }
+
+    /* w_star, :236-342 */
+    for (k = kstart; k <= kend; k++)
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx; i++) {
+                double utz1 = 0.5 * (U(i, j, k + 1) + U(i, j, k));
+                double utz2 = 0.5 * (U(i - 1, j, k + 1) + U(i - 1, j, k));
+                double vtz1 = 0.5 * (V(i, j, k) + V(i, j, k + 1));
+                double vtz2 = 0.5 * (V(i, j - 1, k + 1) + V(i, j - 1, k));
+                double wtz1 = 0.5 * (W(i, j, k + 1) + W(i, j, k));
+                double wtz2 = 0.5 * (W(i, j, k) + W(i, j, k - 1));
+                double we, ww, wn, wsx, wf, wb;
+                if (utz1 > 0)
+                    we = qface(W(i, j, k), W(i + 1, j, k), M(Dxs, i), M(iDx, i),
+                               W(i + 1, j, k), W(i, j, k), M(Dxs, i),
+                               W(i, j, k), W(i - 1, j, k), M(Dxs, i - 1));
+                else
+                    we = qface(W(i, j, k), W(i + 1, j, k), M(Dxs, i), M(iDx, i + 1),
+                               W(i + 2, j, k), W(i + 1, j, k), M(Dxs, i + 1),
+                               W(i + 1, j, k), W(i, j, k), M(Dxs, i));
+                if (utz2 > 0)
+                    ww = qface(W(i - 1, j, k), W(i, j, k), M(Dxs, i - 1), M(iDx, i - 1),
+                               W(i, j, k), W(i - 1, j, k), M(Dxs, i - 1),
+                               W(i - 1, j, k), W(i - 2, j, k), M(Dxs, i - 2));
+                else
+                    ww = qface(W(i - 1, j, k), W(i, j, k), M(Dxs, i - 1), M(iDx, i),
+                               W(i + 1, j, k), W(i, j, k), M(Dxs, i),
+                               W(i, j, k), W(i - 1, j, k), M(Dxs, i - 1));
+                if (vtz1 > 0)
+                    wn = qface(W(i, j, k), W(i, j + 1, k), M(Dys, j), M(iDy, j),
+                               W(i, j + 1, k), W(i, j, k), M(Dys, j),
+                               W(i, j, k), W(i, j - 1, k), M(Dys, j - 1));
+                else
+                    wn = qface(W(i, j, k), W(i, j + 1, k), M(Dys, j), M(iDy, j + 1),
+                               W(i, j + 2, k), W(i, j + 1, k), M(Dys, j + 1),
+                               W(i, j + 1, k), W(i, j, k), M(Dys, j));
+                if (vtz2 > 0)
+                    wsx = qface(W(i, j - 1, k), W(i, j, k), M(Dys, j - 1), M(iDy, j - 1),
+                                W(i, j, k), W(i, j - 1, k), M(Dys, j - 1),
+                                W(i, j - 1, k), W(i, j - 2, k), M(Dys, j - 2));
+                else
+                    wsx = qface(W(i, j - 1, k), W(i, j, k), M(Dys, j - 1), M(iDy, j),
+                                W(i, j + 1, k), W(i, j, k), M(Dys, j),
+                                W(i, j, k), W(i, j - 1, k), M(Dys, j - 1));
+                if (wtz1 > 0)
+                    wf = qface(W(i, j, k), W(i, j, k + 1), M(iDz, k + 1), M(Dzs, k),
+                               W(i, j, k + 1), W(i, j, k), M(iDz, k + 1),
+                               W(i, j, k), W(i, j, k - 1), M(iDz, k));
+                else
+                    wf = qface(W(i, j, k), W(i, j, k + 1), M(iDz, k + 1), M(Dzs, k + 1),
+                               W(i, j, k + 2), W(i, j, k + 1), M(iDz, k + 2),
+                               W(i, j, k + 1), W(i, j, k), M(iDz, k + 1));
+                if (wtz2 > 0)
+                    wb = qface(W(i, j, k - 1), W(i, j, k), M(iDz, k), M(Dzs, k - 1),
+                               W(i, j, k), W(i, j, k - 1), M(iDz, k),
+                               W(i, j, k - 1), W(i, j, k - 2), M(iDz, k - 1));
+                else
+                    wb = qface(W(i, j, k - 1), W(i, j, k), M(iDz, k), M(Dzs, k),
+                               W(i, j, k + 1), W(i, j, k), M(iDz, k + 1),
+                               W(i, j, k), W(i, j, k - 1), M(iDz, k));
+                double nut = Viseff[I3(i, j, k)];
+                double c = nu + nut * les;
+                double w0 = W(i, j, k);
+                /* :333-339 */
+                ws[H(i, j, k)] = w0 - dt * (utz1 * we - utz2 * ww) / M(iDx, i)
+                                 - dt * (vtz1 * wn - vtz2 * wsx) / M(iDy, j)
+                                 - dt * (wtz1 * wf - wtz2 * wb) / M(Dzs, k)
+                                 + c * dt * ((W(i + 1, j, k) - w0) / M(Dxs, i) - (w0 - W(i - 1, j, k)) / M(Dxs, i - 1)) / M(iDx, i)
+                                 + c * dt * ((W(i, j + 1, k) - w0) / M(Dys, j) - (w0 - W(i, j - 1, k)) / M(Dys, j - 1)) / M(iDy, j)
+                                 + c * dt * ((W(i, j, k + 1) - w0) / M(iDz, k + 1) - (w0 - W(i, j, k - 1)) / M(iDz, k)) / M(Dzs, k);
+            }
+}
+
+/* ------------------------------------------------------------------ */
+/* discretisation_upwind.f90 (Q1 = 0)                                   */
+/* ------------------------------------------------------------------ */
+
+void or_predictor_upwind(const or_grid *g, const double *u, const double *v,
+                         const double *w, double dt, double nu,
+                         double *us, double *vs, double *ws)
+{
+    GRID_LOCALS(g);
+    const double *iDx = g->iDx, *Dxs = g->Dxs, *iDy = g->iDy, *Dys = g->Dys;
+    const double *iDz = g->iDz, *Dzs = g->Dzs;
+    /* (nu+nut*Q1) with Q1 = 0 and nut = 0: nu + 0.0 */
+    const double c = nu + 0.0 * 0.0;
+    int i, j, k;
+
+    /* u_star, :8-93 */
+    for (k = 1; k <= nz; k++)
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx - 1; i++) {
+                double utx1 = 0.5 * (U(i, j, k) + U(i + 1, j, k));
+                double utx2 = 0.5 * (U(i - 1, j, k) + U(i, j, k));
+                double vtx1 = 0.5 * (V(i, j, k) + V(i + 1, j, k));
+                double vtx2 = 0.5 * (V(i, j - 1, k) + V(i + 1, j - 1, k));
+                double wtx1 = 0.5 * (W(i + 1, j, k) + W(i, j, k));
+                double wtx2 = 0.5 * (W(i, j, k - 1) + W(i + 1, j, k - 1));
+                double ue = (utx1 >= 0) ? U(i, j, k) : U(i + 1, j, k);
+                double uw = (utx2 >= 0) ? U(i - 1, j, k) : U(i, j, k);
+                double un = (vtx1 >= 0) ? U(i, j, k) : U(i, j + 1, k);
+                double usx = (vtx2 >= 0) ? U(i, j - 1, k) : U(i, j, k);
+                double uf = (wtx1 >= 0) ? U(i, j, k) : U(i, j, k + 1);
+                double ub = (wtx2 >= 0) ? U(i, j, k - 1) : U(i, j, k);
+                double u0 = U(i, j, k);
+                /* :83-89 */
+                us[H(i, j, k)] = u0 - dt * (utx1 * ue - utx2 * uw) / M(Dxs, i)
+                                 - dt * (vtx1 * un - vtx2 * usx) / M(iDy, j)
+                                 - dt * (wtx1 * uf - wtx2 * ub) / M(iDz, k)
+                                 + c * dt * ((U(i + 1, j, k) - u0) / M(iDx, i + 1) - (u0 - U(i - 1, j, k)) / M(iDx, i)) / M(Dxs, i)
+                                 + c * dt * ((U(i, j + 1, k) - u0) / M(Dys, j) - (u0 - U(i, j - 1, k)) / M(Dys, j - 1)) / M(iDy, j)
+                                 + c * dt * ((U(i, j, k + 1) - u0) / M(Dzs, k) - (u0 - U(i, j, k - 1)) / M(Dzs, k - 1)) / M(iDz, k);
+            }
+
+    /* v_star, :99-184 */
+    for (k = 1; k <= nz; k++)
+        for (j = 1; j <= ny - 1; j++)
+            for (i = 1; i <= nx; i++) {
+                double uty1 = 0.5 * (U(i, j, k) + U(i, j + 1, k));
+                double uty2 = 0.5 * (U(i - 1, j, k) + U(i - 1, j + 1, k));
+                double vty1 = 0.5 * (V(i, j, k) + V(i, j + 1, k));
+                double vty2 = 0.5 * (V(i, j - 1, k) + V(i, j, k));
+                double wty1 = 0.5 * (W(i, j, k) + W(i, j + 1, k));
+                double wty2 = 0.5 * (W(i, j, k - 1) + W(i, j + 1, k - 1));
+                double ve = (uty1 >= 0) ? V(i, j, k) : V(i + 1, j, k);
+                double vw = (uty2 >= 0) ? V(i - 1, j, k) : V(i, j, k);
+                double vn = (vty1 >= 0) ? V(i, j, k) : V(i, j + 1, k);
+                double vsx = (vty2 >= 0) ? V(i, j - 1, k) : V(i, j, k);
+                double vf = (wty1 >= 0) ? V(i, j, k) : V(i, j, k + 1);
+                double vb = (wty2 >= 0) ? V(i, j, k - 1) : V(i, j, k);
+                double v0 = V(i, j, k);
+                /* :167-173 */
+                vs[H(i, j, k)] = v0 - dt * (uty1 * ve - uty2 * vw) / M(iDx, i)
+                                 - dt * (vty1 * vn - vty2 * vsx) / M(Dys, j)
+                                 - dt * (wty1 * vf - wty2 * vb) / M(iDz, k)
+                                 + c * dt * ((V(i + 1, j, k) - v0) / M(Dxs, i) - (v0 - V(i - 1, j, k)) / M(Dxs, i - 1)) / M(iDx, i)
+                                 + c * dt * ((V(i, j + 1, k) - v0) / M(iDy, j + 1) - (v0 - V(i, j - 1, k)) / M(iDy, j)) / M(Dys, j)
+                                 + c * dt * ((V(i, j, k + 1) - v0) / M(Dzs, k) - (v0 - V(i, j, k - 1)) / M(Dzs, k - 1)) / M(iDz, k);
+            }
+
+    /* w_star, :188-273.  The x-face advecting v of :194 averages
+     * v(i,j,k) and v(i+1,j,k+1), as the reference writes it. */
+    for (k = 1; k <= nz - 1; k++)
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx; i++) {
+                double utz1 = 0.5 * (U(i, j, k + 1) + U(i, j, k));
+                double utz2 = 0.5 * (U(i - 1, j, k + 1) + U(i - 1, j, k));
+                double vtz1 = 0.5 * (V(i, j, k) + V(i + 1, j, k + 1));
+                double vtz2 = 0.5 * (V(i, j - 1, k + 1) + V(i, j - 1, k));
+                double wtz1 = 0.5 * (W(i, j, k + 1) + W(i, j, k));
+                double wtz2 = 0.5 * (W(i, j, k) + W(i, j, k - 1));
+                double we = (utz1 >= 0) ? W(i, j, k) : W(i + 1, j, k);
+                double ww = (utz2 >= 0) ? W(i - 1, j, k) : W(i, j, k);
+                double wn = (vtz1 >= 0) ? W(i, j, k) : W(i, j + 1, k);
+                double wsx = (vtz2 >= 0) ? W(i, j - 1, k) : W(i, j, k);
+                double wf = (wtz1 >= 0) ? W(i, j, k) : W(i, j, k + 1);
+                double wb = (wtz2 >= 0) ? W(i, j, k - 1) : W(i, j, k);
+                double w0 = W(i, j, k);
+                /* :255-261 */
+                ws[H(i, j, k)] = w0 - dt * (utz1 * we - utz2 * ww) / M(iDx, i)
+                                 - dt * (vtz1 * wn - vtz2 * wsx) / M(iDy, j)
+                                 - dt * (wtz1 * wf - wtz2 * wb) / M(Dzs, k)
+                                 + c * dt * ((W(i + 1, j, k) - w0) / M(Dxs, i) - (w0 - W(i - 1, j, k)) / M(Dxs, i - 1)) / M(iDx, i)
+                                 + c * dt * ((W(i, j + 1, k) - w0) / M(Dys, j) - (w0 - W(i, j - 1, k)) / M(Dys, j - 1)) / M(iDy, j)
+                                 + c * dt * ((W(i, j, k + 1) - w0) / M(iDz, k + 1) - (w0 - W(i, j, k - 1)) / M(iDz, k)) / M(Dzs, k);
+            }
+#undef U
+#undef V
+#undef W
+}
+
+/* ------------------------------------------------------------------ */
+/* GS.f90                                                               */
+/* ------------------------------------------------------------------ */
+
+int or_gauss_seidel(const or_grid *g, const double *us, const double *vs,
+                    const double *ws, double *p, double *pre, double dt,
+                    double omega, double zeta, double itmax, double *pmax)
+{
+    GRID_LOCALS(g);
+    const double *iDx = g->iDx, *Dxs = g->Dxs, *iDy = g->iDy, *Dys = g->Dys;
+    const double *iDz = g->iDz, *Dzs = g->Dzs;
+    int ik = 0;
+    double pChangeMax_ = 1.0;
+    /* GS.f90:11 */
+    while (pChangeMax_ > zeta && (double)ik < itmax) {
+        ik++;
+        double pChangeMax = 0.0;
+        for (int k = 1; k <= nz; k++)
+            for (int j = 1; j <= ny; j++)
+                for (int i = 1; i <= nx; i++) {
+                    double ay = M(iDy, j), az = M(iDz, k), ax = M(iDx, i);
+                    /* :39-41 */
+                    double mChange = (us[H(i, j, k)] - us[H(i - 1, j, k)]) * ay * az
+                                   + (vs[H(i, j, k)] - vs[H(i, j - 1, k)]) * ax * az
+                                   + (ws[H(i, j, k)] - ws[H(i, j, k - 1)]) * ax * ay;
+                    /* :43-51 */
+                    double pNew = (-p[H(i + 1, j, k)] * ay * az / M(Dxs, i)
+                                   - p[H(i - 1, j, k)] * ay * az / M(Dxs, i - 1)
+                                   - p[H(i, j + 1, k)] * ax * az / M(Dys, j)
+                                   - p[H(i, j - 1, k)] * ax * az / M(Dys, j - 1)
+                                   - p[H(i, j, k + 1)] * ax * ay / M(Dzs, k)
+                                   - p[H(i, j, k - 1)] * ax * ay / M(Dzs, k - 1)
+                                   + mChange / dt)
+                                / (-ay * az / M(Dxs, i) - ay * az / M(Dxs, i - 1)
+                                   - ax * az / M(Dys, j) - ax * az / M(Dys, j - 1)
+                                   - ax * ay / M(Dzs, k) - ax * ay / M(Dzs, k - 1));
+                    size_t o = H(i, j, k);
+                    /* :53-63 */
+                    double pChange = fabs(pNew - p[o]);
+                    p[o] = p[o] + (omega * (pNew - p[o]));
+                    if (pChange > pChangeMax) pChangeMax = pChange;
+                }
+        pChangeMax_ = pChangeMax; /* MPI_ALLREDUCE MAX over one rank, :70 */
+    }
+    /* :86-94 */
+    if (pre)
+        for (int k = 1; k <= nz; k++)
+            for (int j = 1; j <= ny; j++)
+                for (int i = 1; i <= nx; i++) pre[I3(i, j, k)] = p[H(i, j, k)];
+    if (pmax) *pmax = pChangeMax_;
+    return ik;
+}
+
+void or_poisson_b(const or_grid *g, const double *us, const double *vs,
+                  const double *ws, double dt, double *b)
+{
+    GRID_LOCALS(g);
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                double ay = M(g->iDy, j), az = M(g->iDz, k), ax = M(g->iDx, i);
+                double mChange = (us[H(i, j, k)] - us[H(i - 1, j, k)]) * ay * az
+                               + (vs[H(i, j, k)] - vs[H(i, j - 1, k)]) * ax * az
+                               + (ws[H(i, j, k)] - ws[H(i, j, k - 1)]) * ax * ay;
+                b[I3(i, j, k)] = mChange / dt;
+            }
+}
+
+int or_sor_redblack(const or_grid *g, const double *b, double *p,
+                    double *pre, double omega, double zeta, double itmax,
+                    double *pmax)
+{
+    GRID_LOCALS(g);
+    double *rx = malloc(sizeof(double) * (nx + 4));
+    double *ry = malloc(sizeof(double) * (ny + 4));
+    double *rz = malloc(sizeof(double) * (nz + 4));
+    for (int m = 0; m < nx + 4; m++) rx[m] = 1.0 / g->Dxs[m];
+    for (int m = 0; m < ny + 4; m++) ry[m] = 1.0 / g->Dys[m];
+    for (int m = 0; m < nz + 4; m++) rz[m] = 1.0 / g->Dzs[m];
+    int ik = 0;
+    double pChangeMax_ = 1.0;
+    while (pChangeMax_ > zeta && (double)ik < itmax) {
+        ik++;
+        double pChangeMax = 0.0;
+        for (int colour = 0; colour < 2; colour++)
+            for (int k = 1; k <= nz; k++)
+                for (int j = 1; j <= ny; j++)
+                    for (int i = 1; i <= nx; i++) {
+                        if (((i + j + k) & 1) != colour) continue;
+                        double ax = M(g->iDx, i), ay = M(g->iDy, j), az = M(g->iDz, k);
+                        double ayz = ay * az, axz = ax * az, axy = ax * ay;
+                        double aE = ayz * M(rx, i), aW = ayz * M(rx, i - 1);
+                        double aN = axz * M(ry, j), aS = axz * M(ry, j - 1);
+                        double aF = axy * M(rz, k), aB = axy * M(rz, k - 1);
+                        double s = fma(aE, p[H(i + 1, j, k)],
+                                   fma(aW, p[H(i - 1, j, k)],
+                                   fma(aN, p[H(i, j + 1, k)],
+                                   fma(aS, p[H(i, j - 1, k)],
+                                   fma(aF, p[H(i, j, k + 1)], aB * p[H(i, j, k - 1)])))));
+                        double aP = ((((aE + aW) + aN) + aS) + aF) + aB;
+                        double pNew = (s - b[I3(i, j, k)]) / aP;
+                        size_t o = H(i, j, k);
+                        double pChange = fabs(pNew - p[o]);
+                        p[o] = fma(omega, pNew - p[o], p[o]);
+                        if (pChange > pChangeMax) pChangeMax = pChange;
+                    }
+        pChangeMax_ = pChangeMax;
+    }
+    if (pre)
+        for (int k = 1; k <= nz; k++)
+            for (int j = 1; j <= ny; j++)
+                for (int i = 1; i <= nx; i++) pre[I3(i, j, k)] = p[H(i, j, k)];
+    if (pmax) *pmax = pChangeMax_;
+    free(rx);
+    free(ry);
+    free(rz);
+    return ik;
+}
+
+/* ------------------------------------------------------------------ */
+/* calcul_new_velocity.f90                                              */
+/* ------------------------------------------------------------------ */
+
+void or_calcul_new_velocity(const or_grid *g, const double *us,
+                            const double *vs, const double *ws,
+                            const double *p, const double *ETA,
+                            double u_solid, double v_solid, double w_solid,
+                            double dt, int kstart, int kend,
+                            double *u1, double *v1, double *w1,
+                            double *u2, double *v2, double *w2,
+                            double *FX, double *FY, double *FZ,
+                            double *uc, double *vc, double *wc)
+{
+    GRID_LOCALS(g);
+    int i, j, k;
+    /* :13-49 */
+    for (k = kstart; k <= kend; k++)
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx; i++) {
+                size_t o = H(i, j, k);
+                u1[o] = us[o] - dt * (p[H(i + 1, j, k)] - p[o]) / M(g->Dxs, i);
+                v1[o] = vs[o] - dt * (p[H(i, j + 1, k)] - p[o]) / M(g->Dys, j);
+                w1[o] = ws[o] - dt * (p[H(i, j, k + 1)] - p[o]) / M(g->Dzs, k);
+            }
+    /* :61-100 */
+    for (k = kstart; k <= kend; k++)
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx; i++) {
+                size_t o = H(i, j, k);
+                double e = ETA[o];
+                u2[o] = e * u_solid + (1.0 - e) * u1[o];
+                FX[o] = (u2[o] - u1[o]) / dt;
+                v2[o] = e * v_solid + (1.0 - e) * v1[o];
+                FY[o] = (v2[o] - v1[o]) / dt;
+                w2[o] = e * w_solid + (1.0 - e) * w1[o];
+                FZ[o] = (w2[o] - w1[o]) / dt;
+            }
+    /* :171-181 */
+    for (k = kstart; k <= kend; k++)
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx; i++) {
+                size_t o = I3(i, j, k);
+                uc[o] = 0.5 * (u2[H(i, j, k)] + u2[H(i - 1, j, k)]);
+                vc[o] = 0.5 * (v2[H(i, j, k)] + v2[H(i, j - 1, k)]);
+                wc[o] = 0.5 * (w2[H(i, j, k)] + w2[H(i, j, k - 1)]);
+            }
+}
+
+void or_updating_velocity(const or_grid *g, const double *u2,
+                          const double *v2, const double *w2, int kstart,
+                          int kend, double *u, double *v, double *w)
+{
+    GRID_LOCALS(g);
+    for (int k = kstart; k <= kend; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                size_t o = H(i, j, k);
+                u[o] = u2[o];
+                v[o] = v2[o];
+                w[o] = w2[o];
+            }
+}
+
+/* ------------------------------------------------------------------ */
+/* virtualForceIntegrator.f90                                           */
+/* ------------------------------------------------------------------ */
+
+void or_virtual_force_integrator(const or_grid *g, const double *FY,
+                                 const double *FZ, double lx, double *cDrag,
+                                 double *cLift)
+{
+    GRID_LOCALS(g);
+    /* FXz/FYz with one zero row on either side in j; FXy/FYy with one zero
+     * entry on either side in i (the out-of-bounds reads of :49,53,68,72) */
+    const int pj = ny + 2;
+    double *FXz = calloc((size_t)(nx + 2) * pj, sizeof(double));
+    double *FYz = calloc((size_t)(nx + 2) * pj, sizeof(double));
+    double *FXy = calloc((size_t)nx + 2, sizeof(double));
+    double *FYy = calloc((size_t)nx + 2, sizeof(double));
+#define ZI(i, j) ((size_t)(i) + (size_t)(nx + 2) * (size_t)(j))
+    const double *Dzs = g->Dzs, *Dys = g->Dys, *Dxs = g->Dxs;
+    /* z pass, :22-36 */
+    for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) {
+            double fx = 0.0, fy = 0.0;
+            for (int k = 1; k <= nz; k++) {
+                fx = fx + (FZ[H(i, j, k - 1)] * M(Dzs, k - 1) + 4.0 * FZ[H(i, j, k)] * M(Dzs, k)
+                           + FZ[H(i, j, k + 1)] * M(Dzs, k + 1)) / 6.0;
+                fy = fy + (FY[H(i, j, k - 1)] * M(Dzs, k - 1) + 4.0 * FY[H(i, j, k)] * M(Dzs, k)
+                           + FY[H(i, j, k + 1)] * M(Dzs, k + 1)) / 6.0;
+            }
+            FXz[ZI(i, j)] = fx;
+            FYz[ZI(i, j)] = fy;
+        }
+    /* y pass, :41-58 */
+    for (int i = 1; i <= nx; i++) {
+        double fx = 0.0, fy = 0.0;
+        for (int j = 1; j <= ny; j++) {
+            if (FXz[ZI(i, j)] != 0.0)
+                fx = fx + (FXz[ZI(i, j - 1)] * M(Dys, j - 1) + 4.0 * FXz[ZI(i, j)] * M(Dys, j)
+                           + FXz[ZI(i, j + 1)] * M(Dys, j + 1)) / 6.0;
+            if (FYz[ZI(i, j)] != 0.0)
+                fy = fy + (FYz[ZI(i, j - 1)] * M(Dys, j - 1) + 4.0 * FYz[ZI(i, j)] * M(Dys, j)
+                           + FYz[ZI(i, j + 1)] * M(Dys, j + 1)) / 6.0;
+        }
+        FXy[i] = fx;
+        FYy[i] = fy;
+    }
+    /* x pass, :63-75 */
+    double totalFX = 0.0, totalFY = 0.0;
+    for (int i = 1; i <= nx; i++) {
+        if (FXy[i] != 0.0)
+            totalFX = totalFX + (FXy[i - 1] * M(Dxs, i - 1) + 4.0 * FXy[i] * M(Dxs, i)
+                                 + FXy[i + 1] * M(Dxs, i + 1)) / 6.0;
+        if (FYy[i] != 0.0)
+            totalFY = totalFY + (FYy[i - 1] * M(Dxs, i - 1) + 4.0 * FYy[i] * M(Dxs, i)
+                                 + FYy[i + 1] * M(Dxs, i + 1)) / 6.0;
+    }
+    /* :80-81 */
+    *cDrag = (-2.0) * totalFX / lx;
+    *cLift = (-2.0) * totalFY / lx;
+#undef ZI
+    free(FXz);
+    free(FYz);
+    free(FXy);
+    free(FYy);
+}
+
+/* ------------------------------------------------------------------ */
+/* solver_1st_member.f90                                                */
+/* ------------------------------------------------------------------ */
+
+void or_poisson_coeffs(const or_grid *g, double *coef, int *jcoef)
+{
+    GRID_LOCALS(g);
+    const size_t nd = (size_t)nx * ny * nz;
+    const double *iDx = g->iDx, *Dxs = g->Dxs, *iDy = g->iDy, *Dys = g->Dys;
+    const double *iDz = g->iDz, *Dzs = g->Dzs;
+    size_t ik = 0;
+    /* :18-29; 1/a/b is (1/a)/b */
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++, ik++) {
+                double cE = 1.0 / M(iDx, i) / M(Dxs, i), cW = 1.0 / M(iDx, i) / M(Dxs, i - 1);
+                double cN = 1.0 / M(iDy, j) / M(Dys, j), cS = 1.0 / M(iDy, j) / M(Dys, j - 1);
+                double cF = 1.0 / M(iDz, k) / M(Dzs, k), cB = 1.0 / M(iDz, k) / M(Dzs, k - 1);
+                coef[ik] = cE + cW + cN + cS + cF + cB;
+                coef[ik + nd] = -1.0 / M(iDx, i) / M(Dxs, i);
+                coef[ik + 2 * nd] = -1.0 / M(iDx, i) / M(Dxs, i - 1);
+                coef[ik + 3 * nd] = -1.0 / M(iDy, j) / M(Dys, j);
+                coef[ik + 4 * nd] = -1.0 / M(iDy, j) / M(Dys, j - 1);
+                coef[ik + 5 * nd] = -1.0 / M(iDz, k) / M(Dzs, k);
+                coef[ik + 6 * nd] = -1.0 / M(iDz, k) / M(Dzs, k - 1);
+            }
+    /* :33-69 boundary rows */
+    ik = 0;
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++, ik++) {
+                double aa = 0.0, bb = 0.0, cc = 0.0;
+                if (i == 1) coef[ik + 2 * nd] = 0.0;
+                else if (i == nx) { aa = -1.0 / M(iDx, i) / M(Dxs, i); coef[ik + nd] = 0.0; }
+                if (j == 1) coef[ik + 4 * nd] = 0.0;
+                else if (j == ny) coef[ik + 3 * nd] = 0.0;
+                if (k == 1) coef[ik + 6 * nd] = 0.0;
+                else if (k == nz) coef[ik + 5 * nd] = 0.0;
+                coef[ik] = coef[ik] + aa + bb + cc;
+            }
+    /* :71-74 */
+    jcoef[0] = 0;
+    jcoef[1] = 1;
+    jcoef[2] = nx;
+    jcoef[3] = nx * ny;
+}
+
+/* ------------------------------------------------------------------ */
+/* main.f90:139-259, one time step                                      */
+/* ------------------------------------------------------------------ */
+
+int or_step(const or_grid *g, const or_params *prm, or_fields *f,
+            double *cd, double *cl, double *pmax)
+{
+    const int nz = g->nz;
+    /* main.f90:209 */
+    or_smagorinsky(g, f->u, f->v, f->w, NULL, NULL, NULL, prm->Cs, 1, nz,
+                   NULL, NULL, NULL, f->Viseff);
+    /* main.f90:213 */
+    if (prm->scheme == 0)
+        or_predictor_quick(g, f->u, f->v, f->w, f->Viseff, prm->dt, prm->nu,
+                           prm->LES, 1, nz, f->us, f->vs, f->ws);
+    else
+        or_predictor_upwind(g, f->u, f->v, f->w, prm->dt, prm->nu,
+                            f->us, f->vs, f->ws);
+    /* main.f90:231 */
+    int ik;
+    if (prm->sor == 0) {
+        ik = or_gauss_seidel(g, f->us, f->vs, f->ws, f->p, f->pre, prm->dt,
+                             prm->omega, prm->zeta, prm->itmax, pmax);
+    } else {
+        size_t n3 = (size_t)g->nx * g->ny * g->nz;
+        double *b = malloc(n3 * sizeof(double));
+        or_poisson_b(g, f->us, f->vs, f->ws, prm->dt, b);
+        ik = or_sor_redblack(g, b, f->p, f->pre, prm->omega, prm->zeta,
+                             prm->itmax, pmax);
+        free(b);
+    }
+    /* main.f90:247 */
+    or_calcul_new_velocity(g, f->us, f->vs, f->ws, f->p, f->ETA, 0.0, 0.0, 0.0,
+                           prm->dt, 1, nz, f->u1, f->v1, f->w1, f->u2, f->v2,
+                           f->w2, f->FX, f->FY, f->FZ, f->uc, f->vc, f->wc);
+    /* main.f90:251 */
+    or_virtual_force_integrator(g, f->FY, f->FZ, prm->lx, cd, cl);
+    /* main.f90:255 */
+    or_updating_velocity(g, f->u2, f->v2, f->w2, 1, nz, f->u, f->v, f->w);
+    /* main.f90:259 */
+    or_boundary_conditions(g, prm->bc, f->u, f->v, f->w, f->p, f->u1, f->v1,
+                           f->w1, f->u2, f->v2, f->w2, f->us, f->vs, f->ws);
+    return ik;
+}

@@ -1,0 +1,161 @@
+/*
+ * ntust_oracle.h -- CPU restatement of the projection-method time step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * library in ntustcfdlab_amd/csrc: only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg may load it.  The product never calls it.
+ *
+ * Every function restates one Fortran subroutine of HSUAN221/NTUSTCFDLAB
+ * with the same operation order (Fortran evaluates * and / left to right,
+ * sums left to right, single-precision literals are widened to double).
+ * It is built with -O2 -ffp-contract=off, so no FMA is formed and every
+ * division is the IEEE division: the HIP exact-mode kernels reproduce these
+ * results bit for bit.
+ *
+ * Storage is the reference's: column-major with x fastest.
+ *   halo arrays  (-1:nx+2,-1:ny+2,-1:nz+2)  element (i,j,k) at
+ *                (i+1) + (nx+4)*((j+1) + (ny+4)*(k+1))
+ *   interior     (1:nx,1:ny,1:nz)           element (i,j,k) at
+ *                (i-1) + nx*((j-1) + ny*(k-1))
+ *   metrics      (-1:n+2)                   element i at i+1
+ *
+ * Functions whose results no closed form, invariant or brute-force check
+ * pins are marked "parity unpinned" below; the tests list the checks that
+ * pin the others.
+ */
+#ifndef NTUST_ORACLE_H
+#define NTUST_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int nx, ny, nz;
+    const double *iDx, *Dxs, *iDy, *Dys, *iDz, *Dzs; /* each (-1:n+2) */
+} or_grid;
+
+/* boundary-condition codes, one per (face, component) in the order
+ * W_u W_v W_w  E_u E_v E_w  S_u S_v S_w  N_u N_v N_w  B_u B_v B_w  F_u F_v F_w
+ * value: 0 'no-slip', 1 'Neumann', 2 'Dirichlet', any other: no rule. */
+enum { OR_NOSLIP = 0, OR_NEUMANN = 1, OR_DIRICHLET = 2 };
+
+/* boundary_conditions.f90:1-376 -- velocity faces W,E,S,N,B,F, the
+ * whole-array copies u->u1,u2,u_star (v, w alike) and the Neumann p ghosts.
+ * Any of the nine copy targets may be NULL (copy skipped). */
+void or_boundary_conditions(const or_grid *g, const int bc[18],
+                            double *u, double *v, double *w, double *p,
+                            double *u1, double *v1, double *w1,
+                            double *u2, double *v2, double *w2,
+                            double *us, double *vs, double *ws);
+
+/* smagorinsky.f90:42-87 GradientPhiGauss.  Phi is the base address of the
+ * argument (element (-1,-1,-1) of a halo array) read through the interior
+ * shape (nx,ny,nz) by sequence association.  Reads that fall before the
+ * base (k=1 plane) come from prebase[nx*ny + off] (off<0) when prebase is
+ * given, else read as 0.0 -- the reference reads whatever memory precedes
+ * the array there (parity unpinned for those cells). */
+void or_gradient_phi_gauss(const or_grid *g, const double *Phi,
+                           const double *prebase, double *dPhidX,
+                           const double *Dxm, const double *Dym,
+                           const double *Dzm, int kstart, int kend);
+
+/* smagorinsky.f90:1-37 CalculateSmagorinskyViscosity.  Cs is the model
+ * constant (the reference widens the literal 0.18).  The filter width is
+ * (iDx*iDy*iDz)**(1./3.) with a single-precision exponent; libm pow.
+ * dudx/dvdx/dwdx (nx,ny,nz,3) may be NULL (then held internally). */
+void or_smagorinsky(const or_grid *g, const double *u, const double *v,
+                    const double *w, const double *pre_u, const double *pre_v,
+                    const double *pre_w, double Cs, int kstart, int kend,
+                    double *dudx, double *dvdx, double *dwdx, double *Viseff);
+
+/* (Cs*delta)**2 for every interior cell, the factor the LES kernel uses */
+void or_smag_filter_table(const or_grid *g, double Cs, double *tab);
+
+/* discretisation_QUICK_centre.f90:1-345 */
+void or_predictor_quick(const or_grid *g, const double *u, const double *v,
+                        const double *w, const double *Viseff, double dt,
+                        double nu, int LES, int kstart, int kend,
+                        double *us, double *vs, double *ws);
+
+/* discretisation_upwind.f90:1-275 with the undeclared Q1 taken as 0: the
+ * inline LES branch (:66-80, :150-164, :238-252) is inactive.  The loops
+ * stop one face short of the east/north/front wall. */
+void or_predictor_upwind(const or_grid *g, const double *u, const double *v,
+                         const double *w, double dt, double nu,
+                         double *us, double *vs, double *ws);
+
+/* GS.f90:1-106 in-place lexicographic SOR (k outer, j, i inner).  Returns
+ * the sweep count ik; *pmax gets the last sweep's max |pNew-p|. */
+int or_gauss_seidel(const or_grid *g, const double *us, const double *vs,
+                    const double *ws, double *p, double *pre, double dt,
+                    double omega, double zeta, double itmax, double *pmax);
+
+/* RHS of the pressure equation in the form GS.f90:39-41 builds, divided
+ * by dt as GS.f90:49 uses it: b = mChange/dt (interior array). */
+void or_poisson_b(const or_grid *g, const double *us, const double *vs,
+                  const double *ws, double dt, double *b);
+
+/* Red-black reordering of the same SOR (the HIP fast mode): red cells are
+ * (i+j+k) even.  Point update, written with explicit fma so the HIP kernel
+ * reproduces it bit for bit:
+ *   aE = (iDy*iDz)*rDxs(i), aW = (iDy*iDz)*rDxs(i-1), ... (r* = 1/D*s)
+ *   s  = fma(aE,pE, fma(aW,pW, fma(aN,pN, fma(aS,pS, fma(aF,pF, aB*pB)))))
+ *   aP = ((((aE+aW)+aN)+aS)+aF)+aB
+ *   pNew = (s - b)/aP;   pChange = |pNew-p|;   p = fma(omega, pNew-p, p)
+ * Stopping rule as GS.f90:11 with the max over both colours. */
+int or_sor_redblack(const or_grid *g, const double *b, double *p,
+                    double *pre, double omega, double zeta, double itmax,
+                    double *pmax);
+
+/* calcul_new_velocity.f90:1-184 (projection, ETA direct forcing, force
+ * density, cell-centred velocity).  Any output may not be NULL. */
+void or_calcul_new_velocity(const or_grid *g, const double *us,
+                            const double *vs, const double *ws,
+                            const double *p, const double *ETA,
+                            double u_solid, double v_solid, double w_solid,
+                            double dt, int kstart, int kend,
+                            double *u1, double *v1, double *w1,
+                            double *u2, double *v2, double *w2,
+                            double *FX, double *FY, double *FZ,
+                            double *uc, double *vc, double *wc);
+
+/* calcul_new_velocity.f90:195-243 Updating_velocity */
+void or_updating_velocity(const or_grid *g, const double *u2,
+                          const double *v2, const double *w2, int kstart,
+                          int kend, double *u, double *v, double *w);
+
+/* virtualForceIntegrator.f90:3-85.  The reference reads FXz(i,0),
+ * FXz(i,ny+1), FXy(0), FXy(nx+1) outside their declared bounds; here those
+ * reads are 0.0 (parity unpinned for them against the reference binary). */
+void or_virtual_force_integrator(const or_grid *g, const double *FY,
+                                 const double *FZ, double lx, double *cDrag,
+                                 double *cLift);
+
+/* solver_1st_member.f90:1-79: DIA matrix coef(ndim,7), offsets jcoef(4) */
+void or_poisson_coeffs(const or_grid *g, double *coef, int *jcoef);
+
+/* One time step in main.f90:139-259 order (1 rank: istart=1, iend=nz).
+ * scheme: 0 QUICK_centre, 1 upwind.  sor: 0 lexicographic (GS.f90),
+ * 1 red-black (or_sor_redblack).  The LES factor table (Cs*delta)**2 is
+ * recomputed from Cs.  Returns ik; cd/cl get the force coefficients. */
+typedef struct {
+    double *u, *v, *w, *p, *ETA;                 /* halo, state */
+    double *u1, *v1, *w1, *u2, *v2, *w2;         /* halo */
+    double *us, *vs, *ws, *FX, *FY, *FZ;         /* halo */
+    double *Viseff, *uc, *vc, *wc, *pre;         /* interior */
+} or_fields;
+
+typedef struct {
+    double dt, nu, Cs, omega, zeta, itmax, lx;
+    int LES, scheme, sor;
+    int bc[18];
+} or_params;
+
+int or_step(const or_grid *g, const or_params *prm, or_fields *f,
+            double *cd, double *cl, double *pmax);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

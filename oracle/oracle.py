@@ -1,0 +1,172 @@
+"""ctypes wrapper of liboracle (ntust_oracle.c), the CPU restatement of the
+projection-method time step.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg as the checker.  Arrays follow the reference's
+Fortran layout (see ntustcfdlab_amd/grid.py): halo arrays (nx+4,ny+4,nz+4),
+interior arrays (nx,ny,nz), Fortran order, float64.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIBPATH = HERE / "libntust_oracle.so"
+
+_D = ct.POINTER(ct.c_double)
+_I = ct.POINTER(ct.c_int)
+
+
+class OrGrid(ct.Structure):
+    _fields_ = [("nx", ct.c_int), ("ny", ct.c_int), ("nz", ct.c_int),
+                ("iDx", _D), ("Dxs", _D), ("iDy", _D), ("Dys", _D), ("iDz", _D), ("Dzs", _D)]
+
+
+class OrFields(ct.Structure):
+    _fields_ = [(n, _D) for n in ("u", "v", "w", "p", "ETA", "u1", "v1", "w1", "u2", "v2", "w2",
+                                  "us", "vs", "ws", "FX", "FY", "FZ", "Viseff", "uc", "vc",
+                                  "wc", "pre")]
+
+
+class OrParams(ct.Structure):
+    _fields_ = [("dt", ct.c_double), ("nu", ct.c_double), ("Cs", ct.c_double),
+                ("omega", ct.c_double), ("zeta", ct.c_double), ("itmax", ct.c_double),
+                ("lx", ct.c_double), ("LES", ct.c_int), ("scheme", ct.c_int),
+                ("sor", ct.c_int), ("bc", ct.c_int * 18)]
+
+
+def _load():
+    if not LIBPATH.exists():
+        subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return ct.CDLL(str(LIBPATH))
+
+
+_lib = _load()
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and (a.flags.f_contiguous or a.ndim == 1), "need float64 F-order"
+    return a.ctypes.data_as(_D)
+
+
+class Oracle:
+    """The CPU restatement bound to one grid (ntustcfdlab_amd.grid.Grid)."""
+
+    def __init__(self, grid):
+        self.grid = grid
+        self._m = [np.ascontiguousarray(m, dtype=np.float64) for m in grid.metrics()]
+        self.g = OrGrid(grid.nx, grid.ny, grid.nz, *[_p(m) for m in self._m])
+
+    # boundary_conditions.f90
+    def boundary_conditions(self, bc, u, v, w, p=None, u1=None, v1=None, w1=None, u2=None,
+                            v2=None, w2=None, us=None, vs=None, ws=None):
+        codes = (ct.c_int * 18)(*[int(c) for c in bc])
+        _lib.or_boundary_conditions(ct.byref(self.g), codes, _p(u), _p(v), _p(w), _p(p),
+                                    _p(u1), _p(v1), _p(w1), _p(u2), _p(v2), _p(w2), _p(us),
+                                    _p(vs), _p(ws))
+
+    # smagorinsky.f90:42-87
+    def gradient_phi_gauss(self, phi, dphidx, Dxm, Dym, Dzm, kstart=1, kend=None, prebase=None):
+        kend = self.grid.nz if kend is None else kend
+        _lib.or_gradient_phi_gauss(ct.byref(self.g), _p(phi), _p(prebase), _p(dphidx),
+                                   _p(Dxm), _p(Dym), _p(Dzm), kstart, kend)
+
+    # smagorinsky.f90:1-37
+    def smagorinsky(self, u, v, w, Cs, visc, dudx=None, dvdx=None, dwdx=None, kstart=1,
+                    kend=None, pre=(None, None, None)):
+        kend = self.grid.nz if kend is None else kend
+        _lib.or_smagorinsky(ct.byref(self.g), _p(u), _p(v), _p(w), _p(pre[0]), _p(pre[1]),
+                            _p(pre[2]), ct.c_double(Cs), kstart, kend, _p(dudx), _p(dvdx),
+                            _p(dwdx), _p(visc))
+
+    def filter_table(self, Cs, tab):
+        _lib.or_smag_filter_table(ct.byref(self.g), ct.c_double(Cs), _p(tab))
+
+    def predictor_quick(self, u, v, w, visc, dt, nu, LES, us, vs, ws, kstart=1, kend=None):
+        kend = self.grid.nz if kend is None else kend
+        _lib.or_predictor_quick(ct.byref(self.g), _p(u), _p(v), _p(w), _p(visc),
+                                ct.c_double(dt), ct.c_double(nu), int(LES), kstart, kend,
+                                _p(us), _p(vs), _p(ws))
+
+    def predictor_upwind(self, u, v, w, dt, nu, us, vs, ws):
+        _lib.or_predictor_upwind(ct.byref(self.g), _p(u), _p(v), _p(w), ct.c_double(dt),
+                                 ct.c_double(nu), _p(us), _p(vs), _p(ws))
+
+    def gauss_seidel(self, us, vs, ws, p, pre, dt, omega, zeta, itmax):
+        pm = ct.c_double(0.0)
+        _lib.or_gauss_seidel.restype = ct.c_int
+        ik = _lib.or_gauss_seidel(ct.byref(self.g), _p(us), _p(vs), _p(ws), _p(p), _p(pre),
+                                  ct.c_double(dt), ct.c_double(omega), ct.c_double(zeta),
+                                  ct.c_double(itmax), ct.byref(pm))
+        return ik, pm.value
+
+    def poisson_b(self, us, vs, ws, dt, b):
+        _lib.or_poisson_b(ct.byref(self.g), _p(us), _p(vs), _p(ws), ct.c_double(dt), _p(b))
+
+    def sor_redblack(self, b, p, pre, omega, zeta, itmax):
+        pm = ct.c_double(0.0)
+        ik = _lib.or_sor_redblack(ct.byref(self.g), _p(b), _p(p), _p(pre), ct.c_double(omega),
+                                  ct.c_double(zeta), ct.c_double(itmax), ct.byref(pm))
+        return ik, pm.value
+
+    def calcul_new_velocity(self, us, vs, ws, p, eta, sol, dt, u1, v1, w1, u2, v2, w2, FX, FY,
+                            FZ, uc, vc, wc, kstart=1, kend=None):
+        kend = self.grid.nz if kend is None else kend
+        _lib.or_calcul_new_velocity(ct.byref(self.g), _p(us), _p(vs), _p(ws), _p(p), _p(eta),
+                                    ct.c_double(sol[0]), ct.c_double(sol[1]),
+                                    ct.c_double(sol[2]), ct.c_double(dt), kstart, kend,
+                                    _p(u1), _p(v1), _p(w1), _p(u2), _p(v2), _p(w2), _p(FX),
+                                    _p(FY), _p(FZ), _p(uc), _p(vc), _p(wc))
+
+    def updating_velocity(self, u2, v2, w2, u, v, w, kstart=1, kend=None):
+        kend = self.grid.nz if kend is None else kend
+        _lib.or_updating_velocity(ct.byref(self.g), _p(u2), _p(v2), _p(w2), kstart, kend,
+                                  _p(u), _p(v), _p(w))
+
+    def virtual_force_integrator(self, FY, FZ, lx):
+        cd, cl = ct.c_double(), ct.c_double()
+        _lib.or_virtual_force_integrator(ct.byref(self.g), _p(FY), _p(FZ), ct.c_double(lx),
+                                         ct.byref(cd), ct.byref(cl))
+        return cd.value, cl.value
+
+    def poisson_coeffs(self):
+        n = self.grid.cells
+        coef = np.zeros((n, 7), order="F")
+        jc = (ct.c_int * 4)()
+        _lib.or_poisson_coeffs(ct.byref(self.g), _p(coef), jc)
+        return coef, list(jc)
+
+
+class OracleRun:
+    """Whole-state time stepping with or_step (main.f90:139-259 order)."""
+
+    def __init__(self, grid, *, dt, nu, Cs=0.18, omega=1.5, zeta=1e-4, itmax=3000.0, lx=1.0,
+                 LES=0, scheme=0, sor=0, bc):
+        self.o = Oracle(grid)
+        self.grid = grid
+        H = grid.halo
+        In = grid.interior
+        self.f = {n: H() for n in ("u", "v", "w", "p", "ETA", "u1", "v1", "w1", "u2", "v2", "w2",
+                                   "us", "vs", "ws", "FX", "FY", "FZ")}
+        self.f.update({n: In() for n in ("Viseff", "uc", "vc", "wc", "pre")})
+        self.prm = OrParams(dt, nu, Cs, omega, zeta, itmax, lx, LES, scheme, sor,
+                            (ct.c_int * 18)(*[int(c) for c in bc]))
+        self.bc = bc
+
+    def apply_bc(self):
+        f = self.f
+        self.o.boundary_conditions(self.bc, f["u"], f["v"], f["w"], f["p"], f["u1"], f["v1"],
+                                   f["w1"], f["u2"], f["v2"], f["w2"], f["us"], f["vs"], f["ws"])
+
+    def step(self):
+        fs = OrFields(*[_p(self.f[n]) for n, _ in OrFields._fields_])
+        cd, cl, pm = ct.c_double(), ct.c_double(), ct.c_double()
+        ik = _lib.or_step(ct.byref(self.o.g), ct.byref(self.prm), ct.byref(fs), ct.byref(cd),
+                          ct.byref(cl), ct.byref(pm))
+        return ik, pm.value, cd.value, cl.value
