@@ -24,16 +24,6 @@
 
 namespace ntust {
 
-// max over a 64-wide wavefront, then over the block; non-negative inputs
-__device__ __forceinline__ double wave_max(double v)
-{
-    for (int o = 32; o > 0; o >>= 1) {
-        double t = __shfl_xor(v, o, 64);
-        v = t > v ? t : v;
-    }
-    return v;
-}
-
 // ---------------------------------------------------------------------
 // exact lexicographic SOR
 // ---------------------------------------------------------------------
@@ -159,6 +149,15 @@ __global__ void k_rb_scatter(KGrid g, SplitGrid s, const double *__restrict__ p0
 // to the CPU red-black restatement:
 //   aE = (iDy*iDz)*rx(i) ... ; s = fma(aE,pE, fma(aW,pW, ... aB*pB))
 //   aP = aE+aW+aN+aS+aF+aB;  pNew = (s-b)/aP;  p = fma(omega, pNew-p, p)
+//
+// The maximum of sweep t is kept in SLOT_BUCKETS buckets on separate cache
+// lines (block b raises bucket b mod SLOT_BUCKETS, and only when its value is
+// larger), because thousands of atomics on one address serialise; every
+// wavefront of the next sweep reads the buckets back with one load.
+//
+// Each thread updates RB_KPT cells along k.  The loads of all of them are
+// issued together with the gate load, before anything waits, so a block
+// pays one memory latency; only the stores depend on the gate.
 template <int C>
 __global__ void __launch_bounds__(256)
 k_rb_sweep(KGrid g, SplitGrid s, const double *__restrict__ rx,
@@ -168,52 +167,70 @@ k_rb_sweep(KGrid g, SplitGrid s, const double *__restrict__ rx,
            double *slots, SorControl *ctl)
 {
     __shared__ double red[4];
-    if (ctl->stop) return;
-    const bool active = ((double)(t - 1) < itmax) && (slots[t - 1] > zeta);
-    if (!active) {
-        if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0 &&
-            threadIdx.y == 0)
-            ctl->stop = 1;
-        return;
-    }
+    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+    const int blin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    const int lane = tid & 63;
+    double gate = lane < SLOT_BUCKETS ? slots[slot_index(t - 1, lane)] : 0.0;
+    const int stopped = ctl->stop;
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
-    const int sh = (C + 1 + j + k) & 1;
-    const int i = 2 * q + 1 + sh;
+    const long pl = (long)s.mp * s.ry;
     double pcm = 0.0;
-    if (i <= g.nx && j <= g.ny) {
-        const int m = q + sh;
-        const long row = (long)s.mp * ((long)j + (long)s.ry * (long)k);
-        const long o = row + m;
-        const double ax = g.iDx[i], ay = g.iDy[j], az = g.iDz[k];
-        const double ayz = ay * az, axz = ax * az, axy = ax * ay;
-        const double aE = ayz * rx[i], aW = ayz * rx[i - 1];
-        const double aN = axz * ry[j], aS = axz * ry[j - 1];
-        const double aF = axy * rz[k], aB = axy * rz[k - 1];
-        const double pE = pc_oth[row + m + 1 - sh], pW = pc_oth[row + m - sh];
-        const double pN = pc_oth[o + s.mp], pS = pc_oth[o - s.mp];
-        const long pl = (long)s.mp * s.ry;
-        const double pF = pc_oth[o + pl], pB = pc_oth[o - pl];
-        const double sum = fma(aE, pE, fma(aW, pW, fma(aN, pN, fma(aS, pS, fma(aF, pF, aB * pB)))));
-        const double aP = ((((aE + aW) + aN) + aS) + aF) + aB;
-        const double pNew = (sum - b_own[o]) / aP;
-        const double pold = pc_own[o];
-        const double d = pNew - pold;
-        pcm = fabs(d);
-        pc_own[o] = fma(omega, d, pold);
-        if (!(pcm > 0.0)) pcm = 0.0;   // a NaN change never raises the max (GS.f90:61)
+    double nv[RB_KPT];
+    long at[RB_KPT];
+    bool in[RB_KPT];
+#pragma unroll
+    for (int r = 0; r < RB_KPT; r++) {
+        const int k = blockIdx.z * RB_KPT + r + 1;
+        const int sh = (C + 1 + j + k) & 1;
+        const int i = 2 * q + 1 + sh;
+        in[r] = i <= g.nx && j <= g.ny && k <= g.nz;
+        at[r] = 0;
+        nv[r] = 0.0;
+        if (in[r]) {
+            const int m = q + sh;
+            const long row = (long)s.mp * ((long)j + (long)s.ry * (long)k);
+            const long o = row + m;
+            const double ax = g.iDx[i], ay = g.iDy[j], az = g.iDz[k];
+            const double ayz = ay * az, axz = ax * az, axy = ax * ay;
+            const double aE = ayz * rx[i], aW = ayz * rx[i - 1];
+            const double aN = axz * ry[j], aS = axz * ry[j - 1];
+            const double aF = axy * rz[k], aB = axy * rz[k - 1];
+            const double pE = pc_oth[row + m + 1 - sh], pW = pc_oth[row + m - sh];
+            const double pN = pc_oth[o + s.mp], pS = pc_oth[o - s.mp];
+            const double pF = pc_oth[o + pl], pB = pc_oth[o - pl];
+            const double sum = fma(aE, pE, fma(aW, pW, fma(aN, pN, fma(aS, pS, fma(aF, pF, aB * pB)))));
+            const double aP = ((((aE + aW) + aN) + aS) + aF) + aB;
+            const double pNew = (sum - b_own[o]) / aP;
+            const double pold = pc_own[o];
+            const double d = pNew - pold;
+            double pc = fabs(d);
+            if (pc > pcm) pcm = pc;   // a NaN change never raises the max (GS.f90:61)
+            nv[r] = fma(omega, d, pold);
+            at[r] = o;
+        }
     }
+    // every wavefront of the block reads the same buckets: the decision is
+    // uniform over the block, so returning here cannot split a barrier
+    gate = wave_max(gate);
+    const bool active = !stopped && ((double)(t - 1) < itmax) && (gate > zeta);
+    if (!active) {
+        if (blin == 0 && tid == 0) ctl->stop = 1;
+        return;
+    }
+#pragma unroll
+    for (int r = 0; r < RB_KPT; r++)
+        if (in[r]) pc_own[at[r]] = nv[r];
     pcm = wave_max(pcm);
-    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
     if ((tid & 63) == 0) red[tid >> 6] = pcm;
     __syncthreads();
     if (tid == 0) {
         double mm = red[0];
         for (int w = 1; w < (int)((blockDim.x * blockDim.y) >> 6); w++) mm = red[w] > mm ? red[w] : mm;
-        if (mm > 0.0)
-            atomicMax((unsigned long long *)&slots[t], (unsigned long long)__double_as_longlong(mm));
-        if (C == 1 && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0) ctl->ik = t;
+        double *bk = &slots[slot_index(t, blin % SLOT_BUCKETS)];
+        if (mm > 0.0 && mm > *(volatile double *)bk)
+            atomicMax((unsigned long long *)bk, (unsigned long long)__double_as_longlong(mm));
+        if (C == 1 && blin == 0) ctl->ik = t;
     }
 }
 
@@ -244,7 +261,7 @@ void launch_rb_sweep(const KGrid &g, const SplitGrid &s, const double *rx, const
                      double *slots, SorControl *ctl, hipStream_t st)
 {
     const int nq = (g.nx + 1) / 2;
-    dim3 blk(64, 4), grd((nq + 63) / 64, (g.ny + 3) / 4, g.nz);
+    dim3 blk(64, 4), grd((nq + 63) / 64, (g.ny + 3) / 4, (g.nz + RB_KPT - 1) / RB_KPT);
     hipLaunchKernelGGL(k_rb_sweep<0>, grd, blk, 0, st, g, s, rx, ry, rz, p0, p1, b0, omega,
                        zeta, itmax, t, slots, ctl);
     hipLaunchKernelGGL(k_rb_sweep<1>, grd, blk, 0, st, g, s, rx, ry, rz, p1, p0, b1, omega,

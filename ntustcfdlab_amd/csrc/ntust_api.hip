@@ -57,6 +57,8 @@ struct Ctx {
     std::vector<double> h_metric[6];   // iDx Dxs iDy Dys iDz Dzs (host copies)
     double *metric[6] = {};            // device, base (element -1)
     double *recip[3] = {};             // 1/Dxs, 1/Dys, 1/Dzs, base
+    float *fmetric[6] = {};            // FP32 iDx, iDy, iDz, 1/Dxs, 1/Dys, 1/Dzs, base
+    long mix_segment = 64;             // FP32 sweeps between FP64 corrections
     BcTable bc{};
     Params prm;
     // named device buffers
@@ -171,19 +173,21 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
         *ik_out = h.ik;
         return 0;
     }
-    if (mode != NTUST_SOR_REDBLACK) return fail(NTUST_E_ARG, "unsupported SOR mode %d", mode);
+    if (mode != NTUST_SOR_REDBLACK && mode != NTUST_SOR_MIXED)
+        return fail(NTUST_E_ARG, "unsupported SOR mode %d", mode);
 
     const size_t np = (size_t)C.sg.plane;
     NEED(P0, "sor_p0", np, true);
     NEED(P1, "sor_p1", np, true);
     NEED(B0, "sor_b0", np, true);
     NEED(B1, "sor_b1", np, true);
-    const long tmax = (long)fmin(itmax, 1.0e8);
-    NEED(slots, "sor_slots", (size_t)(tmax > 0 ? tmax : 0) + 2, false);
+    const long tmax = (long)fmin(itmax, 1.0e8) > 0 ? (long)fmin(itmax, 1.0e8) : 0;
+    const size_t nslot = (size_t)slot_index(tmax + 2, 0);
+    NEED(slots, "sor_slots", nslot, false);
     launch_rb_gather(g, C.sg, p, P0, P1, C.st);
     launch_rb_rhs(g, C.sg, us, vs, ws, dt, B0, B1, C.st);
-    // slots[0] = 1.0 (pChangeMax_ before the first sweep), slots[1..] = 0
-    HIPCHK(hipMemsetAsync(slots, 0, ((size_t)tmax + 2) * sizeof(double), C.st));
+    // sweep 0: 1.0 (pChangeMax_ before the first sweep); sweeps 1.. : 0
+    HIPCHK(hipMemsetAsync(slots, 0, nslot * sizeof(double), C.st));
     const double one = 1.0;
     HIPCHK(hipMemcpyAsync(slots, &one, sizeof one, hipMemcpyHostToDevice, C.st));
     HIPCHK(hipMemsetAsync(C.ctl, 0, sizeof(SorControl), C.st));
@@ -191,6 +195,57 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
     SorControl h{};
     long t = 1;
     long batch = C.ik_hint > 16 ? C.ik_hint - 8 : 16;
+    if (mode == NTUST_SOR_MIXED) {
+        // FP32 correction e and FP32 residual r, colour-split like p
+        const size_t nf = np / 2 + 1;   // float planes carved from double buffers
+        NEED(e0d, "mix_e0", nf, true);
+        NEED(e1d, "mix_e1", nf, true);
+        NEED(r0d, "mix_r0", nf, true);
+        NEED(r1d, "mix_r1", nf, true);
+        float *E0 = (float *)e0d, *E1 = (float *)e1d, *R0 = (float *)r0d, *R1 = (float *)r1d;
+        const long seg = C.mix_segment;
+        const long nseg = (tmax + seg - 1) / seg + 1;
+        NEED(dd, "mix_dirty", (size_t)nseg, true);
+        int *dirty = (int *)dd;
+        HIPCHK(hipMemsetAsync(dirty, 0, nseg * sizeof(int), C.st));
+        HIPCHK(hipMemsetAsync(E0, 0, np * sizeof(float), C.st));
+        HIPCHK(hipMemsetAsync(E1, 0, np * sizeof(float), C.st));
+        launch_rb_residual(g, C.sg, rx, ry, rz, P0, P1, B0, B1, R0, R1, nullptr, C.st);
+        FMetric fm{C.fmetric[0] + 1, C.fmetric[1] + 1, C.fmetric[2] + 1,
+                   C.fmetric[3] + 1, C.fmetric[4] + 1, C.fmetric[5] + 1};
+        long c = 0;
+        long want = C.ik_hint > seg ? C.ik_hint - seg / 2 : 2 * seg;
+        while (t <= tmax) {
+            long queued = 0;
+            bool timed = false;
+            do {
+                const long n = seg < tmax - t + 1 ? seg : tmax - t + 1;
+                if (!timed) HIPCHK(hipEventRecord(C.ev0, C.st));
+                for (long b = 0; b < n; b++)
+                    launch_rb_sweep32(g, C.sg, fm, E0, E1, R0, R1, (float)omega, zeta, itmax,
+                                      (int)(t + b), slots, C.ctl, dirty + c, C.st);
+                if (!timed) HIPCHK(hipEventRecord(C.ev1, C.st));
+                launch_rb_absorb(g, C.sg, P0, P1, E0, E1, dirty + c, C.st);
+                launch_rb_residual(g, C.sg, rx, ry, rz, P0, P1, B0, B1, R0, R1, dirty + c, C.st);
+                HIPCHK(hipGetLastError());
+                if (!timed) { timed = true; batch = n; }
+                t += n;
+                queued += n;
+                c++;
+            } while (queued < want && t <= tmax);
+            HIPCHK(hipMemcpyAsync(&h, C.ctl, sizeof h, hipMemcpyDeviceToHost, C.st));
+            TRY(sync());
+            if (!h.stop && h.ik == t - 1) {
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, C.ev0, C.ev1));
+                C.prof_ms += ms;
+                C.prof_launches += 2 * batch;
+            }
+            if (h.stop) break;
+            want = 2 * seg;
+        }
+        t = tmax + 1;   // the red-black FP64 loop below does not run
+    }
     while (t <= tmax) {
         const long nb = batch < tmax - t + 1 ? batch : tmax - t + 1;
         HIPCHK(hipEventRecord(C.ev0, C.st));
@@ -212,10 +267,15 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
         batch = 16;
     }
     C.ik_hint = h.ik;
-    HIPCHK(hipMemcpyAsync(pmax_out, slots + h.ik, sizeof(double), hipMemcpyDeviceToHost, C.st));
+    double bk[SLOT_BUCKETS * SLOT_STRIDE];
+    HIPCHK(hipMemcpyAsync(bk, slots + slot_index(h.ik, 0), sizeof bk, hipMemcpyDeviceToHost,
+                          C.st));
     launch_rb_scatter(g, C.sg, P0, P1, p, pre, C.st);
     HIPCHK(hipGetLastError());
     TRY(sync());
+    double m = 0.0;
+    for (int b = 0; b < SLOT_BUCKETS; b++) m = bk[b * SLOT_STRIDE] > m ? bk[b * SLOT_STRIDE] : m;
+    *pmax_out = m;
     *ik_out = h.ik;
     return 0;
 }
@@ -242,6 +302,8 @@ int ntust_finalize(void)
         if (C.metric[m]) keep(hipFree(C.metric[m]));
     for (int m = 0; m < 3; m++)
         if (C.recip[m]) keep(hipFree(C.recip[m]));
+    for (int m = 0; m < 6; m++)
+        if (C.fmetric[m]) keep(hipFree(C.fmetric[m]));
     if (C.ctl) keep(hipFree(C.ctl));
     if (C.pmax) keep(hipFree(C.pmax));
     if (C.ev0) keep(hipEventDestroy(C.ev0));
@@ -280,6 +342,13 @@ int ntust_init(int nx, int ny, int nz, const double *iDx, const double *Dxs,
         for (size_t m = 0; m < D.size(); m++) r[m] = 1.0 / D[m];
         HIPCHK(hipMalloc(&C.recip[a], r.size() * sizeof(double)));
         HIPCHK(hipMemcpy(C.recip[a], r.data(), r.size() * sizeof(double), hipMemcpyHostToDevice));
+        // FP32 copies for the FP32 sweeps: cell width and 1/centre spacing
+        const std::vector<double> &W = C.h_metric[2 * a];
+        std::vector<float> fw(W.begin(), W.end()), fr(r.begin(), r.end());
+        HIPCHK(hipMalloc(&C.fmetric[a], fw.size() * sizeof(float)));
+        HIPCHK(hipMemcpy(C.fmetric[a], fw.data(), fw.size() * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(hipMalloc(&C.fmetric[3 + a], fr.size() * sizeof(float)));
+        HIPCHK(hipMemcpy(C.fmetric[3 + a], fr.data(), fr.size() * sizeof(float), hipMemcpyHostToDevice));
     }
     HIPCHK(hipMalloc(&C.ctl, sizeof(SorControl)));
     HIPCHK(hipMalloc(&C.pmax, sizeof(double)));

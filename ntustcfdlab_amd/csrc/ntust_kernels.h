@@ -50,6 +50,16 @@ __host__ __device__ inline long six(const SplitGrid &s, int i, int j, int k)
     return (long)(i >> 1) + (long)s.mp * ((long)j + (long)s.ry * (long)k);
 }
 
+// max over a 64-wide wavefront (non-negative, non-NaN inputs)
+__device__ __forceinline__ double wave_max(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        double t = __shfl_xor(v, o, 64);
+        v = t > v ? t : v;
+    }
+    return v;
+}
+
 // Boundary-condition codes, one per (face, component) in the order
 //   W_u W_v W_w  E_u E_v E_w  S_u S_v S_w  N_u N_v N_w  B_u B_v B_w  F_u F_v F_w
 enum BcCode { BC_NOSLIP = 0, BC_NEUMANN = 1, BC_DIRICHLET = 2 };
@@ -75,6 +85,22 @@ void launch_predict_upwind(const KGrid &g, const double *u, const double *v, con
                            int wall_copy, hipStream_t st);
 
 // pressure: lexicographic SOR (exact) and red-black SOR (fast)
+//
+// Per-sweep maxima of |pNew - p|: sweep t owns SLOT_BUCKETS buckets, each
+// on its own 128-byte line; the sweep's maximum is the largest bucket.
+// Sweep 0 is the state before the first sweep: bucket 0 holds 1.0 (GS.f90:9).
+constexpr int SLOT_BUCKETS = 32;
+constexpr int SLOT_STRIDE = 16;   // doubles
+__host__ __device__ inline long slot_index(long t, int b)
+{
+    return (t * SLOT_BUCKETS + b) * SLOT_STRIDE;
+}
+// cells per thread along k in the red-black sweep kernels
+constexpr int RB_KPT = 4;
+// FP32 copies of the metrics used by the FP32 sweeps (offset by +1 like KGrid)
+struct FMetric {
+    const float *iDx, *iDy, *iDz, *rx, *ry, *rz;
+};
 struct SorControl {
     int stop;            // set by the first sweep that does not run
     int ik;              // sweeps run
@@ -94,6 +120,16 @@ void launch_rb_sweep(const KGrid &g, const SplitGrid &s, const double *rx, const
                      const double *rz, double *p0, double *p1, const double *b0,
                      const double *b1, double omega, double zeta, double itmax, int t,
                      double *slots, SorControl *ctl, hipStream_t st);
+void launch_rb_sweep32(const KGrid &g, const SplitGrid &s, const FMetric &fm, float *e0,
+                       float *e1, const float *r0, const float *r1, float omega, double zeta,
+                       double itmax, int t, double *slots, SorControl *ctl, int *dirty,
+                       hipStream_t st);
+void launch_rb_absorb(const KGrid &g, const SplitGrid &s, double *p0, double *p1, float *e0,
+                      float *e1, const int *dirty, hipStream_t st);
+void launch_rb_residual(const KGrid &g, const SplitGrid &s, const double *rx, const double *ry,
+                        const double *rz, const double *p0, const double *p1, const double *b0,
+                        const double *b1, float *r0, float *r1, const int *dirty,
+                        hipStream_t st);
 
 // projection, direct forcing, centring, update
 void launch_project(const KGrid &g, const double *us, const double *vs, const double *ws,

@@ -1,0 +1,142 @@
+! hotpath.f90 -- drop-in replacements for the solver's hot-path subroutines.
+!
+! Each subroutine keeps the name and the empty argument list of the
+! reference subroutine it replaces (GradientPhiGauss keeps its five
+! arguments), works on the same module variables, and runs on the GPU
+! through libntust_hip.so.  Linking this file instead of smagorinsky.f90,
+! discretisation_QUICK_centre.f90 / discretisation_upwind.f90, GS.f90,
+! calcul_new_velocity.f90, virtualForceIntegrator.f90 and
+! boundary_conditions.f90 leaves the caller (main.f90:139-259) unchanged.
+! ntust_setup must run once after the gridder (main.f90:82-86) and
+! ntust_teardown before MPI_FINALIZE (main.f90:362).
+
+subroutine ntust_setup()
+   use variables
+   use ntust_hip
+   implicit none
+   call ntust_check(ntust_init(nx, ny, nz, iDx, Dxs, iDy, Dys, iDz, Dzs), 'ntust_init')
+   call ntust_push_bc()
+end subroutine ntust_setup
+
+subroutine ntust_teardown()
+   use ntust_hip
+   implicit none
+   call ntust_check(ntust_finalize(), 'ntust_finalize')
+end subroutine ntust_teardown
+
+! BC strings of variables_module.f90:110-132 -> library codes
+subroutine ntust_push_bc()
+   use variables
+   use ntust_hip
+   implicit none
+   integer(c_int) :: codes(18)
+   codes = [code(WestWall_u), code(WestWall_v), code(WestWall_w), &
+            code(EastWall_u), code(EastWall_v), code(EastWall_w), &
+            code(SouthWall_u), code(SouthWall_v), code(SouthWall_w), &
+            code(NorthWall_u), code(NorthWall_v), code(NorthWall_w), &
+            code(BackWall_u), code(BackWall_v), code(BackWall_w), &
+            code(FrontWall_u), code(FrontWall_v), code(FrontWall_w)]
+   call ntust_check(ntust_set_bc(codes), 'ntust_set_bc')
+contains
+   integer(c_int) function code(s)
+      character(len=*), intent(in) :: s
+      if (s == 'no-slip') then
+         code = 0
+      else if (s == 'Neumann') then
+         code = 1
+      else if (s == 'Dirichlet') then
+         code = 2
+      else
+         code = 99
+      end if
+   end function code
+end subroutine ntust_push_bc
+
+! smagorinsky.f90:42-87
+subroutine GradientPhiGauss(Phi, dPhidX, Dxm, Dym, Dzm)
+   use variables, only: istart, iend
+   use ntust_hip
+   implicit none
+   real*8 :: Phi(*), dPhidX(*), Dxm(*), Dym(*), Dzm(*)
+   call ntust_check(ntust_gradient_phi_gauss(Phi, c_null_ptr, dPhidX, Dxm, Dym, Dzm, &
+                                             istart, iend), 'GradientPhiGauss')
+end subroutine GradientPhiGauss
+
+! smagorinsky.f90:1-37 (the three gradients of :6-8 fused in)
+subroutine CalculateSmagorinskyViscosity()
+   use variables
+   use ntust_hip
+   implicit none
+   call ntust_check(ntust_smagorinsky(u, v, w, c_null_ptr, c_null_ptr, c_null_ptr, Cs, &
+                                      istart, iend, dudx, dvdx, dwdx, Viseff), &
+                    'CalculateSmagorinskyViscosity')
+end subroutine CalculateSmagorinskyViscosity
+
+! discretisation_QUICK_centre.f90:1-345
+subroutine discretisation_QUICK_centre()
+   use variables
+   use ntust_hip
+   implicit none
+   call ntust_check(ntust_discretisation_quick_centre(u, v, w, Viseff, dt, nu, LES, istart, &
+                                                      iend, u_star, v_star, w_star), &
+                    'discretisation_QUICK_centre')
+end subroutine discretisation_QUICK_centre
+
+! discretisation_upwind.f90:1-275
+subroutine discretisation_upwind()
+   use variables
+   use ntust_hip
+   implicit none
+   call ntust_check(ntust_discretisation_upwind(u, v, w, dt, nu, u_star, v_star, w_star), &
+                    'discretisation_upwind')
+end subroutine discretisation_upwind
+
+! GS.f90:1-106
+subroutine gauss_seidel()
+   use variables
+   use ntust_hip
+   implicit none
+   integer(c_int) :: n
+   call ntust_check(ntust_gauss_seidel(u_star, v_star, w_star, p, pre, dt, omega, zeta, &
+                                       itmax, n, pChangeMax_), 'gauss_seidel')
+   ik = n
+end subroutine gauss_seidel
+
+! calcul_new_velocity.f90:1-184
+subroutine calcul_new_velocity()
+   use variables
+   use ntust_hip
+   implicit none
+   call ntust_check(ntust_calcul_new_velocity(u_star, v_star, w_star, p, ETA, u_solid, &
+                                              v_solid, w_solid, dt, istart, iend, u1, v1, &
+                                              w1, u2, v2, w2, FX, FY, FZ, uc, vc, wc), &
+                    'calcul_new_velocity')
+end subroutine calcul_new_velocity
+
+! calcul_new_velocity.f90:195-243
+subroutine Updating_velocity()
+   use variables
+   use ntust_hip
+   implicit none
+   call ntust_check(ntust_updating_velocity(u2, v2, w2, istart, iend, u, v, w), &
+                    'Updating_velocity')
+end subroutine Updating_velocity
+
+! virtualForceIntegrator.f90:3-85
+subroutine virtualForceIntegrator()
+   use variables
+   use ntust_hip
+   implicit none
+   call ntust_check(ntust_virtual_force_integrator(FY, FZ, lx, cDrag, cLift), &
+                    'virtualForceIntegrator')
+end subroutine virtualForceIntegrator
+
+! boundary_conditions.f90:1-376
+subroutine boundary_conditions()
+   use variables
+   use ntust_hip
+   implicit none
+   call ntust_push_bc()
+   call ntust_check(ntust_boundary_conditions(u, v, w, p, u1, v1, w1, u2, v2, w2, u_star, &
+                                              v_star, w_star), 'boundary_conditions')
+end subroutine boundary_conditions

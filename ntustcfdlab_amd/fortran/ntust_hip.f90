@@ -1,0 +1,173 @@
+! ntust_hip.f90 -- iso_c_binding interface to libntust_hip.so (include/ntust.h).
+!
+! Arrays bind as assumed-size real(c_double) dummies, so passing a Fortran
+! module array passes its base address without a copy; optional arrays
+! bind as type(c_ptr), value (pass c_loc(a) or c_null_ptr).  Every function
+! returns the library status: 0 ok, > 0 warning, < 0 error.
+module ntust_hip
+   use iso_c_binding
+   implicit none
+
+   integer(c_int), parameter :: NTUST_WARN_ITMAX = 1
+   integer(c_int), parameter :: NTUST_SOR_LEXICOGRAPHIC = 0, NTUST_SOR_REDBLACK = 1, &
+                                NTUST_SOR_MIXED = 2
+   integer(c_int), parameter :: NTUST_QUICK_CENTRE = 0, NTUST_UPWIND = 1
+
+   interface
+      integer(c_int) function ntust_init(nx, ny, nz, iDx, Dxs, iDy, Dys, iDz, Dzs) &
+            bind(C, name="ntust_init")
+         import :: c_int, c_double
+         integer(c_int), value :: nx, ny, nz
+         real(c_double), intent(in) :: iDx(*), Dxs(*), iDy(*), Dys(*), iDz(*), Dzs(*)
+      end function
+
+      integer(c_int) function ntust_finalize() bind(C, name="ntust_finalize")
+         import :: c_int
+      end function
+
+      type(c_ptr) function ntust_last_error() bind(C, name="ntust_last_error")
+         import :: c_ptr
+      end function
+
+      integer(c_int) function ntust_set_bc(codes) bind(C, name="ntust_set_bc")
+         import :: c_int
+         integer(c_int), intent(in) :: codes(18)
+      end function
+
+      integer(c_int) function ntust_gradient_phi_gauss(Phi, prebase, dPhidX, Dxm, Dym, Dzm, &
+            istart, iend) bind(C, name="ntust_gradient_phi_gauss")
+         import :: c_int, c_double, c_ptr
+         real(c_double), intent(in) :: Phi(*)
+         type(c_ptr), value :: prebase
+         real(c_double), intent(inout) :: dPhidX(*)
+         real(c_double), intent(in) :: Dxm(*), Dym(*), Dzm(*)
+         integer(c_int), value :: istart, iend
+      end function
+
+      integer(c_int) function ntust_smagorinsky(u, v, w, pre_u, pre_v, pre_w, Cs, istart, iend, &
+            dudx, dvdx, dwdx, Viseff) bind(C, name="ntust_smagorinsky")
+         import :: c_int, c_double, c_ptr
+         real(c_double), intent(in) :: u(*), v(*), w(*)
+         type(c_ptr), value :: pre_u, pre_v, pre_w
+         real(c_double), value :: Cs
+         integer(c_int), value :: istart, iend
+         real(c_double), intent(inout) :: dudx(*), dvdx(*), dwdx(*)
+         real(c_double), intent(inout) :: Viseff(*)
+      end function
+
+      integer(c_int) function ntust_discretisation_quick_centre(u, v, w, Viseff, dt, nu, LES, &
+            istart, iend, u_star, v_star, w_star) bind(C, name="ntust_discretisation_quick_centre")
+         import :: c_int, c_double
+         real(c_double), intent(in) :: u(*), v(*), w(*), Viseff(*)
+         real(c_double), value :: dt, nu
+         integer(c_int), value :: LES, istart, iend
+         real(c_double), intent(inout) :: u_star(*), v_star(*), w_star(*)
+      end function
+
+      integer(c_int) function ntust_discretisation_upwind(u, v, w, dt, nu, u_star, v_star, &
+            w_star) bind(C, name="ntust_discretisation_upwind")
+         import :: c_int, c_double
+         real(c_double), intent(in) :: u(*), v(*), w(*)
+         real(c_double), value :: dt, nu
+         real(c_double), intent(inout) :: u_star(*), v_star(*), w_star(*)
+      end function
+
+      integer(c_int) function ntust_gauss_seidel(u_star, v_star, w_star, p, pre, dt, omega, &
+            zeta, itmax, ik, pChangeMax) bind(C, name="ntust_gauss_seidel")
+         import :: c_int, c_double
+         real(c_double), intent(in) :: u_star(*), v_star(*), w_star(*)
+         real(c_double), intent(inout) :: p(*), pre(*)
+         real(c_double), value :: dt, omega, zeta, itmax
+         integer(c_int), intent(out) :: ik
+         real(c_double), intent(out) :: pChangeMax
+      end function
+
+      integer(c_int) function ntust_calcul_new_velocity(u_star, v_star, w_star, p, ETA, &
+            u_solid, v_solid, w_solid, dt, istart, iend, u1, v1, w1, u2, v2, w2, FX, FY, FZ, &
+            uc, vc, wc) bind(C, name="ntust_calcul_new_velocity")
+         import :: c_int, c_double
+         real(c_double), intent(in) :: u_star(*), v_star(*), w_star(*), p(*), ETA(*)
+         real(c_double), value :: u_solid, v_solid, w_solid, dt
+         integer(c_int), value :: istart, iend
+         real(c_double), intent(inout) :: u1(*), v1(*), w1(*), u2(*), v2(*), w2(*)
+         real(c_double), intent(inout) :: FX(*), FY(*), FZ(*), uc(*), vc(*), wc(*)
+      end function
+
+      integer(c_int) function ntust_updating_velocity(u2, v2, w2, istart, iend, u, v, w) &
+            bind(C, name="ntust_updating_velocity")
+         import :: c_int, c_double
+         real(c_double), intent(in) :: u2(*), v2(*), w2(*)
+         integer(c_int), value :: istart, iend
+         real(c_double), intent(inout) :: u(*), v(*), w(*)
+      end function
+
+      integer(c_int) function ntust_virtual_force_integrator(FY, FZ, lx, cDrag, cLift) &
+            bind(C, name="ntust_virtual_force_integrator")
+         import :: c_int, c_double
+         real(c_double), intent(in) :: FY(*), FZ(*)
+         real(c_double), value :: lx
+         real(c_double), intent(out) :: cDrag, cLift
+      end function
+
+      integer(c_int) function ntust_boundary_conditions(u, v, w, p, u1, v1, w1, u2, v2, w2, &
+            u_star, v_star, w_star) bind(C, name="ntust_boundary_conditions")
+         import :: c_int, c_double
+         real(c_double), intent(inout) :: u(*), v(*), w(*), p(*)
+         real(c_double), intent(inout) :: u1(*), v1(*), w1(*), u2(*), v2(*), w2(*)
+         real(c_double), intent(inout) :: u_star(*), v_star(*), w_star(*)
+      end function
+
+      integer(c_int) function ntust_configure(dt, nu, Cs, LES, scheme, omega, zeta, itmax, &
+            sor_mode, lx, force_every) bind(C, name="ntust_configure")
+         import :: c_int, c_double
+         real(c_double), value :: dt, nu, Cs
+         integer(c_int), value :: LES, scheme
+         real(c_double), value :: omega, zeta, itmax
+         integer(c_int), value :: sor_mode
+         real(c_double), value :: lx
+         integer(c_int), value :: force_every
+      end function
+
+      integer(c_int) function ntust_state_upload(u, v, w, p, ETA) &
+            bind(C, name="ntust_state_upload")
+         import :: c_int, c_double
+         real(c_double), intent(in) :: u(*), v(*), w(*), p(*), ETA(*)
+      end function
+
+      integer(c_int) function ntust_state_download(u, v, w, p, pre, uc, vc, wc) &
+            bind(C, name="ntust_state_download")
+         import :: c_int, c_double
+         real(c_double), intent(inout) :: u(*), v(*), w(*), p(*), pre(*), uc(*), vc(*), wc(*)
+      end function
+
+      integer(c_int) function ntust_step(nsteps, ik, pchange, cdrag, clift) &
+            bind(C, name="ntust_step")
+         import :: c_int, c_double
+         integer(c_int), value :: nsteps
+         integer(c_int), intent(out) :: ik(*)
+         real(c_double), intent(out) :: pchange(*), cdrag(*), clift(*)
+      end function
+   end interface
+
+contains
+
+   ! Stops the program on a library error, as the reference stops on a
+   ! Fortran runtime error; warnings (status > 0) continue silently, as the
+   ! reference continues after a non-converged pressure solve.
+   subroutine ntust_check(rc, where)
+      integer(c_int), intent(in) :: rc
+      character(len=*), intent(in) :: where
+      character(kind=c_char), pointer :: msg(:)
+      integer :: n
+      if (rc >= 0) return
+      call c_f_pointer(ntust_last_error(), msg, [512])
+      n = 0
+      do while (n < 512)
+         if (msg(n + 1) == c_null_char) exit
+         n = n + 1
+      end do
+      write (*, '(a,a,a,i0,a,512a)') 'libntust_hip: ', where, ' failed (', rc, '): ', msg(1:n)
+      error stop 1
+   end subroutine
+
+end module ntust_hip

@@ -1,0 +1,87 @@
+"""FP32 sweeps inside FP64 defect correction (NTUST_SOR_MIXED) against the
+FP64 red-black solver it accelerates, and against the reference ordering.
+
+The mixed solver performs the FP64 red-black iteration on the correction
+e = p - p0 in FP32, so after the same number of sweeps its pressure must
+agree with the FP64 solver to FP32 rounding of the correction, and the
+velocity after full time steps to the 1e-6 relative bar of the precision
+pattern."""
+import numpy as np
+import pytest
+
+from conftest import rand_halo, stretched_grid
+from ntustcfdlab_amd.grid import Grid, bc_codes, noise_ic
+from ntustcfdlab_amd.hip import QUICK_CENTRE, SOR_LEXICOGRAPHIC, SOR_MIXED, SOR_REDBLACK, \
+    ProjectionSolver
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+@pytest.mark.parametrize("itmax", [1.0, 63.0, 64.0, 65.0, 300.0])
+def test_mixed_solve_tracks_fp64_iterates(itmax, hip_lib):
+    g = stretched_grid(20, 18, 22)
+    s = ProjectionSolver(g)
+    rng = np.random.default_rng(int(itmax))
+    us, vs, ws = (rand_halo(g, rng, 1e-2) for _ in range(3))
+    p = rand_halo(g, rng, 1e-3)
+    out = {}
+    try:
+        for mode in (SOR_REDBLACK, SOR_MIXED):
+            q = p.copy(order="F")
+            pre = g.interior()
+            ik, pm, _ = s.gauss_seidel(us, vs, ws, q, pre, 1e-3, 1.5, 1e-30, itmax, mode)
+            out[mode] = (ik, pm, q)
+    finally:
+        s.close()
+    assert out[SOR_REDBLACK][0] == out[SOR_MIXED][0] == int(itmax)
+    assert rel(out[SOR_MIXED][2], out[SOR_REDBLACK][2]) < 1e-6
+    assert abs(out[SOR_MIXED][1] - out[SOR_REDBLACK][1]) <= 1e-4 * out[SOR_REDBLACK][1]
+
+
+def test_mixed_stops_where_fp64_stops(hip_lib):
+    """zeta reached well before itmax: same sweep count (the change per sweep
+    is far from zeta's resolution limit in FP32 here)."""
+    g = stretched_grid(16, 14, 12)
+    s = ProjectionSolver(g)
+    rng = np.random.default_rng(5)
+    us, vs, ws = (rand_halo(g, rng, 1e-2) for _ in range(3))
+    p = g.halo()
+    res = []
+    try:
+        for mode in (SOR_REDBLACK, SOR_MIXED):
+            q = p.copy(order="F")
+            ik, pm, _ = s.gauss_seidel(us, vs, ws, q, g.interior(), 1e-3, 1.5, 1e-6, 5000.0, mode)
+            res.append((ik, q))
+    finally:
+        s.close()
+    assert abs(res[0][0] - res[1][0]) <= 1
+    assert rel(res[1][1], res[0][1]) < 1e-5
+
+
+def test_mixed_steps_velocity_error(hip_lib):
+    """Resident LES cavity steps: mixed vs FP64 red-black velocity within 1e-6."""
+    g = Grid(32, 32, 32)
+    u0, v0, w0 = noise_ic(g, 1e-3, seed=4)
+    finals = {}
+    for mode in (SOR_REDBLACK, SOR_MIXED):
+        s = ProjectionSolver(g, bc_codes())
+        try:
+            u, v, w, p = u0.copy(order="F"), v0.copy(order="F"), w0.copy(order="F"), g.halo()
+            s.boundary_conditions(u, v, w, p)
+            s.configure(dt=5e-4, nu=1e-3, Cs=0.1, LES=1, scheme=QUICK_CENTRE, omega=1.5,
+                        zeta=1e-10, itmax=400.0, sor=mode)
+            s.upload(u, v, w, p, g.halo())
+            ik, *_ = s.step(5)
+            out = [g.halo() for _ in range(4)]
+            s.download(*out)
+            finals[mode] = (ik, out)
+        finally:
+            s.close()
+    a, b = finals[SOR_MIXED][1], finals[SOR_REDBLACK][1]
+    vel = max(np.abs(x).max() for x in b[:3])
+    err = max(np.abs(x - y).max() for x, y in zip(a[:3], b[:3])) / vel
+    assert err < 1e-6, err
