@@ -79,13 +79,38 @@ def cpu_baseline(grid, w, state, sweeps_per_step):
                        f"{sweeps_per_step:.0f} sweeps/step")}
 
 
+def timed_run(s, hip, hiptime, state, sor, steps, warmup):
+    """Upload the state, W warm-up steps, then K steps bracketed by HIP
+    events on the library stream.  Returns (ms, per-step sweeps, prof, final)."""
+    g = s.grid
+    s.configure_sor(sor)
+    s.upload(*state)
+    if warmup:
+        s.step(warmup)
+    s.prof_reset()
+    e0, e1 = hiptime.Event(), hiptime.Event()
+    hiptime.device_synchronize()
+    t0 = time.perf_counter()
+    e0.record(s.stream())
+    ik, pm, cd, cl, rc = s.step(steps)
+    e1.record(s.stream())
+    ms = e0.elapsed_ms(e1)
+    hiptime.device_synchronize()
+    wall = time.perf_counter() - t0
+    prof = s.prof_read()
+    final = [g.halo() for _ in range(4)]
+    s.download(*final)
+    return ms, ik, prof, final, wall
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", type=int, default=128)
-    ap.add_argument("--sor", choices=["redblack", "lexicographic"], default="redblack")
+    ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
+                    help="time one solver only")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -99,52 +124,71 @@ def main():
     w = workload(a.size)
     g = Grid(w["nx"], w["ny"], w["nz"])
     s = hip.ProjectionSolver(g, bc_codes())
-    sor = hip.SOR_REDBLACK if a.sor == "redblack" else hip.SOR_LEXICOGRAPHIC
-    s.configure(dt=w["dt"], nu=1.0 / w["Re"], Cs=w["Cs"], LES=w["LES"], scheme=w["scheme"],
-                omega=w["omega"], zeta=w["zeta"], itmax=w["itmax"], sor=sor, lx=1.0,
-                force_every=1)
+
+    def configure_sor(sor):
+        s.configure(dt=w["dt"], nu=1.0 / w["Re"], Cs=w["Cs"], LES=w["LES"], scheme=w["scheme"],
+                    omega=w["omega"], zeta=w["zeta"], itmax=w["itmax"], sor=sor, lx=1.0,
+                    force_every=1)
+    s.configure_sor = configure_sor
+    configure_sor(hip.SOR_REDBLACK)
     state = initial_state(g, w, s)
-    s.upload(*state)
-    print(f"# warmup {a.warmup} steps", file=sys.stderr, flush=True)
-    if a.warmup:
-        s.step(a.warmup)
-    s.prof_reset()
-    e0, e1 = hiptime.Event(), hiptime.Event()
-    hiptime.device_synchronize()
-    print(f"# timing {a.steps} steps", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    e0.record(s.stream())
-    ik, pm, cd, cl, rc = s.step(a.steps)
-    e1.record(s.stream())
-    ms = e0.elapsed_ms(e1)
-    hiptime.device_synchronize()
-    wall = time.perf_counter() - t0
-    prof_ms, launches = s.prof_read()
+
+    runs = {}
+    for name, sor in (("fp64", hip.SOR_REDBLACK), ("mixed", hip.SOR_MIXED)):
+        if a.only and a.only != name:
+            continue
+        print(f"# {name}: warmup {a.warmup}, timing {a.steps} steps", file=sys.stderr, flush=True)
+        runs[name] = timed_run(s, hip, hiptime, state, sor, a.steps, a.warmup)
     s.close()
 
     cells = g.cells
-    value = cells * a.steps / (ms / 1e3) / 1e6
-    sweeps = float(np.mean(ik))
-    roof = None
-    if launches:
-        avg_ms = prof_ms / launches
-        bytes_per_launch = 12.0 * cells           # half-sweep: read p, b; write p (FP64)
-        achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
-        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_rb_sweep (one colour half-sweep)",
-                "avg_launch_us": round(avg_ms * 1e3, 3), "launches_timed": launches}
+
+    def summary(name):
+        ms, ik, (prof_ms, launches), final, wall = runs[name]
+        value = cells * a.steps / (ms / 1e3) / 1e6
+        roof = None
+        if launches:
+            avg_ms = prof_ms / launches
+            per_cell = 12.0 if name == "fp64" else 6.0   # algorithmic bytes per half-sweep
+            achieved = per_cell * cells / (avg_ms / 1e3) / 1e9
+            roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "kernel": "k_rb_sweep<C> (one colour half-sweep, FP64)" if name == "fp64"
+                    else "k_rb_sweep32<C> (one colour half-sweep, FP32 correction)",
+                    "bytes_per_launch": per_cell * cells,
+                    "avg_launch_us": round(avg_ms * 1e3, 3), "launches_timed": launches}
+        return value, ms / a.steps, float(np.mean(ik)), roof, wall
+
+    err = None
+    if "fp64" in runs and "mixed" in runs:
+        ref = runs["fp64"][3]
+        vel = max(np.abs(x).max() for x in ref[:3])
+        err = max(np.abs(x - y).max() for x, y in zip(runs["mixed"][3][:3], ref[:3])) / vel
+    # the headline is the FP32-sweep + FP64-refine solver when it meets the
+    # 1e-6 relative velocity bar against the FP64 solver, else the FP64 one
+    head = "mixed" if ("mixed" in runs and (err is None or err <= 1e-6)) else "fp64"
+    value, msps, sweeps, roof, wall = summary(head)
+    extra = {}
+    for name in runs:
+        v, m, sw, r, wl = summary(name)
+        extra[name] = {"value": round(v, 3), "ms_per_step": round(m, 3),
+                       "sweeps_per_step": round(sw, 1), "roofline_frac": r and r["frac"]}
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Mcell-steps/s", "n_gpus": 1,
-        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms / a.steps, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(msps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if head == "fp64" else "f32 sweeps + f64 refine",
         "data": "synthetic",
         "config": {"workload": (f"LES cavity Re=1000 {a.size}^3 uniform, QUICK, Smagorinsky "
-                                f"Cs=0.1, dt=5e-4, SOR omega=1.5 zeta=1e-10 itmax=3000 "
-                                f"({a.sor}), FP64"),
-                   "cells": cells, "sor": a.sor, "sweeps_per_step": round(sweeps, 1),
+                                f"Cs=0.1, dt=5e-4, SOR omega=1.5 zeta=1e-10 itmax=3000, "
+                                f"red-black, {'FP64' if head == 'fp64' else 'FP32 sweeps in FP64 defect correction'}"),
+                   "cells": cells, "solver": head, "sweeps_per_step": round(sweeps, 1),
                    "wall_s": round(wall, 3)},
         "roofline": roof,
+        "solvers": extra,
+        "mixed_vs_fp64_rel_velocity_err": err,
+        "mixed_speedup_vs_fp64": (round(extra["mixed"]["value"] / extra["fp64"]["value"], 3)
+                                  if len(extra) == 2 else None),
     }
     if not a.no_cpu_baseline:
         print("# cpu baseline", file=sys.stderr, flush=True)

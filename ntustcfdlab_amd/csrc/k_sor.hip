@@ -143,21 +143,27 @@ __global__ void k_rb_scatter(KGrid g, SplitGrid s, const double *__restrict__ p0
     if (pre) pre[iix(g, i, j, k)] = val;
 }
 
-// One half-sweep of colour c, sweep t (1-based).  slots[t-1] holds the
-// previous sweep's max|pNew-p| (slots[0] = 1.0, as GS.f90:9 starts), the
-// kernel adds its own maximum into slots[t].  Point update, bit-identical
-// to the CPU red-black restatement:
-//   aE = (iDy*iDz)*rx(i) ... ; s = fma(aE,pE, fma(aW,pW, ... aB*pB))
-//   aP = aE+aW+aN+aS+aF+aB;  pNew = (s-b)/aP;  p = fma(omega, pNew-p, p)
+// One half-sweep of colour c, sweep t (1-based).  The buckets of sweep t-1
+// hold the previous sweep's max|pNew-p| (sweep 0: 1.0, as GS.f90:9 starts);
+// the kernel raises the buckets of sweep t.  Point update, the operator
+// factored by axis, bit-identical to the CPU red-black restatement:
+//   t1 = fma(rE,pE, rW*pW)  t2 = fma(rN,pN, rS*pS)  t3 = fma(rF,pF, rB*pB)
+//   sum = fma(ayz, t1, fma(axz, t2, axy*t3))
+//   aP  = fma(ayz, rE+rW, fma(axz, rN+rS, axy*(rF+rB)))
+//   pNew = (sum-b)/aP;  p = fma(omega, pNew-p, p)
 //
 // The maximum of sweep t is kept in SLOT_BUCKETS buckets on separate cache
 // lines (block b raises bucket b mod SLOT_BUCKETS, and only when its value is
 // larger), because thousands of atomics on one address serialise; every
 // wavefront of the next sweep reads the buckets back with one load.
 //
-// Each thread updates RB_KPT cells along k.  The loads of all of them are
-// issued together with the gate load, before anything waits, so a block
-// pays one memory latency; only the stores depend on the gate.
+// Each thread updates RB_KPT cells along k of one (q, j) column.  The
+// cell pair of a thread alternates between i = 2q+1+sh and 2q+2-sh with the
+// parity of k, so the x metrics are loaded once per thread for both.  All
+// loads of all the thread's cells are issued before any of them is used and
+// before the gate value is waited on (no per-cell branch, indices clamped
+// into the arrays), so a wavefront pays one memory latency; only the stores
+// depend on the gate and on the cell being interior.  Offsets are 32-bit.
 template <int C>
 __global__ void __launch_bounds__(256)
 k_rb_sweep(KGrid g, SplitGrid s, const double *__restrict__ rx,
@@ -172,43 +178,56 @@ k_rb_sweep(KGrid g, SplitGrid s, const double *__restrict__ rx,
     const int lane = tid & 63;
     double gate = lane < SLOT_BUCKETS ? slots[slot_index(t - 1, lane)] : 0.0;
     const int stopped = ctl->stop;
+
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const long pl = (long)s.mp * s.ry;
-    double pcm = 0.0;
-    double nv[RB_KPT];
-    long at[RB_KPT];
+    const int jr = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int j = min(jr, g.ny);
+    const int k0 = blockIdx.z * RB_KPT + 1;
+    const int sh0 = (C + 1 + j + k0) & 1;
+    const int ia = min(2 * q + 1 + sh0, g.nx + 1), ib = min(2 * q + 2 - sh0, g.nx + 1);
+    const double axa = g.iDx[ia], rEa = rx[ia], rWa = rx[ia - 1];
+    const double axb = g.iDx[ib], rEb = rx[ib], rWb = rx[ib - 1];
+    const double ay = g.iDy[j], rN = ry[j], rS = ry[j - 1];
+    const unsigned mp = (unsigned)s.mp, pl = (unsigned)s.mp * (unsigned)s.ry;
+    const unsigned q1 = (unsigned)min(q, (g.nx + 1) / 2);
+    double vW[RB_KPT], vE[RB_KPT], vN[RB_KPT], vS[RB_KPT], vF[RB_KPT], vB[RB_KPT], vb[RB_KPT],
+        vo[RB_KPT];
+    unsigned at[RB_KPT];
+#pragma unroll
+    for (int r = 0; r < RB_KPT; r++) {
+        const int k = min(k0 + r, g.nz);
+        const unsigned row = mp * ((unsigned)j + (unsigned)s.ry * (unsigned)k);
+        const unsigned o = row + q1 + (unsigned)(sh0 ^ (r & 1));
+        at[r] = o;
+        vW[r] = pc_oth[row + q1];
+        vE[r] = pc_oth[row + q1 + 1];
+        vN[r] = pc_oth[o + mp];
+        vS[r] = pc_oth[o - mp];
+        vF[r] = pc_oth[o + pl];
+        vB[r] = pc_oth[o - pl];
+        vb[r] = b_own[o];
+        vo[r] = pc_own[o];
+    }
+    double pcm = 0.0, nv[RB_KPT];
     bool in[RB_KPT];
 #pragma unroll
     for (int r = 0; r < RB_KPT; r++) {
-        const int k = blockIdx.z * RB_KPT + r + 1;
-        const int sh = (C + 1 + j + k) & 1;
-        const int i = 2 * q + 1 + sh;
-        in[r] = i <= g.nx && j <= g.ny && k <= g.nz;
-        at[r] = 0;
-        nv[r] = 0.0;
-        if (in[r]) {
-            const int m = q + sh;
-            const long row = (long)s.mp * ((long)j + (long)s.ry * (long)k);
-            const long o = row + m;
-            const double ax = g.iDx[i], ay = g.iDy[j], az = g.iDz[k];
-            const double ayz = ay * az, axz = ax * az, axy = ax * ay;
-            const double aE = ayz * rx[i], aW = ayz * rx[i - 1];
-            const double aN = axz * ry[j], aS = axz * ry[j - 1];
-            const double aF = axy * rz[k], aB = axy * rz[k - 1];
-            const double pE = pc_oth[row + m + 1 - sh], pW = pc_oth[row + m - sh];
-            const double pN = pc_oth[o + s.mp], pS = pc_oth[o - s.mp];
-            const double pF = pc_oth[o + pl], pB = pc_oth[o - pl];
-            const double sum = fma(aE, pE, fma(aW, pW, fma(aN, pN, fma(aS, pS, fma(aF, pF, aB * pB)))));
-            const double aP = ((((aE + aW) + aN) + aS) + aF) + aB;
-            const double pNew = (sum - b_own[o]) / aP;
-            const double pold = pc_own[o];
-            const double d = pNew - pold;
-            double pc = fabs(d);
-            if (pc > pcm) pcm = pc;   // a NaN change never raises the max (GS.f90:61)
-            nv[r] = fma(omega, d, pold);
-            at[r] = o;
-        }
+        const int k = min(k0 + r, g.nz);
+        const bool odd = r & 1;
+        const double ax = odd ? axb : axa, rE = odd ? rEb : rEa, rW = odd ? rWb : rWa;
+        const double az = g.iDz[k], rF = rz[k], rB = rz[k - 1];
+        const double ayz = ay * az, axz = ax * az, axy = ax * ay;
+        const double t1 = fma(rE, vE[r], rW * vW[r]);
+        const double t2 = fma(rN, vN[r], rS * vS[r]);
+        const double t3 = fma(rF, vF[r], rB * vB[r]);
+        const double sum = fma(ayz, t1, fma(axz, t2, axy * t3));
+        const double aP = fma(ayz, rE + rW, fma(axz, rN + rS, axy * (rF + rB)));
+        const double d = (sum - vb[r]) / aP - vo[r];
+        const int i = 2 * q + 1 + (sh0 ^ (int)odd);
+        in[r] = jr <= g.ny && k0 + r <= g.nz && i <= g.nx;
+        const double pc = in[r] ? fabs(d) : 0.0;
+        if (pc > pcm) pcm = pc;   // a NaN change never raises the max (GS.f90:61)
+        nv[r] = fma(omega, d, vo[r]);
     }
     // every wavefront of the block reads the same buckets: the decision is
     // uniform over the block, so returning here cannot split a barrier

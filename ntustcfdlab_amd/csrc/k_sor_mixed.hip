@@ -33,44 +33,57 @@ k_rb_sweep32(KGrid g, SplitGrid s, FMetric fm, float *__restrict__ e_own,
     const int lane = tid & 63;
     double gate = lane < SLOT_BUCKETS ? slots[slot_index(t - 1, lane)] : 0.0;
     const int stopped = ctl->stop;
+
+    // same thread layout and load-first structure as the FP64 sweep
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const long pl = (long)s.mp * s.ry;
-    float pcm = 0.0f;
-    float nv[RB_KPT];
-    long at[RB_KPT];
+    const int jr = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int j = min(jr, g.ny);
+    const int k0 = blockIdx.z * RB_KPT + 1;
+    const int sh0 = (C + 1 + j + k0) & 1;
+    const int ia = min(2 * q + 1 + sh0, g.nx + 1), ib = min(2 * q + 2 - sh0, g.nx + 1);
+    const float axa = fm.iDx[ia], rEa = fm.rx[ia], rWa = fm.rx[ia - 1];
+    const float axb = fm.iDx[ib], rEb = fm.rx[ib], rWb = fm.rx[ib - 1];
+    const float ay = fm.iDy[j], rN = fm.ry[j], rS = fm.ry[j - 1];
+    const unsigned mp = (unsigned)s.mp, pl = (unsigned)s.mp * (unsigned)s.ry;
+    const unsigned q1 = (unsigned)min(q, (g.nx + 1) / 2);
+    float vW[RB_KPT], vE[RB_KPT], vN[RB_KPT], vS[RB_KPT], vF[RB_KPT], vB[RB_KPT], vr[RB_KPT],
+        vo[RB_KPT];
+    unsigned at[RB_KPT];
+#pragma unroll
+    for (int r = 0; r < RB_KPT; r++) {
+        const int k = min(k0 + r, g.nz);
+        const unsigned row = mp * ((unsigned)j + (unsigned)s.ry * (unsigned)k);
+        const unsigned o = row + q1 + (unsigned)(sh0 ^ (r & 1));
+        at[r] = o;
+        vW[r] = e_oth[row + q1];
+        vE[r] = e_oth[row + q1 + 1];
+        vN[r] = e_oth[o + mp];
+        vS[r] = e_oth[o - mp];
+        vF[r] = e_oth[o + pl];
+        vB[r] = e_oth[o - pl];
+        vr[r] = r_own[o];
+        vo[r] = e_own[o];
+    }
+    float pcm = 0.0f, nv[RB_KPT];
     bool in[RB_KPT];
 #pragma unroll
     for (int r = 0; r < RB_KPT; r++) {
-        const int k = blockIdx.z * RB_KPT + r + 1;
-        const int sh = (C + 1 + j + k) & 1;
-        const int i = 2 * q + 1 + sh;
-        in[r] = i <= g.nx && j <= g.ny && k <= g.nz;
-        at[r] = 0;
-        nv[r] = 0.0f;
-        if (in[r]) {
-            const int m = q + sh;
-            const long row = (long)s.mp * ((long)j + (long)s.ry * (long)k);
-            const long o = row + m;
-            const float ax = fm.iDx[i], ay = fm.iDy[j], az = fm.iDz[k];
-            const float ayz = ay * az, axz = ax * az, axy = ax * ay;
-            const float aE = ayz * fm.rx[i], aW = ayz * fm.rx[i - 1];
-            const float aN = axz * fm.ry[j], aS = axz * fm.ry[j - 1];
-            const float aF = axy * fm.rz[k], aB = axy * fm.rz[k - 1];
-            const float sum = fmaf(aE, e_oth[row + m + 1 - sh],
-                              fmaf(aW, e_oth[row + m - sh],
-                              fmaf(aN, e_oth[o + s.mp],
-                              fmaf(aS, e_oth[o - s.mp],
-                              fmaf(aF, e_oth[o + pl], aB * e_oth[o - pl])))));
-            const float aP = ((((aE + aW) + aN) + aS) + aF) + aB;
-            const float eNew = (sum + r_own[o]) / aP;
-            const float eold = e_own[o];
-            const float d = eNew - eold;
-            const float pc = fabsf(d);
-            if (pc > pcm) pcm = pc;
-            nv[r] = fmaf(omega, d, eold);
-            at[r] = o;
-        }
+        const int k = min(k0 + r, g.nz);
+        const bool odd = r & 1;
+        const float ax = odd ? axb : axa, rE = odd ? rEb : rEa, rW = odd ? rWb : rWa;
+        const float az = fm.iDz[k], rF = fm.rz[k], rB = fm.rz[k - 1];
+        const float ayz = ay * az, axz = ax * az, axy = ax * ay;
+        const float t1 = fmaf(rE, vE[r], rW * vW[r]);
+        const float t2 = fmaf(rN, vN[r], rS * vS[r]);
+        const float t3 = fmaf(rF, vF[r], rB * vB[r]);
+        const float sum = fmaf(ayz, t1, fmaf(axz, t2, axy * t3));
+        const float aP = fmaf(ayz, rE + rW, fmaf(axz, rN + rS, axy * (rF + rB)));
+        const float d = (sum + vr[r]) / aP - vo[r];
+        const int i = 2 * q + 1 + (sh0 ^ (int)odd);
+        in[r] = jr <= g.ny && k0 + r <= g.nz && i <= g.nx;
+        const float pc = in[r] ? fabsf(d) : 0.0f;
+        if (pc > pcm) pcm = pc;
+        nv[r] = fmaf(omega, d, vo[r]);
     }
     gate = wave_max(gate);
     const bool active = !stopped && ((double)(t - 1) < itmax) && (gate > zeta);
@@ -137,13 +150,12 @@ k_rb_residual(KGrid g, SplitGrid s, const double *__restrict__ rx,
     const long pl = (long)s.mp * s.ry;
     const double ax = g.iDx[i], ay = g.iDy[j], az = g.iDz[k];
     const double ayz = ay * az, axz = ax * az, axy = ax * ay;
-    const double aE = ayz * rx[i], aW = ayz * rx[i - 1];
-    const double aN = axz * ry[j], aS = axz * ry[j - 1];
-    const double aF = axy * rz[k], aB = axy * rz[k - 1];
-    const double sum = fma(aE, oth[six(s, i + 1, j, k)], fma(aW, oth[six(s, i - 1, j, k)],
-                       fma(aN, oth[o + s.mp], fma(aS, oth[o - s.mp],
-                       fma(aF, oth[o + pl], aB * oth[o - pl])))));
-    const double aP = ((((aE + aW) + aN) + aS) + aF) + aB;
+    const double rE = rx[i], rW = rx[i - 1], rN = ry[j], rS = ry[j - 1], rF = rz[k], rB = rz[k - 1];
+    const double t1 = fma(rE, oth[six(s, i + 1, j, k)], rW * oth[six(s, i - 1, j, k)]);
+    const double t2 = fma(rN, oth[o + s.mp], rS * oth[o - s.mp]);
+    const double t3 = fma(rF, oth[o + pl], rB * oth[o - pl]);
+    const double sum = fma(ayz, t1, fma(axz, t2, axy * t3));
+    const double aP = fma(ayz, rE + rW, fma(axz, rN + rS, axy * (rF + rB)));
     const double r = (sum - (c ? b1 : b0)[o]) - aP * own[o];
     (c ? r1 : r0)[o] = (float)r;
 }

@@ -711,15 +711,13 @@ int or_sor_redblack(const or_grid *g, const double *b, double *p,
                         if (((i + j + k) & 1) != colour) continue;
                         double ax = M(g->iDx, i), ay = M(g->iDy, j), az = M(g->iDz, k);
                         double ayz = ay * az, axz = ax * az, axy = ax * ay;
-                        double aE = ayz * M(rx, i), aW = ayz * M(rx, i - 1);
-                        double aN = axz * M(ry, j), aS = axz * M(ry, j - 1);
-                        double aF = axy * M(rz, k), aB = axy * M(rz, k - 1);
-                        double s = fma(aE, p[H(i + 1, j, k)],
-                                   fma(aW, p[H(i - 1, j, k)],
-                                   fma(aN, p[H(i, j + 1, k)],
-                                   fma(aS, p[H(i, j - 1, k)],
-                                   fma(aF, p[H(i, j, k + 1)], aB * p[H(i, j, k - 1)])))));
-                        double aP = ((((aE + aW) + aN) + aS) + aF) + aB;
+                        double rE = M(rx, i), rW = M(rx, i - 1), rN = M(ry, j), rS = M(ry, j - 1);
+                        double rF = M(rz, k), rB = M(rz, k - 1);
+                        double t1 = fma(rE, p[H(i + 1, j, k)], rW * p[H(i - 1, j, k)]);
+                        double t2 = fma(rN, p[H(i, j + 1, k)], rS * p[H(i, j - 1, k)]);
+                        double t3 = fma(rF, p[H(i, j, k + 1)], rB * p[H(i, j, k - 1)]);
+                        double s = fma(ayz, t1, fma(axz, t2, axy * t3));
+                        double aP = fma(ayz, rE + rW, fma(axz, rN + rS, axy * (rF + rB)));
                         double pNew = (s - b[I3(i, j, k)]) / aP;
                         size_t o = H(i, j, k);
                         double pChange = fabs(pNew - p[o]);

@@ -97,11 +97,12 @@ void or_poisson_b(const or_grid *g, const double *us, const double *vs,
                   const double *ws, double dt, double *b);
 
 /* Red-black reordering of the same SOR (the HIP fast mode): red cells are
- * (i+j+k) even.  Point update, written with explicit fma so the HIP kernel
- * reproduces it bit for bit:
- *   aE = (iDy*iDz)*rDxs(i), aW = (iDy*iDz)*rDxs(i-1), ... (r* = 1/D*s)
- *   s  = fma(aE,pE, fma(aW,pW, fma(aN,pN, fma(aS,pS, fma(aF,pF, aB*pB)))))
- *   aP = ((((aE+aW)+aN)+aS)+aF)+aB
+ * (i+j+k) even.  Point update, the same operator factored by axis and
+ * written with explicit fma so the HIP kernel reproduces it bit for bit
+ * (rE = 1/Dxs(i), rW = 1/Dxs(i-1), rN = 1/Dys(j), ... ):
+ *   t1 = fma(rE,pE, rW*pW)  t2 = fma(rN,pN, rS*pS)  t3 = fma(rF,pF, rB*pB)
+ *   s  = fma(iDy*iDz, t1, fma(iDx*iDz, t2, (iDx*iDy)*t3))
+ *   aP = fma(iDy*iDz, rE+rW, fma(iDx*iDz, rN+rS, (iDx*iDy)*(rF+rB)))
  *   pNew = (s - b)/aP;   pChange = |pNew-p|;   p = fma(omega, pNew-p, p)
  * Stopping rule as GS.f90:11 with the max over both colours. */
 int or_sor_redblack(const or_grid *g, const double *b, double *p,

@@ -23,11 +23,16 @@ def rel(a, b):
 
 @pytest.mark.parametrize("itmax", [1.0, 63.0, 64.0, 65.0, 300.0])
 def test_mixed_solve_tracks_fp64_iterates(itmax, hip_lib):
-    g = stretched_grid(20, 18, 22)
+    """From p = 0 with a smooth large-scale source the solve is still far from
+    converged at itmax, so the last sweep's max change is a meaningful value
+    (not round-off) and must agree too."""
+    g = stretched_grid(40, 36, 44)
     s = ProjectionSolver(g)
     rng = np.random.default_rng(int(itmax))
     us, vs, ws = (rand_halo(g, rng, 1e-2) for _ in range(3))
-    p = rand_halo(g, rng, 1e-3)
+    x = np.linspace(0, np.pi, g.halo_shape[0])[:, None, None]
+    us += np.asfortranarray(np.broadcast_to(np.sin(x), g.halo_shape))
+    p = g.halo()
     out = {}
     try:
         for mode in (SOR_REDBLACK, SOR_MIXED):
@@ -39,7 +44,9 @@ def test_mixed_solve_tracks_fp64_iterates(itmax, hip_lib):
         s.close()
     assert out[SOR_REDBLACK][0] == out[SOR_MIXED][0] == int(itmax)
     assert rel(out[SOR_MIXED][2], out[SOR_REDBLACK][2]) < 1e-6
-    assert abs(out[SOR_MIXED][1] - out[SOR_REDBLACK][1]) <= 1e-4 * out[SOR_REDBLACK][1]
+    # still far above FP32 resolution of the correction
+    assert out[SOR_REDBLACK][1] > 1e-7 * np.abs(out[SOR_REDBLACK][2]).max()
+    assert abs(out[SOR_MIXED][1] - out[SOR_REDBLACK][1]) <= 1e-3 * out[SOR_REDBLACK][1]
 
 
 def test_mixed_stops_where_fp64_stops(hip_lib):
