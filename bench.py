@@ -112,6 +112,8 @@ def main():
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
                     help="time one solver only")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="one 256 MiB device copy before timing (counter-profile scaling)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported: the path runs "
@@ -124,6 +126,8 @@ def main():
     w = workload(a.size)
     g = Grid(w["nx"], w["ny"], w["nz"])
     s = hip.ProjectionSolver(g, bc_codes())
+    if a.calibrate:
+        hiptime.copy_calibration()
 
     def configure_sor(sor):
         s.configure(dt=w["dt"], nu=1.0 / w["Re"], Cs=w["Cs"], LES=w["LES"], scheme=w["scheme"],
@@ -142,6 +146,14 @@ def main():
     s.close()
 
     cells = g.cells
+    # HBM bytes per launch of the half-sweep kernels, from the counter
+    # profile of this workload (FETCH_SIZE/WRITE_SIZE scaled by a plain copy)
+    traffic = {}
+    tfile = ROOT / "profiles" / f"pmc_traffic_{a.size}.json"
+    if tfile.exists():
+        t = json.loads(tfile.read_text())
+        if t.get("cells") == cells:
+            traffic = {k: v["read_bytes"] + v["write_bytes"] for k, v in t["per_launch"].items()}
 
     def summary(name):
         ms, ik, (prof_ms, launches), final, wall = runs[name]
@@ -151,8 +163,10 @@ def main():
             avg_ms = prof_ms / launches
             per_cell = 12.0 if name == "fp64" else 6.0   # algorithmic bytes per half-sweep
             achieved = per_cell * cells / (avg_ms / 1e3) / 1e9
+            tr = traffic.get("k_rb_sweep" if name == "fp64" else "k_rb_sweep32")
             roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": round(tr) if tr else None,
                     "kernel": "k_rb_sweep<C> (one colour half-sweep, FP64)" if name == "fp64"
                     else "k_rb_sweep32<C> (one colour half-sweep, FP32 correction)",
                     "bytes_per_launch": per_cell * cells,

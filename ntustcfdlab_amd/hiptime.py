@@ -57,3 +57,24 @@ class Event:
 
 def device_synchronize():
     _ok(_rt().hipDeviceSynchronize())
+
+
+def copy_calibration(nbytes: int = 256 << 20) -> int:
+    """One device-to-device copy of `nbytes` (the runtime's copy kernel), so a
+    counter profile of the same process holds a plain copy of known size to
+    scale FETCH_SIZE / WRITE_SIZE by.  Returns nbytes."""
+    rt = _rt()
+    rt.hipMalloc.argtypes = [ct.POINTER(ct.c_void_p), ct.c_size_t]
+    rt.hipFree.argtypes = [ct.c_void_p]
+    rt.hipMemset.argtypes = [ct.c_void_p, ct.c_int, ct.c_size_t]
+    rt.hipMemcpy.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_size_t, ct.c_int]
+    a, b = ct.c_void_p(), ct.c_void_p()
+    _ok(rt.hipMalloc(ct.byref(a), nbytes))
+    _ok(rt.hipMalloc(ct.byref(b), nbytes))
+    _ok(rt.hipMemset(a, 1, nbytes))
+    _ok(rt.hipDeviceSynchronize())
+    _ok(rt.hipMemcpy(b, a, nbytes, 3))   # hipMemcpyDeviceToDevice
+    _ok(rt.hipDeviceSynchronize())
+    _ok(rt.hipFree(a))
+    _ok(rt.hipFree(b))
+    return nbytes
