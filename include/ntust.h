@@ -37,7 +37,8 @@ enum {
     NTUST_E_NOT_INIT = -1,
     NTUST_E_ARG = -2,
     NTUST_E_HIP = -3,
-    NTUST_E_ALLOC = -4
+    NTUST_E_ALLOC = -4,
+    NTUST_E_STATE = -5           /* a recomputation did not reproduce the forward pass */
 };
 
 /* pressure-solver orderings (ntust_configure sor_mode) */
@@ -160,6 +161,64 @@ int ntust_state_download(double *u, double *v, double *w, double *p, double *pre
  * NULL): SOR sweep count, last max|dp|, cDrag, cLift.  Returns
  * NTUST_WARN_ITMAX if any step hit the sweep cap. */
 int ntust_step(int nsteps, int *ik, double *pchange, double *cdrag, double *clift);
+
+/* ---- discrete adjoint of the resident step ------------------------------
+ *
+ * No counterpart in the reference: gradients of an objective of the flow
+ * with respect to the state and to a body force f (a per-face force field,
+ * the control) that the resident step adds to the predictor,
+ * u* = u* + dt*f on the interior (0 unless set).  The adjoint of a step is
+ * the exact transpose of the step the forward ran: the transposed QUICK /
+ * upwind predictor with the upwind choices frozen, the linearised
+ * Smagorinsky viscosity, the SOR sweeps transposed one by one (same sweep
+ * count; the operator is symmetric, so each transposed sweep is a
+ * homogeneous SOR sweep with the colours in reverse order), the projection
+ * with the ETA blend, and the boundary conditions.  Needs the red-black or
+ * mixed pressure solver.  Adjoint arrays are halo arrays like the state. */
+
+/* Body force (three halo arrays, interior used); all NULL removes it. */
+int ntust_set_body_force(const double *fu, const double *fv, const double *fw);
+
+/* Kinetic energy 0.5*sum(u^2+v^2+w^2)*iDx*iDy*iDz over the interior faces of
+ * the device state. */
+int ntust_kinetic_energy(double *ke);
+
+/* Exact FP64 copies of the device state (u, v, w, p) in device memory. */
+int ntust_checkpoint_reserve(int nslots);
+int ntust_checkpoint_save(int slot);
+int ntust_checkpoint_load(int slot);
+
+/* Adjoint state lambda = dJ/d(u, v, w, p) at the current step boundary.
+ * ntust_adjoint_set uploads it (a NULL array is zero) and clears the
+ * body-force gradient; ntust_adjoint_get downloads it (NULL skips). */
+int ntust_adjoint_set(const double *lu, const double *lv, const double *lw, const double *lp);
+int ntust_adjoint_get(double *lu, double *lv, double *lw, double *lp);
+
+/* lambda_u += scale * dKE/du (v, w alike) at the current device state. */
+int ntust_adjoint_add_ke(double scale);
+
+/* One reverse step.  The device state must be the state at the start of the
+ * step; the step is recomputed (its sweep count in *ik), lambda becomes the
+ * adjoint of that state, dt*lambda_u* is added to the body-force gradient,
+ * and the device state is left unchanged. */
+int ntust_adjoint_step(int *ik);
+
+/* Body-force gradient dJ/df accumulated since the last ntust_adjoint_set or
+ * ntust_adjoint_run (halo arrays, interior meaningful; NULL skips). */
+int ntust_adjoint_gradient(double *gfu, double *gfv, double *gfw);
+
+/* Forward nsteps from the device state and back.  Objective: J = mean of
+ * the kinetic energy after each of the last navg steps.  The forward pass
+ * keeps a checkpoint every checkpoint_every steps; the reverse pass
+ * recomputes each segment from its checkpoint (the last segment's states
+ * are kept from the forward pass) and applies the reverse steps.  Outputs:
+ * *J, ik[nsteps] the sweep counts (may be NULL), ms[4] (may be NULL): forward
+ * pass, reverse pass, its recomputation part, its adjoint part, in
+ * milliseconds.  Afterwards the device holds the final forward state,
+ * lambda the adjoint of the initial state and ntust_adjoint_gradient dJ/df.
+ * Returns NTUST_E_STATE if a recomputed step's sweep count differs. */
+int ntust_adjoint_run(int nsteps, int checkpoint_every, int navg, double *J, int *ik,
+                      double *ms);
 
 /* ---- measurement -------------------------------------------------------- */
 

@@ -12,6 +12,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <chrono>
 #include <string>
 #include <vector>
 
@@ -72,6 +73,11 @@ struct Ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double prof_ms = 0.0;
     long long prof_launches = 0;
+    // body force added to the predictor (device halo arrays, "body_*")
+    bool body = false;
+    double *fu = nullptr, *fv = nullptr, *fw = nullptr;
+    // exact FP64 checkpoints of (u, v, w, p), slots of 4 halo arrays
+    int nckpt = 0;
 };
 
 Ctx C;
@@ -277,6 +283,191 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
     for (int b = 0; b < SLOT_BUCKETS; b++) m = bk[b * SLOT_STRIDE] > m ? bk[b * SLOT_STRIDE] : m;
     *pmax_out = m;
     *ik_out = h.ik;
+    return 0;
+}
+
+// ---- one device-resident step -------------------------------------------
+
+struct StepBufs {
+    double *u, *v, *w, *p, *eta, *us, *vs, *ws, *u2, *v2, *w2, *pre, *uc, *vc, *wc, *vis, *filt;
+    double *FY = nullptr, *FZ = nullptr, *fxz = nullptr, *fyz = nullptr, *fxy = nullptr,
+           *fyy = nullptr, *coef = nullptr;
+};
+
+int step_buffers(StepBufs &b, bool force)
+{
+    const char *hn[11] = {"u", "v", "w", "p", "eta", "us", "vs", "ws", "u2", "v2", "w2"};
+    double **hp[11] = {&b.u, &b.v, &b.w, &b.p, &b.eta, &b.us, &b.vs, &b.ws, &b.u2, &b.v2, &b.w2};
+    for (int q = 0; q < 11; q++) {
+        *hp[q] = buf(hn[q], C.nh, true);
+        if (!*hp[q]) return fail(NTUST_E_ALLOC, "device allocation of %s failed", hn[q]);
+    }
+    const char *in[5] = {"pre", "uc", "vc", "wc", "visc"};
+    double **ip[5] = {&b.pre, &b.uc, &b.vc, &b.wc, &b.vis};
+    for (int q = 0; q < 5; q++) {
+        *ip[q] = buf(in[q], C.ni, true);
+        if (!*ip[q]) return fail(NTUST_E_ALLOC, "device allocation of %s failed", in[q]);
+    }
+    if (force) {
+        NEED(a, "FY", C.nh, true);
+        NEED(c, "FZ", C.nh, true);
+        NEED(d, "force_xz", (size_t)(C.nx + 2) * (C.ny + 2), true);
+        NEED(e, "force_yz", (size_t)(C.nx + 2) * (C.ny + 2), true);
+        NEED(f, "force_xy", (size_t)(C.nx + 2), true);
+        NEED(h, "force_yy", (size_t)(C.nx + 2), true);
+        NEED(k, "force_coef", 2, true);
+        b.FY = a; b.FZ = c; b.fxz = d; b.fyz = e; b.fxy = f; b.fyy = h; b.coef = k;
+    }
+    if (C.prm.les) TRY(filter_table(C.prm.Cs));
+    NEED(filt, "filt", C.ni, false);
+    b.filt = filt;
+    return 0;
+}
+
+// One step of main.f90:209-259 on the device state.  force: compute FY, FZ
+// and the force integral (synchronises for cd/cl); centre: write uc, vc, wc.
+int one_step(const StepBufs &b, bool force, bool centre, int *ik, double *pm, double *cd,
+             double *cl)
+{
+    const Params &q = C.prm;
+    const KGrid &g = C.g;
+    // main.f90:209 -- the reference evaluates nu_t every step and scales it
+    // by LES; with LES = 0 the product is nu + 0 for any finite nu_t, so the
+    // evaluation is skipped.
+    if (q.les)
+        launch_smagorinsky(g, b.u, b.v, b.w, nullptr, nullptr, nullptr, b.filt, b.vis, nullptr,
+                           nullptr, nullptr, 1, C.nz, C.st);
+    // main.f90:213
+    if (q.scheme == NTUST_QUICK_CENTRE)
+        launch_predict_quick(g, b.u, b.v, b.w, q.les ? b.vis : nullptr, q.dt, q.nu, q.les, b.us,
+                             b.vs, b.ws, 1, C.nz, C.st);
+    else
+        launch_predict_upwind(g, b.u, b.v, b.w, q.dt, q.nu, b.us, b.vs, b.ws, 1, C.st);
+    // body force (not in the reference): u* = u* + dt*f on the interior
+    if (C.body) launch_axpy_interior(g, q.dt, C.fu, C.fv, C.fw, b.us, b.vs, b.ws, C.st);
+    HIPCHK(hipGetLastError());
+    // main.f90:231
+    TRY(solve_pressure(b.us, b.vs, b.ws, b.p, b.pre, q.dt, q.omega, q.zeta, q.itmax, q.sor, ik,
+                       pm));
+    // main.f90:247,255 -- projection, forcing and the update u = u2
+    launch_project(g, b.us, b.vs, b.ws, b.p, b.eta, 0.0, 0.0, 0.0, q.dt, nullptr, nullptr,
+                   nullptr, nullptr, nullptr, nullptr, nullptr, force ? b.FY : nullptr,
+                   force ? b.FZ : nullptr, centre ? b.uc : nullptr, centre ? b.vc : nullptr,
+                   centre ? b.wc : nullptr, b.u, b.v, b.w, b.u2, b.v2, b.w2, 1, C.nz, C.st);
+    // main.f90:251
+    if (force) {
+        launch_force_integral(g, b.FY, b.FZ, q.lx, b.fxz, b.fyz, b.fxy, b.fyy, b.coef, C.st);
+        double h[2];
+        HIPCHK(hipMemcpyAsync(h, b.coef, sizeof h, hipMemcpyDeviceToHost, C.st));
+        TRY(sync());
+        *cd = h[0];
+        *cl = h[1];
+    }
+    // main.f90:259
+    launch_boundary_conditions(g, C.bc, b.u, b.v, b.w, b.p, C.st);
+    launch_copy_shell(g, b.u, b.v, b.w, b.us, b.vs, b.ws, b.u2, b.v2, b.w2, C.st);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// ---- discrete adjoint ------------------------------------------------------
+
+struct AdjBufs {
+    double *lu, *lv, *lw, *lp, *lus, *lvs, *lws, *lb, *lvisc, *visc, *gu, *gv, *gw;
+    double *z0, *z1, *Z0, *Z1;
+};
+
+int adj_buffers(AdjBufs &a)
+{
+    const char *hn[11] = {"adj_lu", "adj_lv", "adj_lw", "adj_lp", "adj_lus", "adj_lvs",
+                          "adj_lws", "adj_lb", "adj_gu", "adj_gv", "adj_gw"};
+    double **hp[11] = {&a.lu, &a.lv, &a.lw, &a.lp, &a.lus, &a.lvs, &a.lws, &a.lb,
+                       &a.gu, &a.gv, &a.gw};
+    for (int q = 0; q < 11; q++) {
+        *hp[q] = buf(hn[q], C.nh, true);
+        if (!*hp[q]) return fail(NTUST_E_ALLOC, "device allocation of %s failed", hn[q]);
+    }
+    const char *in[2] = {"adj_lvisc", "adj_visc"};
+    double **ip[2] = {&a.lvisc, &a.visc};
+    for (int q = 0; q < 2; q++) {
+        *ip[q] = buf(in[q], C.ni, true);
+        if (!*ip[q]) return fail(NTUST_E_ALLOC, "device allocation of %s failed", in[q]);
+    }
+    // colour planes: only interior cells are ever written, the ghost and
+    // padding entries stay zero from the allocation
+    const char *sn[4] = {"adj_z0", "adj_z1", "adj_Z0", "adj_Z1"};
+    double **sp[4] = {&a.z0, &a.z1, &a.Z0, &a.Z1};
+    for (int q = 0; q < 4; q++) {
+        *sp[q] = buf(sn[q], (size_t)C.sg.plane, true);
+        if (!*sp[q]) return fail(NTUST_E_ALLOC, "device allocation of %s failed", sn[q]);
+    }
+    return 0;
+}
+
+int adjoint_ready()
+{
+    if (C.prm.sor != NTUST_SOR_REDBLACK && C.prm.sor != NTUST_SOR_MIXED)
+        return fail(NTUST_E_ARG, "the adjoint needs the red-black or mixed pressure solver "
+                                 "(ntust_configure sor_mode 1 or 2)");
+    if (!(C.prm.omega > 0.0)) return fail(NTUST_E_ARG, "the adjoint needs omega > 0");
+    return 0;
+}
+
+// lambda (adjoint of the state after a step) -> adjoint of the state before
+// it.  (ui, vi, wi) is the velocity at the start of the step, sweeps the
+// step's SOR sweep count.  Adds dt*lambda_u* to the body-force gradient.
+int adjoint_of_step(const AdjBufs &a, const double *eta, const double *filt, const double *ui,
+                    const double *vi, const double *wi, int sweeps)
+{
+    const Params &q = C.prm;
+    const KGrid &g = C.g;
+    const double *rx = C.recip[0] + 1, *ry = C.recip[1] + 1, *rz = C.recip[2] + 1;
+    launch_adj_boundary_conditions(g, C.bc, a.lu, a.lv, a.lw, a.lp, C.st);
+    launch_adj_project(g, a.lu, a.lv, a.lw, eta, q.dt, a.lus, a.lvs, a.lws, a.lp, a.lu, a.lv,
+                       a.lw, C.st);
+    if (sweeps > 0)
+        launch_adj_pressure(g, C.sg, rx, ry, rz, q.omega, sweeps, a.lp, a.lb, a.z0, a.z1, a.Z0,
+                            a.Z1, C.st);
+    else
+        HIPCHK(hipMemsetAsync(a.lb, 0, C.nh * sizeof(double), C.st));
+    launch_adj_divergence(g, a.lb, q.dt, a.lus, a.lvs, a.lws, a.lu, a.lv, a.lw, C.st);
+    launch_axpy_interior(g, q.dt, a.lus, a.lvs, a.lws, a.gu, a.gv, a.gw, C.st);
+    const bool quick = q.scheme == NTUST_QUICK_CENTRE;
+    const bool les = quick && q.les;
+    if (les) {
+        launch_smagorinsky(g, ui, vi, wi, nullptr, nullptr, nullptr, filt, a.visc, nullptr,
+                           nullptr, nullptr, 1, C.nz, C.st);
+        HIPCHK(hipMemsetAsync(a.lvisc, 0, C.ni * sizeof(double), C.st));
+    }
+    launch_adj_predict(g, quick ? 0 : 1, ui, vi, wi, les ? a.visc : nullptr, q.dt, q.nu, q.les,
+                       a.lus, a.lvs, a.lws, a.lu, a.lv, a.lw, les ? a.lvisc : nullptr, C.st);
+    if (les) launch_adj_smagorinsky(g, ui, vi, wi, filt, a.lvisc, a.lu, a.lv, a.lw, C.st);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int kinetic_energy(const double *u, const double *v, const double *w, double *ke)
+{
+    const size_t nb = (size_t)((C.nx + 63) / 64) * ((C.ny + 3) / 4) * C.nz;
+    NEED(part, "ke_part", nb, false);
+    const int n = launch_kinetic_energy(C.g, u, v, w, part, C.st);
+    HIPCHK(hipGetLastError());
+    std::vector<double> h((size_t)n);
+    TRY(d2h(h.data(), part, (size_t)n));
+    TRY(sync());
+    double s = 0.0;
+    for (double x : h) s += x;
+    *ke = s;
+    return 0;
+}
+
+int copy3(double *du, double *dv, double *dw, const double *su, const double *sv,
+          const double *sw)
+{
+    const size_t bytes = C.nh * sizeof(double);
+    HIPCHK(hipMemcpyAsync(du, su, bytes, hipMemcpyDeviceToDevice, C.st));
+    HIPCHK(hipMemcpyAsync(dv, sv, bytes, hipMemcpyDeviceToDevice, C.st));
+    HIPCHK(hipMemcpyAsync(dw, sw, bytes, hipMemcpyDeviceToDevice, C.st));
     return 0;
 }
 
@@ -744,85 +935,298 @@ int ntust_step(int nsteps, int *ik, double *pchange, double *cdrag, double *clif
     NEED_INIT();
     if (nsteps < 0) return fail(NTUST_E_ARG, "nsteps < 0");
     const Params &q = C.prm;
-    const KGrid &g = C.g;
-    NEED(u, "u", C.nh, true);
-    NEED(v, "v", C.nh, true);
-    NEED(w, "w", C.nh, true);
-    NEED(p, "p", C.nh, true);
-    NEED(eta, "eta", C.nh, true);
-    NEED(us, "us", C.nh, true);
-    NEED(vs, "vs", C.nh, true);
-    NEED(ws, "ws", C.nh, true);
-    NEED(u2, "u2", C.nh, true);
-    NEED(v2, "v2", C.nh, true);
-    NEED(w2, "w2", C.nh, true);
-    NEED(pre, "pre", C.ni, true);
-    NEED(uc, "uc", C.ni, true);
-    NEED(vc, "vc", C.ni, true);
-    NEED(wc, "wc", C.ni, true);
-    NEED(vis, "visc", C.ni, true);
-    double *FY = nullptr, *FZ = nullptr, *fxz = nullptr, *fyz = nullptr, *fxy = nullptr,
-           *fyy = nullptr, *coef = nullptr;
-    if (q.force_every > 0) {
-        NEED(a, "FY", C.nh, true);
-        NEED(b, "FZ", C.nh, true);
-        NEED(c, "force_xz", (size_t)(C.nx + 2) * (C.ny + 2), true);
-        NEED(d, "force_yz", (size_t)(C.nx + 2) * (C.ny + 2), true);
-        NEED(e, "force_xy", (size_t)(C.nx + 2), true);
-        NEED(f, "force_yy", (size_t)(C.nx + 2), true);
-        NEED(h, "force_coef", 2, true);
-        FY = a; FZ = b; fxz = c; fyz = d; fxy = e; fyy = f; coef = h;
-    }
-    if (q.les) TRY(filter_table(q.Cs));
-    NEED(filt, "filt", C.ni, false);
+    StepBufs b;
+    TRY(step_buffers(b, q.force_every > 0));
     int warn = 0;
     for (int s = 0; s < nsteps; s++) {
-        const bool last = (s == nsteps - 1);
-        // main.f90:209 -- the reference evaluates nu_t every step and scales
-        // it by LES; with LES = 0 the product is nu + 0 for any finite nu_t,
-        // so the evaluation is skipped.
-        if (q.les)
-            launch_smagorinsky(g, u, v, w, nullptr, nullptr, nullptr, filt, vis, nullptr,
-                               nullptr, nullptr, 1, C.nz, C.st);
-        // main.f90:213
-        if (q.scheme == NTUST_QUICK_CENTRE)
-            launch_predict_quick(g, u, v, w, q.les ? vis : nullptr, q.dt, q.nu, q.les, us, vs,
-                                 ws, 1, C.nz, C.st);
-        else
-            launch_predict_upwind(g, u, v, w, q.dt, q.nu, us, vs, ws, 1, C.st);
-        HIPCHK(hipGetLastError());
-        // main.f90:231
+        const bool force = q.force_every > 0 && ((s + 1) % q.force_every == 0);
         int n = 0;
-        double pm = 0.0;
-        TRY(solve_pressure(us, vs, ws, p, pre, q.dt, q.omega, q.zeta, q.itmax, q.sor, &n, &pm));
+        double pm = 0.0, cd = NAN, cl = NAN;
+        TRY(one_step(b, force, s == nsteps - 1, &n, &pm, &cd, &cl));
         if ((double)n >= q.itmax && !(pm <= q.zeta)) warn = NTUST_WARN_ITMAX;
         if (ik) ik[s] = n;
         if (pchange) pchange[s] = pm;
-        // main.f90:247,255 -- projection, forcing and the update u = u2
-        const bool force = q.force_every > 0 && ((s + 1) % q.force_every == 0);
-        launch_project(g, us, vs, ws, p, eta, 0.0, 0.0, 0.0, q.dt, nullptr, nullptr, nullptr,
-                       nullptr, nullptr, nullptr, nullptr, force ? FY : nullptr,
-                       force ? FZ : nullptr, last ? uc : nullptr, last ? vc : nullptr,
-                       last ? wc : nullptr, u, v, w, u2, v2, w2, 1, C.nz, C.st);
-        // main.f90:251
-        if (force) {
-            launch_force_integral(g, FY, FZ, q.lx, fxz, fyz, fxy, fyy, coef, C.st);
-            double h[2];
-            HIPCHK(hipMemcpyAsync(h, coef, sizeof h, hipMemcpyDeviceToHost, C.st));
-            TRY(sync());
-            if (cdrag) cdrag[s] = h[0];
-            if (clift) clift[s] = h[1];
-        } else {
-            if (cdrag) cdrag[s] = NAN;
-            if (clift) clift[s] = NAN;
-        }
-        // main.f90:259
-        launch_boundary_conditions(g, C.bc, u, v, w, p, C.st);
-        launch_copy_shell(g, u, v, w, us, vs, ws, u2, v2, w2, C.st);
-        HIPCHK(hipGetLastError());
+        if (cdrag) cdrag[s] = cd;
+        if (clift) clift[s] = cl;
     }
     TRY(sync());
     return warn;
+}
+
+// ---------------------------------------------------------------------
+// discrete adjoint of the resident step
+// ---------------------------------------------------------------------
+
+int ntust_set_body_force(const double *fu, const double *fv, const double *fw)
+{
+    NEED_INIT();
+    if (!fu && !fv && !fw) {
+        C.body = false;
+        return 0;
+    }
+    if (!fu || !fv || !fw) return fail(NTUST_E_ARG, "fu, fv, fw: give all three or none");
+    NEED(a, "body_u", C.nh, true);
+    NEED(b, "body_v", C.nh, true);
+    NEED(c, "body_w", C.nh, true);
+    TRY(h2d(a, fu, C.nh));
+    TRY(h2d(b, fv, C.nh));
+    TRY(h2d(c, fw, C.nh));
+    C.fu = a; C.fv = b; C.fw = c;
+    C.body = true;
+    return sync();
+}
+
+int ntust_kinetic_energy(double *ke)
+{
+    NEED_INIT();
+    if (!ke) return fail(NTUST_E_ARG, "ke is NULL");
+    NEED(u, "u", C.nh, true);
+    NEED(v, "v", C.nh, true);
+    NEED(w, "w", C.nh, true);
+    return kinetic_energy(u, v, w, ke);
+}
+
+int ntust_checkpoint_reserve(int nslots)
+{
+    NEED_INIT();
+    if (nslots < 0) return fail(NTUST_E_ARG, "nslots < 0");
+    if (nslots == 0) return 0;
+    NEED(ck, "ckpt", (size_t)nslots * 4 * C.nh, false);
+    (void)ck;
+    if (nslots > C.nckpt) C.nckpt = nslots;
+    return 0;
+}
+
+int ntust_checkpoint_save(int slot)
+{
+    NEED_INIT();
+    if (slot < 0 || slot >= C.nckpt)
+        return fail(NTUST_E_ARG, "checkpoint slot %d outside 0..%d", slot, C.nckpt - 1);
+    StepBufs b;
+    TRY(step_buffers(b, false));
+    NEED(ck, "ckpt", (size_t)C.nckpt * 4 * C.nh, false);
+    double *d = ck + (size_t)slot * 4 * C.nh;
+    TRY(copy3(d, d + C.nh, d + 2 * C.nh, b.u, b.v, b.w));
+    HIPCHK(hipMemcpyAsync(d + 3 * C.nh, b.p, C.nh * sizeof(double), hipMemcpyDeviceToDevice,
+                          C.st));
+    return sync();
+}
+
+int ntust_checkpoint_load(int slot)
+{
+    NEED_INIT();
+    if (slot < 0 || slot >= C.nckpt)
+        return fail(NTUST_E_ARG, "checkpoint slot %d outside 0..%d", slot, C.nckpt - 1);
+    StepBufs b;
+    TRY(step_buffers(b, false));
+    NEED(ck, "ckpt", (size_t)C.nckpt * 4 * C.nh, false);
+    const double *d = ck + (size_t)slot * 4 * C.nh;
+    TRY(copy3(b.u, b.v, b.w, d, d + C.nh, d + 2 * C.nh));
+    HIPCHK(hipMemcpyAsync(b.p, d + 3 * C.nh, C.nh * sizeof(double), hipMemcpyDeviceToDevice,
+                          C.st));
+    launch_copy_shell(C.g, b.u, b.v, b.w, b.us, b.vs, b.ws, b.u2, b.v2, b.w2, C.st);
+    HIPCHK(hipGetLastError());
+    return sync();
+}
+
+int ntust_adjoint_set(const double *lu, const double *lv, const double *lw, const double *lp)
+{
+    NEED_INIT();
+    AdjBufs a;
+    TRY(adj_buffers(a));
+    const double *src[4] = {lu, lv, lw, lp};
+    double *dst[4] = {a.lu, a.lv, a.lw, a.lp};
+    for (int q = 0; q < 4; q++) {
+        if (src[q]) TRY(h2d(dst[q], src[q], C.nh));
+        else HIPCHK(hipMemsetAsync(dst[q], 0, C.nh * sizeof(double), C.st));
+    }
+    double *gr[3] = {a.gu, a.gv, a.gw};
+    for (int q = 0; q < 3; q++) HIPCHK(hipMemsetAsync(gr[q], 0, C.nh * sizeof(double), C.st));
+    return sync();
+}
+
+int ntust_adjoint_get(double *lu, double *lv, double *lw, double *lp)
+{
+    NEED_INIT();
+    AdjBufs a;
+    TRY(adj_buffers(a));
+    if (lu) TRY(d2h(lu, a.lu, C.nh));
+    if (lv) TRY(d2h(lv, a.lv, C.nh));
+    if (lw) TRY(d2h(lw, a.lw, C.nh));
+    if (lp) TRY(d2h(lp, a.lp, C.nh));
+    return sync();
+}
+
+int ntust_adjoint_gradient(double *gfu, double *gfv, double *gfw)
+{
+    NEED_INIT();
+    AdjBufs a;
+    TRY(adj_buffers(a));
+    if (gfu) TRY(d2h(gfu, a.gu, C.nh));
+    if (gfv) TRY(d2h(gfv, a.gv, C.nh));
+    if (gfw) TRY(d2h(gfw, a.gw, C.nh));
+    return sync();
+}
+
+int ntust_adjoint_add_ke(double scale)
+{
+    NEED_INIT();
+    AdjBufs a;
+    TRY(adj_buffers(a));
+    StepBufs b;
+    TRY(step_buffers(b, false));
+    launch_ke_gradient(C.g, b.u, b.v, b.w, scale, a.lu, a.lv, a.lw, C.st);
+    HIPCHK(hipGetLastError());
+    return sync();
+}
+
+int ntust_adjoint_step(int *ik)
+{
+    NEED_INIT();
+    TRY(adjoint_ready());
+    StepBufs b;
+    TRY(step_buffers(b, false));
+    AdjBufs a;
+    TRY(adj_buffers(a));
+    NEED(iu, "adj_in_u", C.nh, false);
+    NEED(iv, "adj_in_v", C.nh, false);
+    NEED(iw, "adj_in_w", C.nh, false);
+    NEED(ip, "adj_in_p", C.nh, false);
+    TRY(copy3(iu, iv, iw, b.u, b.v, b.w));
+    HIPCHK(hipMemcpyAsync(ip, b.p, C.nh * sizeof(double), hipMemcpyDeviceToDevice, C.st));
+    int n = 0;
+    double pm = 0.0, cd = NAN, cl = NAN;
+    TRY(one_step(b, false, false, &n, &pm, &cd, &cl));
+    TRY(adjoint_of_step(a, b.eta, b.filt, iu, iv, iw, n));
+    // the state is left as it was
+    TRY(copy3(b.u, b.v, b.w, iu, iv, iw));
+    HIPCHK(hipMemcpyAsync(b.p, ip, C.nh * sizeof(double), hipMemcpyDeviceToDevice, C.st));
+    launch_copy_shell(C.g, b.u, b.v, b.w, b.us, b.vs, b.ws, b.u2, b.v2, b.w2, C.st);
+    HIPCHK(hipGetLastError());
+    if (ik) *ik = n;
+    return sync();
+}
+
+int ntust_adjoint_run(int nsteps, int checkpoint_every, int navg, double *J, int *ik,
+                      double *ms)
+{
+    NEED_INIT();
+    TRY(adjoint_ready());
+    if (nsteps < 1) return fail(NTUST_E_ARG, "nsteps must be >= 1");
+    if (checkpoint_every < 1) return fail(NTUST_E_ARG, "checkpoint_every must be >= 1");
+    if (navg < 0 || navg > nsteps) return fail(NTUST_E_ARG, "navg outside 0..nsteps");
+    const int N = nsteps, K = checkpoint_every < N ? checkpoint_every : N;
+    const int nseg = (N + K - 1) / K;
+    StepBufs b;
+    TRY(step_buffers(b, false));
+    AdjBufs a;
+    TRY(adj_buffers(a));
+    NEED(ck, "ckpt", (size_t)nseg * 4 * C.nh, false);
+    if (nseg > C.nckpt) C.nckpt = nseg;
+    // velocity at the start of every step of one segment, and after its last
+    NEED(seg, "adj_seg", (size_t)(K + 1) * 3 * C.nh, false);
+    NEED(fin, "adj_final", 4 * C.nh, false);
+    const size_t nh = C.nh;
+    auto slot = [&](int n) { return ck + (size_t)n * 4 * nh; };
+    auto segv = [&](int j) { return seg + (size_t)j * 3 * nh; };
+    std::vector<int> ikf((size_t)N), ikr((size_t)N);
+    const int last0 = (nseg - 1) * K;
+    const double wJ = navg > 0 ? 1.0 / navg : 0.0;
+    double Jsum = 0.0;
+    double t_fwd = 0.0, t_rec = 0.0, t_adj = 0.0;
+    auto now = [] { return std::chrono::steady_clock::now(); };
+    auto msec = [](std::chrono::steady_clock::time_point x, std::chrono::steady_clock::time_point y) {
+        return std::chrono::duration<double, std::milli>(y - x).count();
+    };
+
+    // forward pass: checkpoints at segment starts; the last segment's
+    // velocities are kept, so that segment needs no recomputation
+    TRY(sync());
+    auto t0 = now();
+    for (int n = 0; n < N; n++) {
+        if (n % K == 0) {
+            double *d = slot(n / K);
+            TRY(copy3(d, d + nh, d + 2 * nh, b.u, b.v, b.w));
+            HIPCHK(hipMemcpyAsync(d + 3 * nh, b.p, nh * sizeof(double), hipMemcpyDeviceToDevice,
+                                  C.st));
+        }
+        if (n >= last0) {
+            double *d = segv(n - last0);
+            TRY(copy3(d, d + nh, d + 2 * nh, b.u, b.v, b.w));
+        }
+        double pm = 0.0, cd = NAN, cl = NAN;
+        TRY(one_step(b, false, n == N - 1, &ikf[n], &pm, &cd, &cl));
+        if (n >= N - navg) {
+            double ke = 0.0;
+            TRY(kinetic_energy(b.u, b.v, b.w, &ke));
+            Jsum += ke;
+        }
+    }
+    {
+        double *d = segv(N - last0);
+        TRY(copy3(d, d + nh, d + 2 * nh, b.u, b.v, b.w));
+        TRY(copy3(fin, fin + nh, fin + 2 * nh, b.u, b.v, b.w));
+        HIPCHK(hipMemcpyAsync(fin + 3 * nh, b.p, nh * sizeof(double), hipMemcpyDeviceToDevice,
+                              C.st));
+    }
+    TRY(sync());
+    auto t1 = now();
+    t_fwd = msec(t0, t1);
+
+    // reverse pass
+    double *clear[7] = {a.lu, a.lv, a.lw, a.lp, a.gu, a.gv, a.gw};
+    for (double *x : clear) HIPCHK(hipMemsetAsync(x, 0, nh * sizeof(double), C.st));
+    for (int sg = nseg - 1; sg >= 0; sg--) {
+        const int s0 = sg * K, s1 = s0 + K < N ? s0 + K : N;
+        if (sg != nseg - 1) {
+            auto r0 = now();
+            const double *d = slot(sg);
+            TRY(copy3(b.u, b.v, b.w, d, d + nh, d + 2 * nh));
+            HIPCHK(hipMemcpyAsync(b.p, d + 3 * nh, nh * sizeof(double), hipMemcpyDeviceToDevice,
+                                  C.st));
+            launch_copy_shell(C.g, b.u, b.v, b.w, b.us, b.vs, b.ws, b.u2, b.v2, b.w2, C.st);
+            for (int n = s0; n < s1; n++) {
+                double *e = segv(n - s0);
+                TRY(copy3(e, e + nh, e + 2 * nh, b.u, b.v, b.w));
+                double pm = 0.0, cd = NAN, cl = NAN;
+                TRY(one_step(b, false, false, &ikr[n], &pm, &cd, &cl));
+                if (ikr[n] != ikf[n])
+                    return fail(NTUST_E_STATE,
+                                "step %d recomputed from its checkpoint ran %d SOR sweeps, "
+                                "the forward pass %d", n, ikr[n], ikf[n]);
+            }
+            double *e = segv(s1 - s0);
+            TRY(copy3(e, e + nh, e + 2 * nh, b.u, b.v, b.w));
+            TRY(sync());
+            t_rec += msec(r0, now());
+        }
+        auto a0 = now();
+        for (int n = s1 - 1; n >= s0; n--) {
+            if (n >= N - navg) {
+                const double *e = segv(n - s0 + 1);
+                launch_ke_gradient(C.g, e, e + nh, e + 2 * nh, wJ, a.lu, a.lv, a.lw, C.st);
+            }
+            const double *e = segv(n - s0);
+            TRY(adjoint_of_step(a, b.eta, b.filt, e, e + nh, e + 2 * nh, ikf[n]));
+        }
+        TRY(sync());
+        t_adj += msec(a0, now());
+    }
+    // leave the final forward state on the device
+    TRY(copy3(b.u, b.v, b.w, fin, fin + nh, fin + 2 * nh));
+    HIPCHK(hipMemcpyAsync(b.p, fin + 3 * nh, nh * sizeof(double), hipMemcpyDeviceToDevice, C.st));
+    launch_copy_shell(C.g, b.u, b.v, b.w, b.us, b.vs, b.ws, b.u2, b.v2, b.w2, C.st);
+    HIPCHK(hipGetLastError());
+    TRY(sync());
+    if (J) *J = Jsum * wJ;
+    if (ik) for (int n = 0; n < N; n++) ik[n] = ikf[n];
+    if (ms) {
+        ms[0] = t_fwd;
+        ms[1] = t_rec + t_adj;
+        ms[2] = t_rec;
+        ms[3] = t_adj;
+    }
+    return 0;
 }
 
 void *ntust_stream(void) { return (void *)C.st; }

@@ -155,4 +155,34 @@ void launch_copy_shell(const KGrid &g, const double *u, const double *v, const d
                        double *a0, double *a1, double *a2, double *b0, double *b1,
                        double *b2, hipStream_t st);
 
+// discrete adjoint of the resident step (transposed stages, reverse order)
+void launch_adj_boundary_conditions(const KGrid &g, const BcTable &bc, double *lu, double *lv,
+                                    double *lw, double *lp, hipStream_t st);
+void launch_adj_project(const KGrid &g, const double *lu, const double *lv, const double *lw,
+                        const double *eta, double dt, double *lus, double *lvs, double *lws,
+                        double *lp, double *lu_clear, double *lv_clear, double *lw_clear,
+                        hipStream_t st);
+void launch_adj_pressure(const KGrid &g, const SplitGrid &s, const double *rx, const double *ry,
+                         const double *rz, double omega, int sweeps, double *lp, double *lb,
+                         double *z0, double *z1, double *Z0, double *Z1, hipStream_t st);
+void launch_adj_divergence(const KGrid &g, const double *lb, double dt, double *lus,
+                           double *lvs, double *lws, double *lu, double *lv, double *lw,
+                           hipStream_t st);
+void launch_adj_predict(const KGrid &g, int upwind, const double *u, const double *v,
+                        const double *w, const double *visc, double dt, double nu, int les,
+                        const double *lus, const double *lvs, const double *lws, double *lu,
+                        double *lv, double *lw, double *lvisc, hipStream_t st);
+void launch_adj_smagorinsky(const KGrid &g, const double *u, const double *v, const double *w,
+                            const double *filt, const double *lvisc, double *lu, double *lv,
+                            double *lw, hipStream_t st);
+// kinetic energy 0.5 sum (u^2+v^2+w^2) iDx iDy iDz over the interior: block
+// partial sums into part[], returns their count
+int launch_kinetic_energy(const KGrid &g, const double *u, const double *v, const double *w,
+                          double *part, hipStream_t st);
+void launch_ke_gradient(const KGrid &g, const double *u, const double *v, const double *w,
+                        double scale, double *lu, double *lv, double *lw, hipStream_t st);
+// X += a x on the interior of three halo arrays
+void launch_axpy_interior(const KGrid &g, double a, const double *x, const double *y,
+                          const double *z, double *X, double *Y, double *Z, hipStream_t st);
+
 }  // namespace ntust

@@ -155,3 +155,24 @@ def noise_ic(grid: Grid, amp: float, seed: int):
         a[2:grid.nx + 2, 2:grid.ny + 2, 2:grid.nz + 2] = rng.uniform(-amp, amp, grid.shape)
         out.append(a)
     return out
+
+
+def gaussian_blob(grid: Grid, amp: float, width: float, centre) -> list:
+    """A body force f = amp*exp(-|x - centre|^2/width^2) on every interior
+    face of each component (u faces at (X(i+1), yc(j), zc(k)), v and w
+    alike; yc, zc cell centres), ghosts zero: three halo arrays."""
+    def axes(N, n):
+        node = N[3:n + 3]                      # X(i+1), i = 1..n
+        cen = 0.5 * (N[2:n + 2] + N[3:n + 3])  # (X(i) + X(i+1))/2
+        return node, cen
+    (xn, xc), (yn, yc), (zn, zc) = (axes(grid.X, grid.nx), axes(grid.Y, grid.ny),
+                                    axes(grid.Z, grid.nz))
+    out = []
+    for comp in range(3):
+        ax = (xn if comp == 0 else xc, yn if comp == 1 else yc, zn if comp == 2 else zc)
+        gx, gy, gz = np.meshgrid(*ax, indexing="ij")
+        r2 = (gx - centre[0]) ** 2 + (gy - centre[1]) ** 2 + (gz - centre[2]) ** 2
+        f = grid.halo()
+        f[2:grid.nx + 2, 2:grid.ny + 2, 2:grid.nz + 2] = amp * np.exp(-r2 / width ** 2)
+        out.append(f)
+    return out

@@ -25,6 +25,7 @@ LIB = Path(__file__).resolve().parent / "libntust_hip.so"
 SOR_LEXICOGRAPHIC, SOR_REDBLACK, SOR_MIXED = 0, 1, 2
 QUICK_CENTRE, UPWIND = 0, 1
 WARN_ITMAX = 1
+E_STATE = -5
 
 _D = ct.POINTER(ct.c_double)
 _I = ct.POINTER(ct.c_int)
@@ -63,6 +64,17 @@ def load_library() -> ct.CDLL:
         "ntust_step": [i, _I, _D, _D, _D],
         "ntust_prof_reset": [],
         "ntust_prof_read": [_D, ct.POINTER(ct.c_longlong)],
+        "ntust_set_body_force": [_D, _D, _D],
+        "ntust_kinetic_energy": [_D],
+        "ntust_checkpoint_reserve": [i],
+        "ntust_checkpoint_save": [i],
+        "ntust_checkpoint_load": [i],
+        "ntust_adjoint_set": [_D] * 4,
+        "ntust_adjoint_get": [_D] * 4,
+        "ntust_adjoint_add_ke": [d],
+        "ntust_adjoint_step": [_I],
+        "ntust_adjoint_gradient": [_D] * 3,
+        "ntust_adjoint_run": [i, i, i, _D, _I, _D],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -213,3 +225,53 @@ class ProjectionSolver:
         n = ct.c_longlong()
         self.lib.ntust_prof_read(ct.byref(ms), ct.byref(n))
         return ms.value, n.value
+
+    # ---- discrete adjoint (no counterpart in the reference) --------------
+
+    def set_body_force(self, fu=None, fv=None, fw=None):
+        """Body force added to the predictor, u* += dt*f (None removes it)."""
+        _check(self.lib.ntust_set_body_force(_p(fu), _p(fv), _p(fw)))
+
+    def kinetic_energy(self) -> float:
+        ke = ct.c_double()
+        _check(self.lib.ntust_kinetic_energy(ct.byref(ke)))
+        return ke.value
+
+    def checkpoint_reserve(self, n: int):
+        _check(self.lib.ntust_checkpoint_reserve(int(n)))
+
+    def checkpoint_save(self, slot: int):
+        _check(self.lib.ntust_checkpoint_save(int(slot)))
+
+    def checkpoint_load(self, slot: int):
+        _check(self.lib.ntust_checkpoint_load(int(slot)))
+
+    def adjoint_set(self, lu=None, lv=None, lw=None, lp=None):
+        _check(self.lib.ntust_adjoint_set(_p(lu), _p(lv), _p(lw), _p(lp)))
+
+    def adjoint_get(self):
+        out = [self.grid.halo() for _ in range(4)]
+        _check(self.lib.ntust_adjoint_get(*[_p(a) for a in out]))
+        return out
+
+    def adjoint_add_ke(self, scale: float):
+        _check(self.lib.ntust_adjoint_add_ke(scale))
+
+    def adjoint_step(self) -> int:
+        ik = ct.c_int()
+        _check(self.lib.ntust_adjoint_step(ct.byref(ik)))
+        return ik.value
+
+    def adjoint_gradient(self):
+        out = [self.grid.halo() for _ in range(3)]
+        _check(self.lib.ntust_adjoint_gradient(*[_p(a) for a in out]))
+        return out
+
+    def adjoint_run(self, nsteps: int, checkpoint_every: int, navg: int):
+        """Returns (J, per-step sweep counts, ms[forward, reverse, recompute, adjoint])."""
+        J = ct.c_double()
+        ik = np.zeros(nsteps, dtype=np.int32)
+        ms = np.zeros(4)
+        _check(self.lib.ntust_adjoint_run(int(nsteps), int(checkpoint_every), int(navg),
+                                          ct.byref(J), ik.ctypes.data_as(_I), _p(ms)))
+        return J.value, ik, ms

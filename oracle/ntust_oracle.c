@@ -931,6 +931,17 @@ int or_step(const or_grid *g, const or_params *prm, or_fields *f,
     else
         or_predictor_upwind(g, f->u, f->v, f->w, prm->dt, prm->nu,
                             f->us, f->vs, f->ws);
+    if (f->fu) {
+        const long sx = g->nx + 4, sxy = sx * (g->ny + 4);
+        for (int k = 1; k <= nz; k++)
+            for (int j = 1; j <= g->ny; j++)
+                for (int i = 1; i <= g->nx; i++) {
+                    const long o = (i + 1) + sx * (j + 1) + sxy * (k + 1);
+                    f->us[o] = f->us[o] + prm->dt * f->fu[o];
+                    f->vs[o] = f->vs[o] + prm->dt * f->fv[o];
+                    f->ws[o] = f->ws[o] + prm->dt * f->fw[o];
+                }
+    }
     /* main.f90:231 */
     int ik;
     if (prm->sor == 0) {

@@ -139,12 +139,16 @@ void or_poisson_coeffs(const or_grid *g, double *coef, int *jcoef);
 /* One time step in main.f90:139-259 order (1 rank: istart=1, iend=nz).
  * scheme: 0 QUICK_centre, 1 upwind.  sor: 0 lexicographic (GS.f90),
  * 1 red-black (or_sor_redblack).  The LES factor table (Cs*delta)**2 is
- * recomputed from Cs.  Returns ik; cd/cl get the force coefficients. */
+ * recomputed from Cs.  Returns ik; cd/cl get the force coefficients.
+ * A body force (fu, fv, fw non-NULL; not in the reference, whose momentum
+ * equation has none) is added after the predictor: us = us + dt*fu on the
+ * interior, v and w alike. */
 typedef struct {
     double *u, *v, *w, *p, *ETA;                 /* halo, state */
     double *u1, *v1, *w1, *u2, *v2, *w2;         /* halo */
     double *us, *vs, *ws, *FX, *FY, *FZ;         /* halo */
     double *Viseff, *uc, *vc, *wc, *pre;         /* interior */
+    const double *fu, *fv, *fw;                  /* halo, body force or NULL */
 } or_fields;
 
 typedef struct {

@@ -29,7 +29,7 @@ class OrGrid(ct.Structure):
 class OrFields(ct.Structure):
     _fields_ = [(n, _D) for n in ("u", "v", "w", "p", "ETA", "u1", "v1", "w1", "u2", "v2", "w2",
                                   "us", "vs", "ws", "FX", "FY", "FZ", "Viseff", "uc", "vc",
-                                  "wc", "pre")]
+                                  "wc", "pre", "fu", "fv", "fw")]
 
 
 class OrParams(ct.Structure):
@@ -165,7 +165,7 @@ class OracleRun:
                                    f["w1"], f["u2"], f["v2"], f["w2"], f["us"], f["vs"], f["ws"])
 
     def step(self):
-        fs = OrFields(*[_p(self.f[n]) for n, _ in OrFields._fields_])
+        fs = OrFields(*[_p(self.f.get(n)) for n, _ in OrFields._fields_])
         cd, cl, pm = ct.c_double(), ct.c_double(), ct.c_double()
         ik = _lib.or_step(ct.byref(self.o.g), ct.byref(self.prm), ct.byref(fs), ct.byref(cd),
                           ct.byref(cl), ct.byref(pm))
