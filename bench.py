@@ -79,6 +79,87 @@ def cpu_baseline(grid, w, state, sweeps_per_step):
                        f"{sweeps_per_step:.0f} sweeps/step")}
 
 
+def adjoint_workload(n=96):
+    """Adjoint case: Re=400 LES cavity on n^3, QUICK, Cs=0.1, FP64 red-black SOR
+    (omega 1.5, zeta 1e-10, itmax 3000), dt 5e-4, from rest; body force a
+    Gaussian blob of amplitude 0.1 and width 0.1 at a seeded centre;
+    objective the kinetic energy averaged over the last 32 of 128 steps;
+    checkpoints every 16 steps."""
+    return dict(n=n, Re=400.0, dt=5e-4, Cs=0.1, omega=1.5, zeta=1e-10, itmax=3000.0,
+                steps=128, every=16, navg=32, amp=0.1, width=0.1, seed=96)
+
+
+def adjoint_leg(hip, hiptime, n, steps, fd_dirs):
+    """Forward + checkpointed reverse pass; cost ratio against a plain
+    forward of the same steps; central finite differences (step 1e-6) of
+    the plain forward along seeded random directions of the body force."""
+    from ntustcfdlab_amd.grid import Grid, bc_codes, gaussian_blob
+    a = adjoint_workload(n)
+    a["steps"] = steps
+    a["navg"] = min(a["navg"], steps)
+    a["every"] = min(a["every"], steps)
+    g = Grid(n, n, n)
+    rng = np.random.default_rng(a["seed"])
+    centre = rng.uniform(0.3, 0.7, 3)
+    f = gaussian_blob(g, a["amp"], a["width"], centre)
+    s = hip.ProjectionSolver(g, bc_codes())
+    try:
+        s.configure(dt=a["dt"], nu=1.0 / a["Re"], Cs=a["Cs"], LES=1, scheme=hip.QUICK_CENTRE,
+                    omega=a["omega"], zeta=a["zeta"], itmax=a["itmax"], sor=hip.SOR_REDBLACK,
+                    lx=1.0, force_every=0)
+        rest = [g.halo() for _ in range(4)]
+        s.boundary_conditions(*rest)
+        eta = g.halo()
+        s.set_body_force(*f)
+
+        def plain(fu):
+            s.set_body_force(*fu)
+            s.upload(*[x.copy(order="F") for x in rest], eta)
+            e0, e1 = hiptime.Event(), hiptime.Event()
+            hiptime.device_synchronize()
+            e0.record(s.stream())
+            tot = 0.0
+            for k in range(a["steps"]):
+                s.step(1)
+                if k >= a["steps"] - a["navg"]:
+                    tot += s.kinetic_energy()
+            e1.record(s.stream())
+            return tot / a["navg"], e0.elapsed_ms(e1)
+
+        J0, t_plain = plain(f)
+        s.set_body_force(*f)
+        s.upload(*[x.copy(order="F") for x in rest], eta)
+        J, ik, ms = s.adjoint_run(a["steps"], a["every"], a["navg"])
+        gf = s.adjoint_gradient()
+        frng = np.random.default_rng(a["seed"] + 1)
+        fd = []
+        h = 1e-6
+        for _ in range(fd_dirs):
+            d = [g.halo() for _ in range(3)]
+            for x in d:
+                x[2:-2, 2:-2, 2:-2] = frng.standard_normal((n, n, n))
+            jp, _ = plain([x + h * y for x, y in zip(f, d)])
+            jm, _ = plain([x - h * y for x, y in zip(f, d)])
+            fdv = (jp - jm) / (2 * h)
+            adv = float(sum(np.vdot(x, y) for x, y in zip(gf, d)))
+            fd.append({"fd": fdv, "adjoint": adv, "rel_err": abs(fdv - adv) / abs(adv)})
+    finally:
+        s.close()
+    return {
+        "workload": (f"LES cavity Re=400 {n}^3, QUICK, Cs=0.1, dt=5e-4, FP64 red-black SOR "
+                     f"zeta=1e-10 itmax=3000; body force Gaussian amp 0.1 width 0.1; J = mean "
+                     f"KE over the last {a['navg']} of {a['steps']} steps; checkpoint every "
+                     f"{a['every']}"),
+        "J": J, "J_plain_forward": J0, "sweeps_per_step": float(np.mean(ik)),
+        "forward_ms": round(float(ms[0]), 1), "plain_forward_ms": round(t_plain, 1),
+        "reverse_ms": round(float(ms[1]), 1), "recompute_ms": round(float(ms[2]), 1),
+        "adjoint_kernels_ms": round(float(ms[3]), 1),
+        "adjoint_over_forward": round(float(ms[1]) / t_plain, 3),
+        "fd_step": h, "fd": fd,
+        "fd_max_rel_err": max((x["rel_err"] for x in fd), default=None),
+    }
+
+
 def timed_run(s, hip, hiptime, state, sor, steps, warmup):
     """Upload the state, W warm-up steps, then K steps bracketed by HIP
     events on the library stream.  Returns (ms, per-step sweeps, prof, final)."""
@@ -114,6 +195,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate", action="store_true",
                     help="one 256 MiB device copy before timing (counter-profile scaling)")
+    ap.add_argument("--adjoint-size", type=int, default=96)
+    ap.add_argument("--adjoint-steps", type=int, default=128)
+    ap.add_argument("--fd-directions", type=int, default=2)
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--only-adjoint", action="store_true",
+                    help="run the adjoint leg alone and print its JSON")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported: the path runs "
@@ -122,6 +209,11 @@ def main():
 
     from ntustcfdlab_amd import hip, hiptime
     from ntustcfdlab_amd.grid import Grid, bc_codes
+
+    if a.only_adjoint:
+        print(json.dumps({"adjoint": adjoint_leg(hip, hiptime, a.adjoint_size,
+                                                 a.adjoint_steps, a.fd_directions)}))
+        return
 
     w = workload(a.size)
     g = Grid(w["nx"], w["ny"], w["nz"])
@@ -204,6 +296,10 @@ def main():
         "mixed_speedup_vs_fp64": (round(extra["mixed"]["value"] / extra["fp64"]["value"], 3)
                                   if len(extra) == 2 else None),
     }
+    if not a.no_adjoint:
+        print("# adjoint", file=sys.stderr, flush=True)
+        out["adjoint"] = adjoint_leg(hip, hiptime, a.adjoint_size, a.adjoint_steps,
+                                     a.fd_directions)
     if not a.no_cpu_baseline:
         print("# cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(g, w, state, sweeps)
