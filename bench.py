@@ -178,7 +178,7 @@ def timed_run(s, hip, hiptime, state, sor, steps, warmup):
     ms = e0.elapsed_ms(e1)
     hiptime.device_synchronize()
     wall = time.perf_counter() - t0
-    prof = s.prof_read()
+    prof = s.prof_read() + s.prof_kernel()
     final = [g.halo() for _ in range(4)]
     s.download(*final)
     return ms, ik, prof, final, wall
@@ -199,6 +199,8 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=128)
     ap.add_argument("--fd-directions", type=int, default=2)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--sweep", choices=["fused", "two-pass"], default="two-pass",
+                    help="red-black sweep kernel (bit-identical results)")
     ap.add_argument("--only-adjoint", action="store_true",
                     help="run the adjoint leg alone and print its JSON")
     a = ap.parse_args()
@@ -218,6 +220,7 @@ def main():
     w = workload(a.size)
     g = Grid(w["nx"], w["ny"], w["nz"])
     s = hip.ProjectionSolver(g, bc_codes())
+    s.set_sweep_variant(hip.SWEEP_FUSED if a.sweep == "fused" else hip.SWEEP_TWO_PASS)
     if a.calibrate:
         hiptime.copy_calibration()
 
@@ -248,19 +251,18 @@ def main():
             traffic = {k: v["read_bytes"] + v["write_bytes"] for k, v in t["per_launch"].items()}
 
     def summary(name):
-        ms, ik, (prof_ms, launches), final, wall = runs[name]
+        ms, ik, (prof_ms, launches, kid, per_cell), final, wall = runs[name]
         value = cells * a.steps / (ms / 1e3) / 1e6
         roof = None
         if launches:
             avg_ms = prof_ms / launches
-            per_cell = 12.0 if name == "fp64" else 6.0   # algorithmic bytes per half-sweep
             achieved = per_cell * cells / (avg_ms / 1e3) / 1e9
-            tr = traffic.get("k_rb_sweep" if name == "fp64" else "k_rb_sweep32")
+            kname, kdesc = hip.PROF_KERNELS[kid]
+            tr = traffic.get(kname)
             roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": round(tr) if tr else None,
-                    "kernel": "k_rb_sweep<C> (one colour half-sweep, FP64)" if name == "fp64"
-                    else "k_rb_sweep32<C> (one colour half-sweep, FP32 correction)",
+                    "kernel": f"{kname} ({kdesc})",
                     "bytes_per_launch": per_cell * cells,
                     "avg_launch_us": round(avg_ms * 1e3, 3), "launches_timed": launches}
         return value, ms / a.steps, float(np.mean(ik)), roof, wall

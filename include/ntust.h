@@ -48,6 +48,12 @@ enum {
     NTUST_SOR_MIXED = 2           /* red-black, FP32 sweeps inside FP64 defect correction */
 };
 
+/* red-black sweep kernels (ntust_set_sweep_variant); both give the same bits */
+enum {
+    NTUST_SWEEP_TWO_PASS = 0,     /* one launch per colour, in place (default) */
+    NTUST_SWEEP_FUSED = 1         /* one launch per sweep on LDS tiles */
+};
+
 /* advection schemes (ntust_configure scheme) */
 enum {
     NTUST_QUICK_CENTRE = 0,       /* discretisation_QUICK_centre.f90 */
@@ -231,6 +237,15 @@ void *ntust_stream(void);
  * summed milliseconds and the number of half-sweep launches they cover. */
 int ntust_prof_reset(void);
 int ntust_prof_read(double *ms, long long *launches);
+
+/* The timed kernel: returns 0 two-pass FP64, 1 fused FP64, 2 two-pass FP32,
+ * 3 fused FP32 sweep; *bytes_per_cell gets its algorithmic bytes per grid
+ * cell and launch (read p and b, write p). */
+int ntust_prof_kernel(double *bytes_per_cell);
+
+/* Selects the red-black sweep kernel (NTUST_SWEEP_*); not a reference
+ * setting, the iterates are bit-identical either way. */
+int ntust_set_sweep_variant(int variant);
 
 #ifdef __cplusplus
 }

@@ -73,6 +73,8 @@ struct Ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double prof_ms = 0.0;
     long long prof_launches = 0;
+    // red-black sweep kernel: one launch per sweep (fused) or per colour
+    int sweep_variant = NTUST_SWEEP_TWO_PASS;
     // body force added to the predictor (device halo arrays, "body_*")
     bool body = false;
     double *fu = nullptr, *fv = nullptr, *fw = nullptr;
@@ -183,14 +185,25 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
         return fail(NTUST_E_ARG, "unsupported SOR mode %d", mode);
 
     const size_t np = (size_t)C.sg.plane;
+    const bool fused = C.sweep_variant == NTUST_SWEEP_FUSED;
     NEED(P0, "sor_p0", np, true);
     NEED(P1, "sor_p1", np, true);
     NEED(B0, "sor_b0", np, true);
     NEED(B1, "sor_b1", np, true);
+    // the fused FP64 sweep reads one copy of the planes and writes the
+    // other: copy 0 = (P0, P1), copy 1 = (Q0, Q1); sweep t writes copy t&1
+    double *Q0 = nullptr, *Q1 = nullptr;
+    if (fused && mode == NTUST_SOR_REDBLACK) {
+        NEED(a, "sor_q0", np, true);
+        NEED(b, "sor_q1", np, true);
+        Q0 = a;
+        Q1 = b;
+    }
     const long tmax = (long)fmin(itmax, 1.0e8) > 0 ? (long)fmin(itmax, 1.0e8) : 0;
     const size_t nslot = (size_t)slot_index(tmax + 2, 0);
     NEED(slots, "sor_slots", nslot, false);
     launch_rb_gather(g, C.sg, p, P0, P1, C.st);
+    if (Q0) launch_rb_gather(g, C.sg, p, Q0, Q1, C.st);   // ghosts in both copies
     launch_rb_rhs(g, C.sg, us, vs, ws, dt, B0, B1, C.st);
     // sweep 0: 1.0 (pChangeMax_ before the first sweep); sweeps 1.. : 0
     HIPCHK(hipMemsetAsync(slots, 0, nslot * sizeof(double), C.st));
@@ -209,6 +222,16 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
         NEED(r0d, "mix_r0", nf, true);
         NEED(r1d, "mix_r1", nf, true);
         float *E0 = (float *)e0d, *E1 = (float *)e1d, *R0 = (float *)r0d, *R1 = (float *)r1d;
+        // second copy of e for the fused sweeps (ghost entries stay zero)
+        float *F0 = nullptr, *F1 = nullptr;
+        if (fused) {
+            NEED(f0d, "mix_f0", nf, true);
+            NEED(f1d, "mix_f1", nf, true);
+            F0 = (float *)f0d;
+            F1 = (float *)f1d;
+            HIPCHK(hipMemsetAsync(F0, 0, np * sizeof(float), C.st));
+            HIPCHK(hipMemsetAsync(F1, 0, np * sizeof(float), C.st));
+        }
         const long seg = C.mix_segment;
         const long nseg = (tmax + seg - 1) / seg + 1;
         NEED(dd, "mix_dirty", (size_t)nseg, true);
@@ -227,11 +250,24 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
             do {
                 const long n = seg < tmax - t + 1 ? seg : tmax - t + 1;
                 if (!timed) HIPCHK(hipEventRecord(C.ev0, C.st));
-                for (long b = 0; b < n; b++)
-                    launch_rb_sweep32(g, C.sg, fm, E0, E1, R0, R1, (float)omega, zeta, itmax,
-                                      (int)(t + b), slots, C.ctl, dirty + c, C.st);
+                for (long b = 0; b < n; b++) {
+                    const long tt = t + b;
+                    if (fused) {
+                        const bool rd = (tt - 1) & 1, wr = tt & 1;
+                        launch_rb_sweep32_fused(g, C.sg, fm, rd ? F0 : E0, rd ? F1 : E1,
+                                                wr ? F0 : E0, wr ? F1 : E1, R0, R1,
+                                                (float)omega, zeta, itmax, (int)tt, slots,
+                                                C.ctl, dirty + c, C.st);
+                    } else {
+                        launch_rb_sweep32(g, C.sg, fm, E0, E1, R0, R1, (float)omega, zeta,
+                                          itmax, (int)tt, slots, C.ctl, dirty + c, C.st);
+                    }
+                }
                 if (!timed) HIPCHK(hipEventRecord(C.ev1, C.st));
-                launch_rb_absorb(g, C.sg, P0, P1, E0, E1, dirty + c, C.st);
+                if (fused)
+                    launch_rb_absorb_pp(g, C.sg, P0, P1, E0, E1, F0, F1, dirty + c, C.ctl, C.st);
+                else
+                    launch_rb_absorb(g, C.sg, P0, P1, E0, E1, dirty + c, C.st);
                 launch_rb_residual(g, C.sg, rx, ry, rz, P0, P1, B0, B1, R0, R1, dirty + c, C.st);
                 HIPCHK(hipGetLastError());
                 if (!timed) { timed = true; batch = n; }
@@ -245,7 +281,7 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
                 float ms = 0.f;
                 HIPCHK(hipEventElapsedTime(&ms, C.ev0, C.ev1));
                 C.prof_ms += ms;
-                C.prof_launches += 2 * batch;
+                C.prof_launches += fused ? batch : 2 * batch;
             }
             if (h.stop) break;
             want = 2 * seg;
@@ -255,9 +291,18 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
     while (t <= tmax) {
         const long nb = batch < tmax - t + 1 ? batch : tmax - t + 1;
         HIPCHK(hipEventRecord(C.ev0, C.st));
-        for (long b = 0; b < nb; b++)
-            launch_rb_sweep(g, C.sg, rx, ry, rz, P0, P1, B0, B1, omega, zeta, itmax,
-                            (int)(t + b), slots, C.ctl, C.st);
+        for (long b = 0; b < nb; b++) {
+            const long tt = t + b;
+            if (fused) {
+                const bool rd = (tt - 1) & 1, wr = tt & 1;
+                launch_rb_sweep_fused(g, C.sg, rx, ry, rz, rd ? Q0 : P0, rd ? Q1 : P1,
+                                      wr ? Q0 : P0, wr ? Q1 : P1, B0, B1, omega, zeta, itmax,
+                                      (int)tt, slots, C.ctl, C.st);
+            } else {
+                launch_rb_sweep(g, C.sg, rx, ry, rz, P0, P1, B0, B1, omega, zeta, itmax,
+                                (int)tt, slots, C.ctl, C.st);
+            }
+        }
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(C.ev1, C.st));
         HIPCHK(hipMemcpyAsync(&h, C.ctl, sizeof h, hipMemcpyDeviceToHost, C.st));
@@ -266,7 +311,7 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
             float ms = 0.f;
             HIPCHK(hipEventElapsedTime(&ms, C.ev0, C.ev1));
             C.prof_ms += ms;
-            C.prof_launches += 2 * nb;
+            C.prof_launches += fused ? nb : 2 * nb;
         }
         if (h.stop) break;
         t += nb;
@@ -276,7 +321,9 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
     double bk[SLOT_BUCKETS * SLOT_STRIDE];
     HIPCHK(hipMemcpyAsync(bk, slots + slot_index(h.ik, 0), sizeof bk, hipMemcpyDeviceToHost,
                           C.st));
-    launch_rb_scatter(g, C.sg, P0, P1, p, pre, C.st);
+    // the fused FP64 sweeps leave the result in copy ik&1
+    const bool in_q = Q0 && (h.ik & 1);
+    launch_rb_scatter(g, C.sg, in_q ? Q0 : P0, in_q ? Q1 : P1, p, pre, C.st);
     HIPCHK(hipGetLastError());
     TRY(sync());
     double m = 0.0;
@@ -1236,6 +1283,24 @@ int ntust_prof_reset(void)
     C.prof_ms = 0.0;
     C.prof_launches = 0;
     return 0;
+}
+
+int ntust_set_sweep_variant(int variant)
+{
+    if (variant != NTUST_SWEEP_TWO_PASS && variant != NTUST_SWEEP_FUSED)
+        return fail(NTUST_E_ARG, "unknown sweep variant %d", variant);
+    C.sweep_variant = variant;
+    return 0;
+}
+
+int ntust_prof_kernel(double *bytes_per_cell)
+{
+    // algorithmic bytes per grid cell of one timed launch: read p and b,
+    // write p per sweep (24 B FP64, 12 B FP32); a two-pass launch is half
+    const bool mixed = C.prm.sor == NTUST_SOR_MIXED;
+    const bool fused = C.sweep_variant == NTUST_SWEEP_FUSED;
+    if (bytes_per_cell) *bytes_per_cell = (mixed ? 12.0 : 24.0) / (fused ? 1.0 : 2.0);
+    return mixed ? (fused ? 3 : 2) : (fused ? 1 : 0);
 }
 
 int ntust_prof_read(double *ms, long long *launches)

@@ -126,6 +126,21 @@ void launch_rb_sweep32(const KGrid &g, const SplitGrid &s, const FMetric &fm, fl
                        hipStream_t st);
 void launch_rb_absorb(const KGrid &g, const SplitGrid &s, double *p0, double *p1, float *e0,
                       float *e1, const int *dirty, hipStream_t st);
+// one launch per sweep (red then black on LDS tiles), reading copy "in" of
+// the colour planes and writing copy "out"
+void launch_rb_sweep_fused(const KGrid &g, const SplitGrid &s, const double *rx,
+                           const double *ry, const double *rz, const double *in0,
+                           const double *in1, double *out0, double *out1, const double *b0,
+                           const double *b1, double omega, double zeta, double itmax, int t,
+                           double *slots, SorControl *ctl, hipStream_t st);
+void launch_rb_sweep32_fused(const KGrid &g, const SplitGrid &s, const FMetric &fm,
+                             const float *in0, const float *in1, float *out0, float *out1,
+                             const float *r0, const float *r1, float omega, double zeta,
+                             double itmax, int t, double *slots, SorControl *ctl, int *dirty,
+                             hipStream_t st);
+void launch_rb_absorb_pp(const KGrid &g, const SplitGrid &s, double *p0, double *p1,
+                         float *ea0, float *ea1, float *eb0, float *eb1, const int *dirty,
+                         const SorControl *ctl, hipStream_t st);
 void launch_rb_residual(const KGrid &g, const SplitGrid &s, const double *rx, const double *ry,
                         const double *rz, const double *p0, const double *p1, const double *b0,
                         const double *b1, float *r0, float *r1, const int *dirty,

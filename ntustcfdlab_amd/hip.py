@@ -26,6 +26,11 @@ SOR_LEXICOGRAPHIC, SOR_REDBLACK, SOR_MIXED = 0, 1, 2
 QUICK_CENTRE, UPWIND = 0, 1
 WARN_ITMAX = 1
 E_STATE = -5
+SWEEP_TWO_PASS, SWEEP_FUSED = 0, 1
+PROF_KERNELS = {0: ("k_rb_sweep<C>", "two-pass FP64 half-sweep"),
+                1: ("k_rb_sweep_fused<double>", "fused FP64 sweep"),
+                2: ("k_rb_sweep32<C>", "two-pass FP32 half-sweep"),
+                3: ("k_rb_sweep_fused<float>", "fused FP32 sweep")}
 
 _D = ct.POINTER(ct.c_double)
 _I = ct.POINTER(ct.c_int)
@@ -75,6 +80,8 @@ def load_library() -> ct.CDLL:
         "ntust_adjoint_step": [_I],
         "ntust_adjoint_gradient": [_D] * 3,
         "ntust_adjoint_run": [i, i, i, _D, _I, _D],
+        "ntust_set_sweep_variant": [i],
+        "ntust_prof_kernel": [_D],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -219,6 +226,16 @@ class ProjectionSolver:
 
     def prof_reset(self):
         self.lib.ntust_prof_reset()
+
+    def set_sweep_variant(self, variant: int):
+        """SWEEP_TWO_PASS (default) or SWEEP_FUSED; the iterates are bit-identical."""
+        _check(self.lib.ntust_set_sweep_variant(int(variant)))
+
+    def prof_kernel(self):
+        """(kernel id, algorithmic bytes per grid cell of one timed launch)."""
+        b = ct.c_double()
+        k = self.lib.ntust_prof_kernel(ct.byref(b))
+        return k, b.value
 
     def prof_read(self):
         ms = ct.c_double()
