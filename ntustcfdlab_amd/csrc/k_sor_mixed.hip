@@ -35,10 +35,12 @@ k_rb_sweep32(KGrid g, SplitGrid s, FMetric fm, float *__restrict__ e_own,
     const int stopped = ctl->stop;
 
     // same thread layout and load-first structure as the FP64 sweep
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    const int jr = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    int bx, by, bz;
+    xcd_block(bx, by, bz);
+    const int q = bx * blockDim.x + threadIdx.x;
+    const int jr = by * blockDim.y + threadIdx.y + 1;
     const int j = min(jr, g.ny);
-    const int k0 = blockIdx.z * RB_KPT + 1;
+    const int k0 = bz * RB_KPT + 1;
     const int sh0 = (C + 1 + j + k0) & 1;
     const int ia = min(2 * q + 1 + sh0, g.nx + 1), ib = min(2 * q + 2 - sh0, g.nx + 1);
     const float axa = fm.iDx[ia], rEa = fm.rx[ia], rWa = fm.rx[ia - 1];

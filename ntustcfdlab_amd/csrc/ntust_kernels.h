@@ -60,6 +60,25 @@ __device__ __forceinline__ double wave_max(double v)
     return v;
 }
 
+// XCD-aware block order for the stencil sweeps.  The hardware hands the
+// workgroups of a launch to the 8 XCDs round-robin (workgroup n to XCD
+// n mod 8, in order), and each XCD has its own L2.  Renumbering so that the
+// workgroups of XCD x take a contiguous range of logical blocks (x fastest,
+// then y, then z) gives each XCD a slab of the domain, whose halo rows its
+// L2 then serves.  A bijection for any block count; only the speed depends
+// on the dispatch order.
+__device__ __forceinline__ void xcd_block(int &bx, int &by, int &bz)
+{
+    const int nb = gridDim.x * gridDim.y * gridDim.z;
+    const int n = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    const int x = n & 7, idx = n >> 3, per = nb >> 3, rem = nb & 7;
+    const int logical = x * per + (x < rem ? x : rem) + idx;
+    bx = logical % gridDim.x;
+    const int r = logical / gridDim.x;
+    by = r % gridDim.y;
+    bz = r / gridDim.y;
+}
+
 // Boundary-condition codes, one per (face, component) in the order
 //   W_u W_v W_w  E_u E_v E_w  S_u S_v S_w  N_u N_v N_w  B_u B_v B_w  F_u F_v F_w
 enum BcCode { BC_NOSLIP = 0, BC_NEUMANN = 1, BC_DIRICHLET = 2 };

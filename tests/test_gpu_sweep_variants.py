@@ -14,7 +14,7 @@ from ntustcfdlab_amd.hip import (QUICK_CENTRE, SOR_MIXED, SOR_REDBLACK, SWEEP_FU
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("shape", [(40, 36, 44), (33, 9, 17), (64, 16, 8), (7, 5, 3)])
+@pytest.mark.parametrize("shape", [(40, 36, 44), (33, 9, 17), (64, 16, 8), (7, 5, 3), (70, 18, 90)])
 @pytest.mark.parametrize("mode", [SOR_REDBLACK, SOR_MIXED])
 @pytest.mark.parametrize("itmax,zeta", [(1.0, 0.0), (2.0, 0.0), (65.0, 0.0), (400.0, 1e-7)])
 def test_fused_equals_two_pass(shape, mode, itmax, zeta, hip_lib):
