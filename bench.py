@@ -11,7 +11,15 @@ A step is one full pass of the hot path (LES viscosity, predictor, pressure
 solve, projection with direct forcing, force integral, boundary conditions)
 with the state resident in HBM.  W warm-up steps, then K timed steps
 bracketed by HIP events on the library's stream (device synchronised on both
-sides).  One JSON line on stdout.
+sides).  Both pressure solvers run from the same state: FP64 red-black SOR and
+FP32 sweeps inside FP64 defect correction; the headline is the latter when its
+velocity is within 1e-6 (relative) of the FP64 solver's.
+
+Further legs in the same JSON line: the same cavity at 256^3 (the size the
+>= 70 % HBM and >= 2x FP32-refine targets refer to), the discrete adjoint on
+a 96^3 Re=400 cavity (cost relative to the forward, central-difference
+check), and the CPU restatement timed on a bounded sample.  One JSON line on
+stdout.
 """
 from __future__ import annotations
 
@@ -184,6 +192,79 @@ def timed_run(s, hip, hiptime, state, sor, steps, warmup):
     return ms, ik, prof, final, wall
 
 
+def measure(hip, hiptime, size, steps, warmup, sweep, only, calibrate):
+    """Both solvers on the LES cavity of `size`^3 from the same state."""
+    from ntustcfdlab_amd.grid import Grid, bc_codes
+    w = workload(size)
+    g = Grid(w["nx"], w["ny"], w["nz"])
+    s = hip.ProjectionSolver(g, bc_codes())
+    s.set_sweep_variant(hip.SWEEP_FUSED if sweep == "fused" else hip.SWEEP_TWO_PASS)
+    if calibrate:
+        hiptime.copy_calibration()
+
+    def configure_sor(sor):
+        s.configure(dt=w["dt"], nu=1.0 / w["Re"], Cs=w["Cs"], LES=w["LES"], scheme=w["scheme"],
+                    omega=w["omega"], zeta=w["zeta"], itmax=w["itmax"], sor=sor, lx=1.0,
+                    force_every=1)
+    s.configure_sor = configure_sor
+    configure_sor(hip.SOR_REDBLACK)
+    state = initial_state(g, w, s)
+    runs = {}
+    for name, sor in (("fp64", hip.SOR_REDBLACK), ("mixed", hip.SOR_MIXED)):
+        if only and only != name:
+            continue
+        print(f"# {size}^3 {name}: warmup {warmup}, timing {steps} steps", file=sys.stderr,
+              flush=True)
+        runs[name] = timed_run(s, hip, hiptime, state, sor, steps, warmup)
+    s.close()
+
+    cells = g.cells
+    # HBM bytes per launch of the sweep kernels, from the counter profile of
+    # this workload (FETCH_SIZE/WRITE_SIZE scaled by a plain copy)
+    traffic = {}
+    tfile = ROOT / "profiles" / f"pmc_traffic_{size}.json"
+    if tfile.exists():
+        t = json.loads(tfile.read_text())
+        if t.get("cells") == cells:
+            traffic = {k: v["read_bytes"] + v["write_bytes"] for k, v in t["per_launch"].items()}
+
+    solvers, roofs = {}, {}
+    wall = 0.0
+    for name, (ms, ik, (prof_ms, launches, kid, per_cell), final, wl) in runs.items():
+        roof = None
+        if launches:
+            avg_ms = prof_ms / launches
+            achieved = per_cell * cells / (avg_ms / 1e3) / 1e9
+            kname, kdesc = hip.PROF_KERNELS[kid]
+            tr = traffic.get(kname)
+            roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": round(tr) if tr else None,
+                    "kernel": f"{kname} ({kdesc})",
+                    "bytes_per_launch": per_cell * cells,
+                    "avg_launch_us": round(avg_ms * 1e3, 3), "launches_timed": launches}
+        roofs[name] = roof
+        solvers[name] = {"value": round(cells * steps / (ms / 1e3) / 1e6, 3),
+                         "ms_per_step": round(ms / steps, 3),
+                         "sweeps_per_step": round(float(np.mean(ik)), 1),
+                         "roofline_frac": roof and roof["frac"]}
+        wall = wl
+    err = None
+    if "fp64" in runs and "mixed" in runs:
+        ref = runs["fp64"][3]
+        vel = max(np.abs(x).max() for x in ref[:3])
+        err = max(np.abs(x - y).max() for x, y in zip(runs["mixed"][3][:3], ref[:3])) / vel
+    # the headline is the FP32-sweep + FP64-refine solver when it meets the
+    # 1e-6 relative velocity bar against the FP64 solver, else the FP64 one
+    head = "mixed" if ("mixed" in runs and (err is None or err <= 1e-6)) else "fp64"
+    return {"grid": g, "workload": w, "state": state,
+            "summary": {"head": head, "solvers": solvers, "roofline": roofs,
+                        "wall_s": round(wall, 3), "mixed_vs_fp64_rel_velocity_err": err,
+                        "mixed_speedup_vs_fp64": (round(solvers["mixed"]["value"] /
+                                                        solvers["fp64"]["value"], 3)
+                                                  if len(solvers) == 2 else None)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -199,6 +280,7 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=128)
     ap.add_argument("--fd-directions", type=int, default=2)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-256", action="store_true", help="skip the 256^3 leg")
     ap.add_argument("--sweep", choices=["fused", "two-pass"], default="two-pass",
                     help="red-black sweep kernel (bit-identical results)")
     ap.add_argument("--only-adjoint", action="store_true",
@@ -210,94 +292,43 @@ def main():
         sys.exit(2)
 
     from ntustcfdlab_amd import hip, hiptime
-    from ntustcfdlab_amd.grid import Grid, bc_codes
 
     if a.only_adjoint:
         print(json.dumps({"adjoint": adjoint_leg(hip, hiptime, a.adjoint_size,
                                                  a.adjoint_steps, a.fd_directions)}))
         return
 
-    w = workload(a.size)
-    g = Grid(w["nx"], w["ny"], w["nz"])
-    s = hip.ProjectionSolver(g, bc_codes())
-    s.set_sweep_variant(hip.SWEEP_FUSED if a.sweep == "fused" else hip.SWEEP_TWO_PASS)
-    if a.calibrate:
-        hiptime.copy_calibration()
-
-    def configure_sor(sor):
-        s.configure(dt=w["dt"], nu=1.0 / w["Re"], Cs=w["Cs"], LES=w["LES"], scheme=w["scheme"],
-                    omega=w["omega"], zeta=w["zeta"], itmax=w["itmax"], sor=sor, lx=1.0,
-                    force_every=1)
-    s.configure_sor = configure_sor
-    configure_sor(hip.SOR_REDBLACK)
-    state = initial_state(g, w, s)
-
-    runs = {}
-    for name, sor in (("fp64", hip.SOR_REDBLACK), ("mixed", hip.SOR_MIXED)):
-        if a.only and a.only != name:
-            continue
-        print(f"# {name}: warmup {a.warmup}, timing {a.steps} steps", file=sys.stderr, flush=True)
-        runs[name] = timed_run(s, hip, hiptime, state, sor, a.steps, a.warmup)
-    s.close()
-
-    cells = g.cells
-    # HBM bytes per launch of the half-sweep kernels, from the counter
-    # profile of this workload (FETCH_SIZE/WRITE_SIZE scaled by a plain copy)
-    traffic = {}
-    tfile = ROOT / "profiles" / f"pmc_traffic_{a.size}.json"
-    if tfile.exists():
-        t = json.loads(tfile.read_text())
-        if t.get("cells") == cells:
-            traffic = {k: v["read_bytes"] + v["write_bytes"] for k, v in t["per_launch"].items()}
-
-    def summary(name):
-        ms, ik, (prof_ms, launches, kid, per_cell), final, wall = runs[name]
-        value = cells * a.steps / (ms / 1e3) / 1e6
-        roof = None
-        if launches:
-            avg_ms = prof_ms / launches
-            achieved = per_cell * cells / (avg_ms / 1e3) / 1e9
-            kname, kdesc = hip.PROF_KERNELS[kid]
-            tr = traffic.get(kname)
-            roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": round(tr) if tr else None,
-                    "kernel": f"{kname} ({kdesc})",
-                    "bytes_per_launch": per_cell * cells,
-                    "avg_launch_us": round(avg_ms * 1e3, 3), "launches_timed": launches}
-        return value, ms / a.steps, float(np.mean(ik)), roof, wall
-
-    err = None
-    if "fp64" in runs and "mixed" in runs:
-        ref = runs["fp64"][3]
-        vel = max(np.abs(x).max() for x in ref[:3])
-        err = max(np.abs(x - y).max() for x, y in zip(runs["mixed"][3][:3], ref[:3])) / vel
-    # the headline is the FP32-sweep + FP64-refine solver when it meets the
-    # 1e-6 relative velocity bar against the FP64 solver, else the FP64 one
-    head = "mixed" if ("mixed" in runs and (err is None or err <= 1e-6)) else "fp64"
-    value, msps, sweeps, roof, wall = summary(head)
-    extra = {}
-    for name in runs:
-        v, m, sw, r, wl = summary(name)
-        extra[name] = {"value": round(v, 3), "ms_per_step": round(m, 3),
-                       "sweeps_per_step": round(sw, 1), "roofline_frac": r and r["frac"]}
+    head_run = measure(hip, hiptime, a.size, a.steps, a.warmup, a.sweep, a.only, a.calibrate)
+    g, w, state, sweeps = head_run["grid"], head_run["workload"], head_run["state"], None
+    res = head_run["summary"]
+    head = res["head"]
+    sweeps = res["solvers"][head]["sweeps_per_step"]
     out = {
-        "metric": METRIC, "value": round(value, 3), "unit": "Mcell-steps/s", "n_gpus": 1,
-        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(msps, 3),
+        "metric": METRIC, "value": res["solvers"][head]["value"], "unit": "Mcell-steps/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": res["solvers"][head]["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if head == "fp64" else "f32 sweeps + f64 refine",
         "data": "synthetic",
         "config": {"workload": (f"LES cavity Re=1000 {a.size}^3 uniform, QUICK, Smagorinsky "
                                 f"Cs=0.1, dt=5e-4, SOR omega=1.5 zeta=1e-10 itmax=3000, "
                                 f"red-black, {'FP64' if head == 'fp64' else 'FP32 sweeps in FP64 defect correction'}"),
-                   "cells": cells, "solver": head, "sweeps_per_step": round(sweeps, 1),
-                   "wall_s": round(wall, 3)},
-        "roofline": roof,
-        "solvers": extra,
-        "mixed_vs_fp64_rel_velocity_err": err,
-        "mixed_speedup_vs_fp64": (round(extra["mixed"]["value"] / extra["fp64"]["value"], 3)
-                                  if len(extra) == 2 else None),
+                   "cells": g.cells, "solver": head, "sweeps_per_step": sweeps,
+                   "wall_s": res["wall_s"]},
+        "roofline": res["roofline"][head],
+        "solvers": res["solvers"],
+        "mixed_vs_fp64_rel_velocity_err": res["mixed_vs_fp64_rel_velocity_err"],
+        "mixed_speedup_vs_fp64": res["mixed_speedup_vs_fp64"],
     }
+    if a.size != 256 and not a.no_256 and not a.only:
+        print("# 256^3 LES cavity", file=sys.stderr, flush=True)
+        big = measure(hip, hiptime, 256, min(a.steps, 3), 1, a.sweep, None, False)["summary"]
+        out["les_cavity_256"] = {
+            "workload": "the same LES cavity on uniform 256^3 (16.8 M cells)",
+            "steps": min(a.steps, 3), "warmup": 1, "solvers": big["solvers"],
+            "roofline": big["roofline"],
+            "mixed_speedup_vs_fp64": big["mixed_speedup_vs_fp64"],
+            "mixed_vs_fp64_rel_velocity_err": big["mixed_vs_fp64_rel_velocity_err"]}
     if not a.no_adjoint:
         print("# adjoint", file=sys.stderr, flush=True)
         out["adjoint"] = adjoint_leg(hip, hiptime, a.adjoint_size, a.adjoint_steps,
