@@ -84,6 +84,24 @@ k_sor_lexicographic(KGrid g, const double *__restrict__ us, const double *__rest
     }
 }
 
+__global__ void k_sor_init(double *slots, SorControl *ctl)
+{
+    const int b = threadIdx.x;
+    if (b < SLOT_BUCKETS) {
+        slots[slot_index(0, b)] = b == 0 ? 1.0 : 0.0;
+        slots[slot_index(1, b)] = 0.0;
+    }
+    if (b == 0) {
+        ctl->stop = 0;
+        ctl->ik = 0;
+    }
+}
+
+void launch_sor_init(double *slots, SorControl *ctl, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_sor_init, dim3(1), dim3(64), 0, st, slots, ctl);
+}
+
 void launch_sor_lexicographic(const KGrid &g, const double *us, const double *vs,
                               const double *ws, double *p, double dt, double omega,
                               double zeta, double itmax, SorControl *ctl, double *pmax,
@@ -176,8 +194,8 @@ k_rb_sweep(KGrid g, SplitGrid s, const double *__restrict__ rx,
     const int tid = threadIdx.x + threadIdx.y * blockDim.x;
     const int blin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
     const int lane = tid & 63;
-    double gate = lane < SLOT_BUCKETS ? slots[slot_index(t - 1, lane)] : 0.0;
-    const int stopped = ctl->stop;
+    double gate = lane < SLOT_BUCKETS ? load_flag(&slots[slot_index(t - 1, lane)]) : 0.0;
+    const int stopped = load_flag(&ctl->stop);
 
     // natural block order: measured faster for this FP64 kernel at 256^3
     // than the XCD-slab order the FP32 kernel uses
@@ -240,6 +258,7 @@ k_rb_sweep(KGrid g, SplitGrid s, const double *__restrict__ rx,
         if (blin == 0 && tid == 0) ctl->stop = 1;
         return;
     }
+    if (C == 0 && blin == 0) clear_next_buckets(slots, t, tid);
 #pragma unroll
     for (int r = 0; r < RB_KPT; r++)
         if (in[r]) pc_own[at[r]] = nv[r];
@@ -250,7 +269,7 @@ k_rb_sweep(KGrid g, SplitGrid s, const double *__restrict__ rx,
         double mm = red[0];
         for (int w = 1; w < (int)((blockDim.x * blockDim.y) >> 6); w++) mm = red[w] > mm ? red[w] : mm;
         double *bk = &slots[slot_index(t, blin % SLOT_BUCKETS)];
-        if (mm > 0.0 && mm > *(volatile double *)bk)
+        if (mm > 0.0 && mm > load_flag(bk))
             atomicMax((unsigned long long *)bk, (unsigned long long)__double_as_longlong(mm));
         if (C == 1 && blin == 0) ctl->ik = t;
     }

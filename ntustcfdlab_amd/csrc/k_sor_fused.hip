@@ -94,8 +94,8 @@ k_rb_sweep_stream(KGrid g, SplitGrid s, SweepMetric<T> m, const T *__restrict__ 
     const int tid = threadIdx.x;
     const int blin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
     const int lane = tid & 63;
-    double gate = lane < SLOT_BUCKETS ? slots[slot_index(t - 1, lane)] : 0.0;
-    const int stopped = ctl->stop;
+    double gate = lane < SLOT_BUCKETS ? load_flag(&slots[slot_index(t - 1, lane)]) : 0.0;
+    const int stopped = load_flag(&ctl->stop);
 
     const int x0 = blockIdx.x * TX, j0 = blockIdx.y * TY + 1;
     const int k0 = blockIdx.z * tzc + 1;
@@ -190,6 +190,7 @@ k_rb_sweep_stream(KGrid g, SplitGrid s, SweepMetric<T> m, const T *__restrict__ 
         if (blin == 0 && tid == 0) ctl->stop = 1;
         return;
     }
+    if (blin == 0) clear_next_buckets(slots, t, tid);
 
     T pcm = (T)0;
     auto step = [&](int r, T (&pre)[NLD], T &cR, T &cB) {
@@ -264,7 +265,7 @@ k_rb_sweep_stream(KGrid g, SplitGrid s, SweepMetric<T> m, const T *__restrict__ 
         double mm = red[0];
         for (int w = 1; w < NT / 64; w++) mm = red[w] > mm ? red[w] : mm;
         double *bk = &slots[slot_index(t, blin % SLOT_BUCKETS)];
-        if (mm > 0.0 && mm > *(volatile double *)bk)
+        if (mm > 0.0 && mm > load_flag(bk))
             atomicMax((unsigned long long *)bk, (unsigned long long)__double_as_longlong(mm));
         if (blin == 0) {
             ctl->ik = t;
@@ -323,12 +324,12 @@ k_rb_absorb_pp(KGrid g, SplitGrid s, double *__restrict__ p0, double *__restrict
                float *ea0, float *ea1, float *eb0, float *eb1, const int *dirty,
                const SorControl *ctl)
 {
-    if (!*dirty) return;
+    if (!load_flag(dirty)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const int k = blockIdx.z + 1;
     if (i > g.nx || j > g.ny) return;
-    const bool second = ctl->ik & 1;
+    const bool second = load_flag(&ctl->ik) & 1;
     float *e0 = second ? eb0 : ea0, *e1 = second ? eb1 : ea1;
     const long o = six(s, i, j, k);
     if ((i + j + k) & 1) {

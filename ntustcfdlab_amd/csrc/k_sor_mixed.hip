@@ -31,8 +31,8 @@ k_rb_sweep32(KGrid g, SplitGrid s, FMetric fm, float *__restrict__ e_own,
     const int tid = threadIdx.x + threadIdx.y * blockDim.x;
     const int blin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
     const int lane = tid & 63;
-    double gate = lane < SLOT_BUCKETS ? slots[slot_index(t - 1, lane)] : 0.0;
-    const int stopped = ctl->stop;
+    double gate = lane < SLOT_BUCKETS ? load_flag(&slots[slot_index(t - 1, lane)]) : 0.0;
+    const int stopped = load_flag(&ctl->stop);
 
     // same thread layout and load-first structure as the FP64 sweep
     int bx, by, bz;
@@ -93,6 +93,7 @@ k_rb_sweep32(KGrid g, SplitGrid s, FMetric fm, float *__restrict__ e_own,
         if (blin == 0 && tid == 0) ctl->stop = 1;
         return;
     }
+    if (C == 0 && blin == 0) clear_next_buckets(slots, t, tid);
 #pragma unroll
     for (int r = 0; r < RB_KPT; r++)
         if (in[r]) e_own[at[r]] = nv[r];
@@ -103,7 +104,7 @@ k_rb_sweep32(KGrid g, SplitGrid s, FMetric fm, float *__restrict__ e_own,
         double mm = red[0];
         for (int w = 1; w < (int)((blockDim.x * blockDim.y) >> 6); w++) mm = red[w] > mm ? red[w] : mm;
         double *bk = &slots[slot_index(t, blin % SLOT_BUCKETS)];
-        if (mm > 0.0 && mm > *(volatile double *)bk)
+        if (mm > 0.0 && mm > load_flag(bk))
             atomicMax((unsigned long long *)bk, (unsigned long long)__double_as_longlong(mm));
         if (blin == 0) {
             *dirty = 1;
@@ -117,7 +118,7 @@ __global__ void __launch_bounds__(256)
 k_rb_absorb(KGrid g, SplitGrid s, double *__restrict__ p0, double *__restrict__ p1,
             float *__restrict__ e0, float *__restrict__ e1, const int *dirty)
 {
-    if (!*dirty) return;
+    if (!load_flag(dirty)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const int k = blockIdx.z + 1;
@@ -141,7 +142,7 @@ k_rb_residual(KGrid g, SplitGrid s, const double *__restrict__ rx,
               const double *__restrict__ b0, const double *__restrict__ b1,
               float *__restrict__ r0, float *__restrict__ r1, const int *dirty)
 {
-    if (dirty && !*dirty) return;
+    if (dirty && !load_flag(dirty)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const int k = blockIdx.z + 1;

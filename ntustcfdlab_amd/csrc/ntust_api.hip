@@ -85,23 +85,28 @@ struct Ctx {
 Ctx C;
 
 // A buffer asked for again with a larger size is reallocated (its contents
-// are not kept); only the sweep-history buffer of the SOR grows that way.
+// are not kept).  The zero fill is queued on the library stream: the stream
+// is non-blocking, so a fill on the null stream would not be ordered before
+// the kernels that follow on it (a late fill can wipe their results).
 double *buf(const char *name, size_t n, bool zero = false)
 {
     for (auto &b : C.bufs) {
         if (b.name != name) continue;
         if (b.n >= n) return b.ptr;
+        // the old buffer may still be in use by queued work
+        if (hipStreamSynchronize(C.st) != hipSuccess) return nullptr;
         if (hipFree(b.ptr) != hipSuccess) return nullptr;
         b.ptr = nullptr;
         b.n = 0;
         if (hipMalloc(&b.ptr, n * sizeof(double)) != hipSuccess) return nullptr;
-        if (zero && hipMemset(b.ptr, 0, n * sizeof(double)) != hipSuccess) return nullptr;
+        if (zero && hipMemsetAsync(b.ptr, 0, n * sizeof(double), C.st) != hipSuccess)
+            return nullptr;
         b.n = n;
         return b.ptr;
     }
     double *p = nullptr;
     if (hipMalloc(&p, n * sizeof(double)) != hipSuccess) return nullptr;
-    if (zero && hipMemset(p, 0, n * sizeof(double)) != hipSuccess) return nullptr;
+    if (zero && hipMemsetAsync(p, 0, n * sizeof(double), C.st) != hipSuccess) return nullptr;
     C.bufs.push_back({name, p, n});
     return p;
 }
@@ -150,7 +155,11 @@ int filter_table(double Cs)
                 const double cd = Cs * d;
                 t[o++] = cd * cd;
             }
-    HIPCHK(hipMemcpy(filt, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+    // on the library stream (ordered with the kernels that read the table);
+    // t is released only after the copy has finished
+    HIPCHK(hipMemcpyAsync(filt, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice,
+                          C.st));
+    HIPCHK(hipStreamSynchronize(C.st));
     C.filt_cs = Cs;
     return 0;
 }
@@ -205,11 +214,9 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
     launch_rb_gather(g, C.sg, p, P0, P1, C.st);
     if (Q0) launch_rb_gather(g, C.sg, p, Q0, Q1, C.st);   // ghosts in both copies
     launch_rb_rhs(g, C.sg, us, vs, ws, dt, B0, B1, C.st);
-    // sweep 0: 1.0 (pChangeMax_ before the first sweep); sweeps 1.. : 0
-    HIPCHK(hipMemsetAsync(slots, 0, nslot * sizeof(double), C.st));
-    const double one = 1.0;
-    HIPCHK(hipMemcpyAsync(slots, &one, sizeof one, hipMemcpyHostToDevice, C.st));
-    HIPCHK(hipMemsetAsync(C.ctl, 0, sizeof(SorControl), C.st));
+    // sweep 0: 1.0 (pChangeMax_ before the first sweep), sweep 1: 0, stop
+    // and sweep count cleared; each sweep clears the buckets of the next
+    launch_sor_init(slots, C.ctl, C.st);
     const double *rx = C.recip[0] + 1, *ry = C.recip[1] + 1, *rz = C.recip[2] + 1;
     SorControl h{};
     long t = 1;

@@ -50,6 +50,20 @@ __host__ __device__ inline long six(const SplitGrid &s, int i, int j, int k)
     return (long)(i >> 1) + (long)s.mp * ((long)j + (long)s.ry * (long)k);
 }
 
+// Loads of values that earlier launches wrote with atomics or as flags (the
+// per-sweep maxima, the stop and dirty flags).  System scope: the load
+// bypasses the XCD's L2, so it does not depend on that L2 having been
+// invalidated since a stale copy of the line was cached there (a memset
+// before the solve, or the filter read of a bucket).
+__device__ __forceinline__ double load_flag(const double *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ int load_flag(const int *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // max over a 64-wide wavefront (non-negative, non-NaN inputs)
 __device__ __forceinline__ double wave_max(double v)
 {
@@ -125,6 +139,13 @@ struct SorControl {
     int ik;              // sweeps run
     int pad[2];
 };
+// buckets of sweep 0 (1.0 in bucket 0, GS.f90:9) and sweep 1 (0), control
+// cleared; every running sweep t clears the buckets of sweep t+1
+void launch_sor_init(double *slots, SorControl *ctl, hipStream_t st);
+__device__ __forceinline__ void clear_next_buckets(double *slots, int t, int tid)
+{
+    if (tid < SLOT_BUCKETS) slots[slot_index(t + 1, tid)] = 0.0;
+}
 void launch_sor_lexicographic(const KGrid &g, const double *us, const double *vs,
                               const double *ws, double *p, double dt, double omega,
                               double zeta, double itmax, SorControl *ctl, double *pmax,
