@@ -143,6 +143,28 @@ int ntust_boundary_conditions(double *u, double *v, double *w, double *p, double
                               double *v1, double *w1, double *u2, double *v2, double *w2,
                               double *u_star, double *v_star, double *w_star);
 
+/* ---- members of the Krylov pressure path ------------------------------
+ * The reference's BiCGSTAB route (solver.f90, called from no time step of
+ * main.f90) builds its system with these; they are provided for callers of
+ * that route.  Interior arrays in the reference's x-fastest order. */
+
+/* solver_1st_member.f90:1-79: the 7-diagonal matrix coef(ndim,7), ndim =
+ * nx*ny*nz, column d holding diagonal d (1 centre, 2/3 +-x, 4/5 +-y, 6/7 +-z)
+ * with the reference's boundary-row edits, and the offsets jcoef(4) =
+ * (0, 1, nx, nx*ny). */
+int ntust_poisson_coeffs(double *coef, int *jcoef);
+
+/* The right-hand side b = mChange/dt of the pressure equation in the
+ * face-area form of GS.f90:39-41,49 (interior array).  The reference's
+ * solver_2nd_member.f90 (a volume-normalised variant with boundary pressure
+ * terms) does not compile; this is the form the solve uses. */
+int ntust_poisson_rhs(const double *u_star, const double *v_star, const double *w_star,
+                      double dt, double *b);
+
+/* solver_pressure_field.f90:1-21: pre = x and p = pre on the interior (x in
+ * the order of the unknowns, one rank). */
+int ntust_pressure_field(const double *x, double *p, double *pre);
+
 /* ---- device-resident time stepping ------------------------------------ */
 
 /* Step parameters (variables_module.f90:9-23,277; main.f90:44-60):

@@ -275,6 +275,104 @@ k_rb_sweep(KGrid g, SplitGrid s, const double *__restrict__ rx,
     }
 }
 
+// ---------------------------------------------------------------------
+// members of the Krylov pressure path (solver_1st_member.f90,
+// solver_2nd_member.f90, solver_pressure_field.f90)
+// ---------------------------------------------------------------------
+
+// The 7-diagonal matrix coef(ndim,7) of solver_1st_member.f90:18-69 (row ik
+// of cell (i,j,k) in x-fastest order; 1/a/b evaluated as (1/a)/b), boundary
+// rows as the reference edits them: the neighbour coefficient outside the
+// domain is zeroed, and on the east face its value is also taken off the
+// diagonal.
+__global__ void __launch_bounds__(256)
+k_poisson_coeffs(KGrid g, double *__restrict__ coef)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.nx || j > g.ny) return;
+    const long nd = (long)g.nx * g.ny * g.nz;
+    const long ik = iix(g, i, j, k);
+    double c1 = 1 / g.iDx[i] / g.Dxs[i] + 1 / g.iDx[i] / g.Dxs[i - 1] + 1 / g.iDy[j] / g.Dys[j] +
+                1 / g.iDy[j] / g.Dys[j - 1] + 1 / g.iDz[k] / g.Dzs[k] + 1 / g.iDz[k] / g.Dzs[k - 1];
+    double c2 = -1 / g.iDx[i] / g.Dxs[i], c3 = -1 / g.iDx[i] / g.Dxs[i - 1];
+    double c4 = -1 / g.iDy[j] / g.Dys[j], c5 = -1 / g.iDy[j] / g.Dys[j - 1];
+    double c6 = -1 / g.iDz[k] / g.Dzs[k], c7 = -1 / g.iDz[k] / g.Dzs[k - 1];
+    double aa = 0.0, bb = 0.0, cc = 0.0;
+    if (i == 1) {
+        c3 = 0.0;
+    } else if (i == g.nx) {
+        aa = -1 / g.iDx[i] / g.Dxs[i];
+        c2 = 0.0;
+    }
+    if (j == 1) c5 = 0.0;
+    else if (j == g.ny) c4 = 0.0;
+    if (k == 1) c7 = 0.0;
+    else if (k == g.nz) c6 = 0.0;
+    c1 = c1 + aa + bb + cc;
+    coef[ik] = c1;
+    coef[ik + nd] = c2;
+    coef[ik + 2 * nd] = c3;
+    coef[ik + 3 * nd] = c4;
+    coef[ik + 4 * nd] = c5;
+    coef[ik + 5 * nd] = c6;
+    coef[ik + 6 * nd] = c7;
+}
+
+// b = mChange/dt on the interior, interior layout (GS.f90:39-41,49; the
+// right-hand side the red-black solver gathers)
+__global__ void __launch_bounds__(256)
+k_poisson_rhs(KGrid g, const double *__restrict__ us, const double *__restrict__ vs,
+              const double *__restrict__ ws, double dt, double *__restrict__ b)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.nx || j > g.ny) return;
+    const double ax = g.iDx[i], ay = g.iDy[j], az = g.iDz[k];
+    const long o = hix(g, i, j, k);
+    const double mChange = (us[o] - us[o - 1]) * ay * az
+                         + (vs[o] - vs[o - g.sx]) * ax * az
+                         + (ws[o] - ws[o - g.sxy]) * ax * ay;
+    b[iix(g, i, j, k)] = mChange / dt;
+}
+
+// solver_pressure_field.f90:8-17: pre = x, p = pre on the interior
+__global__ void __launch_bounds__(256)
+k_pressure_field(KGrid g, const double *__restrict__ x, double *__restrict__ p,
+                 double *__restrict__ pre)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > g.nx || j > g.ny) return;
+    const long ik = iix(g, i, j, k);
+    const double v = x[ik];
+    pre[ik] = v;
+    p[hix(g, i, j, k)] = v;
+}
+
+void launch_poisson_coeffs(const KGrid &g, double *coef, hipStream_t st)
+{
+    dim3 blk(64, 4), grd((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz);
+    hipLaunchKernelGGL(k_poisson_coeffs, grd, blk, 0, st, g, coef);
+}
+
+void launch_poisson_rhs(const KGrid &g, const double *us, const double *vs, const double *ws,
+                        double dt, double *b, hipStream_t st)
+{
+    dim3 blk(64, 4), grd((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz);
+    hipLaunchKernelGGL(k_poisson_rhs, grd, blk, 0, st, g, us, vs, ws, dt, b);
+}
+
+void launch_pressure_field(const KGrid &g, const double *x, double *p, double *pre,
+                           hipStream_t st)
+{
+    dim3 blk(64, 4), grd((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz);
+    hipLaunchKernelGGL(k_pressure_field, grd, blk, 0, st, g, x, p, pre);
+}
+
 void launch_rb_gather(const KGrid &g, const SplitGrid &s, const double *p, double *p0,
                       double *p1, hipStream_t st)
 {

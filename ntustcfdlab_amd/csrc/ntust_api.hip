@@ -920,6 +920,61 @@ int ntust_boundary_conditions(double *u, double *v, double *w, double *p, double
 }
 
 // ---------------------------------------------------------------------
+// members of the Krylov pressure path
+// ---------------------------------------------------------------------
+
+int ntust_poisson_coeffs(double *coef, int *jcoef)
+{
+    NEED_INIT();
+    if (!coef || !jcoef) return fail(NTUST_E_ARG, "NULL argument");
+    NEED(c, "poisson_coef", 7 * C.ni, false);
+    launch_poisson_coeffs(C.g, c, C.st);
+    HIPCHK(hipGetLastError());
+    TRY(d2h(coef, c, 7 * C.ni));
+    TRY(sync());
+    // solver_1st_member.f90:71-74
+    jcoef[0] = 0;
+    jcoef[1] = 1;
+    jcoef[2] = C.nx;
+    jcoef[3] = C.nx * C.ny;
+    return 0;
+}
+
+int ntust_poisson_rhs(const double *u_star, const double *v_star, const double *w_star,
+                      double dt, double *b)
+{
+    NEED_INIT();
+    if (!u_star || !v_star || !w_star || !b) return fail(NTUST_E_ARG, "NULL array");
+    NEED(us, "us", C.nh, true);
+    NEED(vs, "vs", C.nh, true);
+    NEED(ws, "ws", C.nh, true);
+    NEED(db, "poisson_b", C.ni, false);
+    TRY(h2d(us, u_star, C.nh));
+    TRY(h2d(vs, v_star, C.nh));
+    TRY(h2d(ws, w_star, C.nh));
+    launch_poisson_rhs(C.g, us, vs, ws, dt, db, C.st);
+    HIPCHK(hipGetLastError());
+    TRY(d2h(b, db, C.ni));
+    return sync();
+}
+
+int ntust_pressure_field(const double *x, double *p, double *pre)
+{
+    NEED_INIT();
+    if (!x || !p || !pre) return fail(NTUST_E_ARG, "NULL array");
+    NEED(dx, "pressure_x", C.ni, false);
+    NEED(dp, "p", C.nh, true);
+    NEED(dpre, "pre", C.ni, true);
+    TRY(h2d(dx, x, C.ni));
+    TRY(h2d(dp, p, C.nh));
+    launch_pressure_field(C.g, dx, dp, dpre, C.st);
+    HIPCHK(hipGetLastError());
+    TRY(d2h(p, dp, C.nh));
+    TRY(d2h(pre, dpre, C.ni));
+    return sync();
+}
+
+// ---------------------------------------------------------------------
 // device-resident stepping
 // ---------------------------------------------------------------------
 

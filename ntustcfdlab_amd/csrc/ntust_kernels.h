@@ -150,6 +150,12 @@ void launch_sor_lexicographic(const KGrid &g, const double *us, const double *vs
                               const double *ws, double *p, double dt, double omega,
                               double zeta, double itmax, SorControl *ctl, double *pmax,
                               hipStream_t st);
+// members of the Krylov pressure path
+void launch_poisson_coeffs(const KGrid &g, double *coef, hipStream_t st);
+void launch_poisson_rhs(const KGrid &g, const double *us, const double *vs, const double *ws,
+                        double dt, double *b, hipStream_t st);
+void launch_pressure_field(const KGrid &g, const double *x, double *p, double *pre,
+                           hipStream_t st);
 void launch_rb_rhs(const KGrid &g, const SplitGrid &s, const double *us, const double *vs,
                    const double *ws, double dt, double *b0, double *b1, hipStream_t st);
 void launch_rb_gather(const KGrid &g, const SplitGrid &s, const double *p, double *p0,

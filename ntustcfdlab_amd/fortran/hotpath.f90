@@ -140,3 +140,23 @@ subroutine boundary_conditions()
    call ntust_check(ntust_boundary_conditions(u, v, w, p, u1, v1, w1, u2, v2, w2, u_star, &
                                               v_star, w_star), 'boundary_conditions')
 end subroutine boundary_conditions
+
+! solver_1st_member.f90:1-79, with the reference's explicit arguments: the
+! 7-diagonal matrix of the Krylov pressure path (mdim = 7), computed on the
+! GPU from the metrics given to ntust_setup.
+subroutine solver_1st_member(coef, jcoef, nx, ny, nz, ndim, mdim, iDx, Dxs, iDy, Dys, iDz, Dzs)
+   use ntust_hip
+   implicit none
+   integer :: nx, ny, nz, ndim, mdim
+   integer :: jcoef(1:mdim)
+   double precision :: coef(1:ndim, 1:mdim)
+   double precision :: iDx(-1:nx+2), Dxs(-1:nx+2), iDy(-1:nx+2), Dys(-1:nx+2)
+   double precision :: iDz(-1:nx+2), Dzs(-1:nx+2)
+   integer(c_int) :: jc(4)
+   if (mdim /= 7 .or. ndim /= nx*ny*nz) then
+      write (*, '(a)') 'solver_1st_member: need ndim = nx*ny*nz and mdim = 7'
+      error stop 1
+   end if
+   call ntust_check(ntust_poisson_coeffs(coef, jc), 'solver_1st_member')
+   jcoef(1:4) = jc
+end subroutine solver_1st_member

@@ -12,6 +12,7 @@ module ntust_hip
    integer(c_int), parameter :: NTUST_SOR_LEXICOGRAPHIC = 0, NTUST_SOR_REDBLACK = 1, &
                                 NTUST_SOR_MIXED = 2
    integer(c_int), parameter :: NTUST_QUICK_CENTRE = 0, NTUST_UPWIND = 1
+   integer(c_int), parameter :: NTUST_SWEEP_TWO_PASS = 0, NTUST_SWEEP_FUSED = 1
 
    interface
       integer(c_int) function ntust_init(nx, ny, nz, iDx, Dxs, iDy, Dys, iDz, Dzs) &
@@ -146,6 +147,99 @@ module ntust_hip
          integer(c_int), value :: nsteps
          integer(c_int), intent(out) :: ik(*)
          real(c_double), intent(out) :: pchange(*), cdrag(*), clift(*)
+      end function
+
+      integer(c_int) function ntust_set_sweep_variant(variant) &
+            bind(C, name="ntust_set_sweep_variant")
+         import :: c_int
+         integer(c_int), value :: variant
+      end function
+
+      ! members of the Krylov pressure path (solver_1st_member.f90,
+      ! solver_pressure_field.f90; the right-hand side in the form GS.f90 uses)
+      integer(c_int) function ntust_poisson_coeffs(coef, jcoef) &
+            bind(C, name="ntust_poisson_coeffs")
+         import :: c_int, c_double
+         real(c_double), intent(out) :: coef(*)
+         integer(c_int), intent(out) :: jcoef(4)
+      end function
+
+      integer(c_int) function ntust_poisson_rhs(u_star, v_star, w_star, dt, b) &
+            bind(C, name="ntust_poisson_rhs")
+         import :: c_int, c_double
+         real(c_double), intent(in) :: u_star(*), v_star(*), w_star(*)
+         real(c_double), value :: dt
+         real(c_double), intent(out) :: b(*)
+      end function
+
+      integer(c_int) function ntust_pressure_field(x, p, pre) &
+            bind(C, name="ntust_pressure_field")
+         import :: c_int, c_double
+         real(c_double), intent(in) :: x(*)
+         real(c_double), intent(inout) :: p(*), pre(*)
+      end function
+
+      ! discrete adjoint of the resident step (no counterpart in the reference)
+      integer(c_int) function ntust_set_body_force(fu, fv, fw) &
+            bind(C, name="ntust_set_body_force")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: fu, fv, fw
+      end function
+
+      integer(c_int) function ntust_kinetic_energy(ke) bind(C, name="ntust_kinetic_energy")
+         import :: c_int, c_double
+         real(c_double), intent(out) :: ke
+      end function
+
+      integer(c_int) function ntust_checkpoint_reserve(nslots) &
+            bind(C, name="ntust_checkpoint_reserve")
+         import :: c_int
+         integer(c_int), value :: nslots
+      end function
+
+      integer(c_int) function ntust_checkpoint_save(slot) bind(C, name="ntust_checkpoint_save")
+         import :: c_int
+         integer(c_int), value :: slot
+      end function
+
+      integer(c_int) function ntust_checkpoint_load(slot) bind(C, name="ntust_checkpoint_load")
+         import :: c_int
+         integer(c_int), value :: slot
+      end function
+
+      integer(c_int) function ntust_adjoint_set(lu, lv, lw, lp) bind(C, name="ntust_adjoint_set")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: lu, lv, lw, lp
+      end function
+
+      integer(c_int) function ntust_adjoint_get(lu, lv, lw, lp) bind(C, name="ntust_adjoint_get")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: lu, lv, lw, lp
+      end function
+
+      integer(c_int) function ntust_adjoint_add_ke(scale) bind(C, name="ntust_adjoint_add_ke")
+         import :: c_int, c_double
+         real(c_double), value :: scale
+      end function
+
+      integer(c_int) function ntust_adjoint_step(ik) bind(C, name="ntust_adjoint_step")
+         import :: c_int
+         integer(c_int), intent(out) :: ik
+      end function
+
+      integer(c_int) function ntust_adjoint_gradient(gfu, gfv, gfw) &
+            bind(C, name="ntust_adjoint_gradient")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: gfu, gfv, gfw
+      end function
+
+      integer(c_int) function ntust_adjoint_run(nsteps, checkpoint_every, navg, J, ik, ms) &
+            bind(C, name="ntust_adjoint_run")
+         import :: c_int, c_double
+         integer(c_int), value :: nsteps, checkpoint_every, navg
+         real(c_double), intent(out) :: J
+         integer(c_int), intent(out) :: ik(*)
+         real(c_double), intent(out) :: ms(4)
       end function
    end interface
 

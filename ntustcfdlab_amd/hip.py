@@ -81,6 +81,9 @@ def load_library() -> ct.CDLL:
         "ntust_adjoint_gradient": [_D] * 3,
         "ntust_adjoint_run": [i, i, i, _D, _I, _D],
         "ntust_set_sweep_variant": [i],
+        "ntust_poisson_coeffs": [_D, _I],
+        "ntust_poisson_rhs": [_D, _D, _D, d, _D],
+        "ntust_pressure_field": [_D, _D, _D],
         "ntust_prof_kernel": [_D],
     }
     for name, args in sig.items():
@@ -197,6 +200,26 @@ class ProjectionSolver:
         _check(self.lib.ntust_boundary_conditions(_p(u), _p(v), _p(w), _p(p), _p(u1), _p(v1),
                                                   _p(w1), _p(u2), _p(v2), _p(w2), _p(u_star),
                                                   _p(v_star), _p(w_star)))
+
+    # ---- members of the Krylov pressure path ----------------------------
+
+    def solver_1st_member(self):
+        """solver_1st_member.f90: (coef (ndim, 7) Fortran order, jcoef[4])."""
+        coef = np.zeros((self.grid.cells, 7), order="F")
+        jc = (ct.c_int * 4)()
+        _check(self.lib.ntust_poisson_coeffs(_p(coef), jc))
+        return coef, list(jc)
+
+    def poisson_rhs(self, u_star, v_star, w_star, dt):
+        """b = mChange/dt (GS.f90:39-41,49), interior array."""
+        b = self.grid.interior()
+        _check(self.lib.ntust_poisson_rhs(_p(u_star), _p(v_star), _p(w_star), dt, _p(b)))
+        return b
+
+    def solver_pressure_field(self, x, p, pre):
+        """solver_pressure_field.f90: pre = x, p = pre on the interior."""
+        _check(self.lib.ntust_pressure_field(_p(np.ascontiguousarray(x, dtype=np.float64)),
+                                             _p(p), _p(pre)))
 
     # ---- device-resident stepping --------------------------------------
 

@@ -190,3 +190,41 @@ def test_virtual_force_integrator(setup, layout):
     cdr, clr = o.virtual_force_integrator(FY, FZ, 1.0)
     assert np.float64(cd).view(np.uint64) == np.float64(cdr).view(np.uint64)
     assert np.float64(cl).view(np.uint64) == np.float64(clr).view(np.uint64)
+
+
+# ---- members of the Krylov pressure path ------------------------------------
+
+def test_poisson_coeffs(setup):
+    """solver_1st_member.f90: the 7-diagonal matrix and its offsets."""
+    g, s, o, rng = setup
+    coef, jc = s.solver_1st_member()
+    rcoef, rjc = o.poisson_coeffs()
+    assert jc == rjc == [0, 1, g.nx, g.nx * g.ny]
+    assert same(coef, rcoef)
+
+
+def test_poisson_rhs(setup):
+    """b = mChange/dt in the face-area form of GS.f90:39-41,49."""
+    g, s, o, rng = setup
+    us, vs, ws = (rand_halo(g, rng) for _ in range(3))
+    b = s.poisson_rhs(us, vs, ws, 1.00000004749745131e-03)
+    rb = g.interior()
+    o.poisson_b(us, vs, ws, 1.00000004749745131e-03, rb)
+    assert same(b, rb)
+
+
+def test_pressure_field(setup):
+    """solver_pressure_field.f90: pre = x, p = pre on the interior only."""
+    g, s, o, rng = setup
+    x = rng.uniform(-1, 1, g.cells)
+    p = rand_halo(g, rng)
+    p0 = p.copy(order="F")
+    pre = g.interior()
+    s.solver_pressure_field(x, p, pre)
+    xi = x.reshape(g.shape, order="F")
+    assert same(pre, np.asfortranarray(xi))
+    inner = (slice(2, g.nx + 2), slice(2, g.ny + 2), slice(2, g.nz + 2))
+    assert same(np.ascontiguousarray(p[inner]), np.ascontiguousarray(xi))
+    mask = np.ones(p.shape, bool)
+    mask[inner] = False
+    assert same(p[mask], p0[mask])

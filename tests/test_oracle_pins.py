@@ -490,6 +490,42 @@ def test_poisson_coeffs_rows():
                 r += 1
 
 
+def _dia_apply(coef, g, p):
+    """A p for the DIA matrix of solver_1st_member.f90 (offsets 1, nx, nx*ny):
+    neighbours outside the domain contribute nothing (their coefficients are
+    zero there)."""
+    nx, ny, nz = g.shape
+    P = np.pad(p.reshape(g.shape, order="F"), 1)
+    c = [coef[:, d].reshape(g.shape, order="F") for d in range(7)]
+    inner = P[1:-1, 1:-1, 1:-1]
+    return (c[0] * inner + c[1] * P[2:, 1:-1, 1:-1] + c[2] * P[:-2, 1:-1, 1:-1]
+            + c[3] * P[1:-1, 2:, 1:-1] + c[4] * P[1:-1, :-2, 1:-1]
+            + c[5] * P[1:-1, 1:-1, 2:] + c[6] * P[1:-1, 1:-1, :-2])
+
+
+def test_poisson_coeffs_discrete_laplacian():
+    """Away from the boundary rows the DIA rows are the negative discrete
+    Laplacian per unit volume: a field linear in the cell-centre coordinate
+    is harmonic on a stretched grid (centre spacing Dxs, cell width iDx), and
+    on a uniform grid x^2 + y^2 + z^2 gives -6 (-2 per axis)."""
+    g = stretched_grid(9, 8, 7)
+    coef, _ = Oracle(g).poisson_coeffs()
+    # centres of cells 1..n: (X(i) + X(i+1))/2, node m at N[m+1]
+    X, Y, Z = (0.5 * (N[2:n + 2] + N[3:n + 3]) for N, n in zip((g.X, g.Y, g.Z), g.shape))
+    cx, cy, cz = np.meshgrid(X, Y, Z, indexing="ij")
+    inner = (slice(1, -1),) * 3
+    for f in (cx, cy, cz, 2.0 * cx - 3.0 * cy + 0.5 * cz):
+        r = _dia_apply(coef, g, f.ravel(order="F"))
+        np.testing.assert_allclose(r[inner], 0.0, atol=1e-9)
+    from ntustcfdlab_amd.grid import Grid
+    gu = Grid(8, 7, 6)
+    cu, _ = Oracle(gu).poisson_coeffs()
+    Xu, Yu, Zu = (0.5 * (N[2:n + 2] + N[3:n + 3]) for N, n in zip((gu.X, gu.Y, gu.Z), gu.shape))
+    ux, uy, uz = np.meshgrid(Xu, Yu, Zu, indexing="ij")
+    r = _dia_apply(cu, gu, (ux ** 2 + uy ** 2 + uz ** 2).ravel(order="F"))
+    np.testing.assert_allclose(r[inner], -6.0, rtol=1e-9)
+
+
 # ---------------------------------------------------------------------------
 # whole step
 # ---------------------------------------------------------------------------
