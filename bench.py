@@ -198,7 +198,8 @@ def measure(hip, hiptime, size, steps, warmup, sweep, only, calibrate):
     w = workload(size)
     g = Grid(w["nx"], w["ny"], w["nz"])
     s = hip.ProjectionSolver(g, bc_codes())
-    s.set_sweep_variant(hip.SWEEP_FUSED if sweep == "fused" else hip.SWEEP_TWO_PASS)
+    s.set_sweep_variant({"fused": hip.SWEEP_FUSED, "two-pass": hip.SWEEP_TWO_PASS,
+                         "graph": hip.SWEEP_GRAPH}[sweep])
     if calibrate:
         hiptime.copy_calibration()
 
@@ -281,8 +282,10 @@ def main():
     ap.add_argument("--fd-directions", type=int, default=2)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-256", action="store_true", help="skip the 256^3 leg")
-    ap.add_argument("--sweep", choices=["fused", "two-pass"], default="two-pass",
-                    help="red-black sweep kernel (bit-identical results)")
+    ap.add_argument("--sweep", choices=["two-pass", "graph", "fused"], default="two-pass",
+                    help="red-black sweep launches (bit-identical results): two-pass "
+                         "kernels from the stream, the same replayed from a HIP graph, "
+                         "or the fused one-launch sweep")
     ap.add_argument("--only-adjoint", action="store_true",
                     help="run the adjoint leg alone and print its JSON")
     a = ap.parse_args()

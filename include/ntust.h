@@ -48,10 +48,14 @@ enum {
     NTUST_SOR_MIXED = 2           /* red-black, FP32 sweeps inside FP64 defect correction */
 };
 
-/* red-black sweep kernels (ntust_set_sweep_variant); both give the same bits */
+/* red-black sweep kernels (ntust_set_sweep_variant); all give the same bits */
 enum {
-    NTUST_SWEEP_TWO_PASS = 0,     /* one launch per colour, in place (default) */
-    NTUST_SWEEP_FUSED = 1         /* one launch per sweep on LDS tiles */
+    NTUST_SWEEP_TWO_PASS = 0,     /* one launch per colour, in place, from the
+                                     stream (default) */
+    NTUST_SWEEP_FUSED = 1,        /* one launch per sweep on LDS tiles */
+    NTUST_SWEEP_GRAPH = 2         /* the two-pass launches replayed from a captured
+                                     HIP graph, 32 sweeps (FP64) or one FP32
+                                     segment per replay */
 };
 
 /* advection schemes (ntust_configure scheme) */

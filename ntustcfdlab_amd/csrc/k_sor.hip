@@ -94,6 +94,8 @@ __global__ void k_sor_init(double *slots, SorControl *ctl)
     if (b == 0) {
         ctl->stop = 0;
         ctl->ik = 0;
+        ctl->tr = 1;
+        ctl->tb = 0;
     }
 }
 
@@ -187,13 +189,14 @@ __global__ void __launch_bounds__(256)
 k_rb_sweep(KGrid g, SplitGrid s, const double *__restrict__ rx,
            const double *__restrict__ ry, const double *__restrict__ rz,
            double *__restrict__ pc_own, const double *__restrict__ pc_oth,
-           const double *__restrict__ b_own, double omega, double zeta, double itmax, int t,
+           const double *__restrict__ b_own, double omega, double zeta, double itmax, int targ,
            double *slots, SorControl *ctl)
 {
     __shared__ double red[4];
     const int tid = threadIdx.x + threadIdx.y * blockDim.x;
     const int blin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
     const int lane = tid & 63;
+    const int t = sweep_index<C>(targ, ctl);
     double gate = lane < SLOT_BUCKETS ? load_flag(&slots[slot_index(t - 1, lane)]) : 0.0;
     const int stopped = load_flag(&ctl->stop);
 
@@ -258,7 +261,10 @@ k_rb_sweep(KGrid g, SplitGrid s, const double *__restrict__ rx,
         if (blin == 0 && tid == 0) ctl->stop = 1;
         return;
     }
-    if (C == 0 && blin == 0) clear_next_buckets(slots, t, tid);
+    if (blin == 0) {
+        if (C == 0) clear_next_buckets(slots, t, tid);
+        if (tid == 0) pass_sweep_index<C>(t, ctl);
+    }
 #pragma unroll
     for (int r = 0; r < RB_KPT; r++)
         if (in[r]) pc_own[at[r]] = nv[r];

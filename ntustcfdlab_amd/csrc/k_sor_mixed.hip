@@ -25,12 +25,14 @@ template <int C>
 __global__ void __launch_bounds__(256)
 k_rb_sweep32(KGrid g, SplitGrid s, FMetric fm, float *__restrict__ e_own,
              const float *__restrict__ e_oth, const float *__restrict__ r_own, float omega,
-             double zeta, double itmax, int t, double *slots, SorControl *ctl, int *dirty)
+             double zeta, double itmax, int targ, double *slots, SorControl *ctl, int *dirty,
+             int first)
 {
     __shared__ double red[4];
     const int tid = threadIdx.x + threadIdx.y * blockDim.x;
     const int blin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
     const int lane = tid & 63;
+    const int t = sweep_index<C>(targ, ctl);
     double gate = lane < SLOT_BUCKETS ? load_flag(&slots[slot_index(t - 1, lane)]) : 0.0;
     const int stopped = load_flag(&ctl->stop);
 
@@ -90,10 +92,16 @@ k_rb_sweep32(KGrid g, SplitGrid s, FMetric fm, float *__restrict__ e_own,
     gate = wave_max(gate);
     const bool active = !stopped && ((double)(t - 1) < itmax) && (gate > zeta);
     if (!active) {
-        if (blin == 0 && tid == 0) ctl->stop = 1;
+        if (blin == 0 && tid == 0) {
+            ctl->stop = 1;
+            if (first) *dirty = 0;   // no sweep of this segment runs
+        }
         return;
     }
-    if (C == 0 && blin == 0) clear_next_buckets(slots, t, tid);
+    if (blin == 0) {
+        if (C == 0) clear_next_buckets(slots, t, tid);
+        if (tid == 0) pass_sweep_index<C>(t, ctl);
+    }
 #pragma unroll
     for (int r = 0; r < RB_KPT; r++)
         if (in[r]) e_own[at[r]] = nv[r];
@@ -166,14 +174,14 @@ k_rb_residual(KGrid g, SplitGrid s, const double *__restrict__ rx,
 void launch_rb_sweep32(const KGrid &g, const SplitGrid &s, const FMetric &fm, float *e0,
                        float *e1, const float *r0, const float *r1, float omega, double zeta,
                        double itmax, int t, double *slots, SorControl *ctl, int *dirty,
-                       hipStream_t st)
+                       int first, hipStream_t st)
 {
     const int nq = (g.nx + 1) / 2;
     dim3 blk(64, 4), grd((nq + 63) / 64, (g.ny + 3) / 4, (g.nz + RB_KPT - 1) / RB_KPT);
     hipLaunchKernelGGL(k_rb_sweep32<0>, grd, blk, 0, st, g, s, fm, e0, e1, r0, omega, zeta,
-                       itmax, t, slots, ctl, dirty);
+                       itmax, t, slots, ctl, dirty, first);
     hipLaunchKernelGGL(k_rb_sweep32<1>, grd, blk, 0, st, g, s, fm, e1, e0, r1, omega, zeta,
-                       itmax, t, slots, ctl, dirty);
+                       itmax, t, slots, ctl, dirty, 0);
 }
 
 void launch_rb_absorb(const KGrid &g, const SplitGrid &s, double *p0, double *p1, float *e0,

@@ -73,8 +73,13 @@ struct Ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double prof_ms = 0.0;
     long long prof_launches = 0;
-    // red-black sweep kernel: one launch per sweep (fused) or per colour
+    // red-black sweep kernel: one launch per sweep (fused) or per colour,
+    // the latter from the stream or replayed from a captured graph
     int sweep_variant = NTUST_SWEEP_TWO_PASS;
+    struct Graph {
+        hipGraphExec_t exec = nullptr;
+        std::vector<char> key;   // bytes of the captured launch arguments
+    } graph64, graph32;
     // body force added to the predictor (device halo arrays, "body_*")
     bool body = false;
     double *fu = nullptr, *fv = nullptr, *fw = nullptr;
@@ -137,6 +142,30 @@ int d2h(double *h, const double *d, size_t n)
     } while (0)
 
 int sync() { HIPCHK(hipStreamSynchronize(C.st)); HIPCHK(hipGetLastError()); return 0; }
+
+// Captures the launches `body` makes on the library stream into an
+// executable graph, unless the graph already holds launches with the same
+// arguments (key).  The launches must read everything that changes between
+// replays from device memory.
+template <typename F>
+int capture(Ctx::Graph &G, const void *key, size_t klen, F body)
+{
+    if (G.exec && G.key.size() == klen && memcmp(G.key.data(), key, klen) == 0) return 0;
+    if (G.exec) {
+        HIPCHK(hipGraphExecDestroy(G.exec));
+        G.exec = nullptr;
+    }
+    hipGraph_t gr = nullptr;
+    HIPCHK(hipStreamBeginCapture(C.st, hipStreamCaptureModeThreadLocal));
+    body();
+    const hipError_t e = hipStreamEndCapture(C.st, &gr);
+    if (e != hipSuccess)
+        return fail(NTUST_E_HIP, "graph capture of the sweeps: %s", hipGetErrorString(e));
+    HIPCHK(hipGraphInstantiate(&G.exec, gr, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(gr));
+    G.key.assign((const char *)key, (const char *)key + klen);
+    return 0;
+}
 
 // (Cs*delta)**2 with delta = (iDx*iDy*iDz)**(1./3.), smagorinsky.f90:20-22:
 // libm pow on the host, the exponent the single-precision literal 1./3.
@@ -209,7 +238,11 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
         Q1 = b;
     }
     const long tmax = (long)fmin(itmax, 1.0e8) > 0 ? (long)fmin(itmax, 1.0e8) : 0;
-    const size_t nslot = (size_t)slot_index(tmax + 2, 0);
+    // graph replay runs whole replays: up to one replay of sweeps past tmax
+    // is launched (they do not run, but read the buckets before them)
+    const bool graphs = C.sweep_variant == NTUST_SWEEP_GRAPH;
+    const long per_replay = mode == NTUST_SOR_MIXED ? C.mix_segment : 32;
+    const size_t nslot = (size_t)slot_index(tmax + (graphs ? per_replay : 0) + 2, 0);
     NEED(slots, "sor_slots", nslot, false);
     launch_rb_gather(g, C.sg, p, P0, P1, C.st);
     if (Q0) launch_rb_gather(g, C.sg, p, Q0, Q1, C.st);   // ghosts in both copies
@@ -251,6 +284,46 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
                    C.fmetric[3] + 1, C.fmetric[4] + 1, C.fmetric[5] + 1};
         long c = 0;
         long want = C.ik_hint > seg ? C.ik_hint - seg / 2 : 2 * seg;
+        if (graphs) {
+            // one segment of sweeps per replay, the sweep index kept on the
+            // device; one dirty flag, reset by the segment's first sweep
+            struct {   // 8-byte members only: no padding bytes in the key
+                long nx, ny, nz;
+                const void *e0, *e1, *r0, *r1, *fm0, *slots, *ctl, *dirty;
+                double omega, zeta, itmax;
+                long seg;
+            } key = {g.nx, g.ny, g.nz, E0, E1, R0, R1, fm.iDx, slots, C.ctl, dirty,
+                     omega, zeta, itmax, seg};
+            TRY(capture(C.graph32, &key, sizeof key, [&] {
+                for (long b = 0; b < seg; b++)
+                    launch_rb_sweep32(g, C.sg, fm, E0, E1, R0, R1, (float)omega, zeta, itmax,
+                                      0, slots, C.ctl, dirty, b == 0 ? 1 : 0, C.st);
+            }));
+            while (t <= tmax) {
+                const long nrep = want / seg > 1 ? want / seg : 1;
+                for (long r = 0; r < nrep; r++) {
+                    if (r == 0) HIPCHK(hipEventRecord(C.ev0, C.st));
+                    HIPCHK(hipGraphLaunch(C.graph32.exec, C.st));
+                    if (r == 0) HIPCHK(hipEventRecord(C.ev1, C.st));
+                    launch_rb_absorb(g, C.sg, P0, P1, E0, E1, dirty, C.st);
+                    launch_rb_residual(g, C.sg, rx, ry, rz, P0, P1, B0, B1, R0, R1, dirty,
+                                       C.st);
+                }
+                HIPCHK(hipGetLastError());
+                HIPCHK(hipMemcpyAsync(&h, C.ctl, sizeof h, hipMemcpyDeviceToHost, C.st));
+                TRY(sync());
+                if (h.ik >= t + seg - 1) {   // every sweep of the timed replay ran
+                    float ms = 0.f;
+                    HIPCHK(hipEventElapsedTime(&ms, C.ev0, C.ev1));
+                    C.prof_ms += ms;
+                    C.prof_launches += 2 * seg;
+                }
+                if (h.stop) break;
+                t += nrep * seg;
+                want = 2 * seg;
+            }
+            t = tmax + 1;
+        }
         while (t <= tmax) {
             long queued = 0;
             bool timed = false;
@@ -267,7 +340,8 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
                                                 C.ctl, dirty + c, C.st);
                     } else {
                         launch_rb_sweep32(g, C.sg, fm, E0, E1, R0, R1, (float)omega, zeta,
-                                          itmax, (int)tt, slots, C.ctl, dirty + c, C.st);
+                                          itmax, (int)tt, slots, C.ctl, dirty + c,
+                                          b == 0 ? 1 : 0, C.st);
                     }
                 }
                 if (!timed) HIPCHK(hipEventRecord(C.ev1, C.st));
@@ -294,6 +368,41 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
             want = 2 * seg;
         }
         t = tmax + 1;   // the red-black FP64 loop below does not run
+    }
+    if (graphs && t <= tmax) {
+        // per_replay sweeps per replay, the sweep index kept on the device
+        struct {   // 8-byte members only: no padding bytes in the key
+            long nx, ny, nz;
+            const void *p0, *p1, *b0, *b1, *rx, *slots, *ctl;
+            double omega, zeta, itmax;
+            long n;
+        } key = {g.nx, g.ny, g.nz, P0, P1, B0, B1, rx, slots, C.ctl, omega, zeta, itmax,
+                 per_replay};
+        TRY(capture(C.graph64, &key, sizeof key, [&] {
+            for (long b = 0; b < per_replay; b++)
+                launch_rb_sweep(g, C.sg, rx, ry, rz, P0, P1, B0, B1, omega, zeta, itmax, 0,
+                                slots, C.ctl, C.st);
+        }));
+        long want = C.ik_hint > per_replay ? C.ik_hint - per_replay / 2 : per_replay;
+        while (t <= tmax) {
+            const long nrep = want / per_replay > 1 ? want / per_replay : 1;
+            HIPCHK(hipEventRecord(C.ev0, C.st));
+            HIPCHK(hipGraphLaunch(C.graph64.exec, C.st));
+            HIPCHK(hipEventRecord(C.ev1, C.st));
+            for (long r = 1; r < nrep; r++) HIPCHK(hipGraphLaunch(C.graph64.exec, C.st));
+            HIPCHK(hipMemcpyAsync(&h, C.ctl, sizeof h, hipMemcpyDeviceToHost, C.st));
+            TRY(sync());
+            if (h.ik >= t + per_replay - 1) {   // every sweep of the timed replay ran
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, C.ev0, C.ev1));
+                C.prof_ms += ms;
+                C.prof_launches += 2 * per_replay;
+            }
+            if (h.stop) break;
+            t += nrep * per_replay;
+            want = 2 * per_replay;
+        }
+        t = tmax + 1;
     }
     while (t <= tmax) {
         const long nb = batch < tmax - t + 1 ? batch : tmax - t + 1;
@@ -541,6 +650,8 @@ int ntust_finalize(void)
     hipError_t first = hipSuccess;
     auto keep = [&first](hipError_t e) { if (first == hipSuccess) first = e; };
     if (C.st) keep(hipStreamSynchronize(C.st));
+    if (C.graph64.exec) keep(hipGraphExecDestroy(C.graph64.exec));
+    if (C.graph32.exec) keep(hipGraphExecDestroy(C.graph32.exec));
     for (auto &b : C.bufs) keep(hipFree(b.ptr));
     C.bufs.clear();
     for (int m = 0; m < 6; m++)
@@ -1349,7 +1460,8 @@ int ntust_prof_reset(void)
 
 int ntust_set_sweep_variant(int variant)
 {
-    if (variant != NTUST_SWEEP_TWO_PASS && variant != NTUST_SWEEP_FUSED)
+    if (variant != NTUST_SWEEP_TWO_PASS && variant != NTUST_SWEEP_FUSED &&
+        variant != NTUST_SWEEP_GRAPH)
         return fail(NTUST_E_ARG, "unknown sweep variant %d", variant);
     C.sweep_variant = variant;
     return 0;

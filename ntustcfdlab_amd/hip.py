@@ -26,7 +26,7 @@ SOR_LEXICOGRAPHIC, SOR_REDBLACK, SOR_MIXED = 0, 1, 2
 QUICK_CENTRE, UPWIND = 0, 1
 WARN_ITMAX = 1
 E_STATE = -5
-SWEEP_TWO_PASS, SWEEP_FUSED = 0, 1
+SWEEP_TWO_PASS, SWEEP_FUSED, SWEEP_GRAPH = 0, 1, 2
 PROF_KERNELS = {0: ("k_rb_sweep<C>", "two-pass FP64 half-sweep"),
                 1: ("k_rb_sweep_fused<double>", "fused FP64 sweep"),
                 2: ("k_rb_sweep32<C>", "two-pass FP32 half-sweep"),
@@ -251,7 +251,8 @@ class ProjectionSolver:
         self.lib.ntust_prof_reset()
 
     def set_sweep_variant(self, variant: int):
-        """SWEEP_TWO_PASS (default) or SWEEP_FUSED; the iterates are bit-identical."""
+        """SWEEP_TWO_PASS (default), SWEEP_GRAPH or SWEEP_FUSED; the iterates are
+        bit-identical."""
         _check(self.lib.ntust_set_sweep_variant(int(variant)))
 
     def prof_kernel(self):

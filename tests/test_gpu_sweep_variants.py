@@ -1,5 +1,7 @@
 """The fused red-black sweep (one launch per sweep on LDS tiles, two copies
-of the colour planes) against the two-pass sweep (one launch per colour, in
+of the colour planes) and the graph-replayed two-pass sweep (the sweep
+index kept on the device, whole replays past the stopping sweep) against
+the two-pass sweep launched from the stream (one launch per colour, in
 place): same sweep count, same last change, same pressure, bit for bit, in
 FP64 and in the FP32-sweep solver, on grids whose extents are not
 multiples of the tile."""
@@ -9,7 +11,7 @@ import pytest
 from conftest import rand_halo, stretched_grid
 from ntustcfdlab_amd.grid import Grid, bc_codes, noise_ic
 from ntustcfdlab_amd.hip import (QUICK_CENTRE, SOR_MIXED, SOR_REDBLACK, SWEEP_FUSED,
-                                 SWEEP_TWO_PASS, ProjectionSolver)
+                                 SWEEP_GRAPH, SWEEP_TWO_PASS, ProjectionSolver)
 
 pytestmark = pytest.mark.gpu
 
@@ -17,7 +19,8 @@ pytestmark = pytest.mark.gpu
 @pytest.mark.parametrize("shape", [(40, 36, 44), (33, 9, 17), (64, 16, 8), (7, 5, 3), (70, 18, 90)])
 @pytest.mark.parametrize("mode", [SOR_REDBLACK, SOR_MIXED])
 @pytest.mark.parametrize("itmax,zeta", [(1.0, 0.0), (2.0, 0.0), (65.0, 0.0), (400.0, 1e-7)])
-def test_fused_equals_two_pass(shape, mode, itmax, zeta, hip_lib):
+@pytest.mark.parametrize("variant", [SWEEP_FUSED, SWEEP_GRAPH])
+def test_variant_equals_two_pass(shape, mode, itmax, zeta, variant, hip_lib):
     g = stretched_grid(*shape)
     s = ProjectionSolver(g)
     rng = np.random.default_rng(sum(shape) + int(itmax))
@@ -25,15 +28,15 @@ def test_fused_equals_two_pass(shape, mode, itmax, zeta, hip_lib):
     p0 = rand_halo(g, rng, 1e-3)
     out = {}
     try:
-        for variant in (SWEEP_TWO_PASS, SWEEP_FUSED):
-            s.set_sweep_variant(variant)
+        for v in (SWEEP_TWO_PASS, variant):
+            s.set_sweep_variant(v)
             p = p0.copy(order="F")
             pre = g.interior()
             ik, pm, _ = s.gauss_seidel(us, vs, ws, p, pre, 1e-3, 1.5, zeta, itmax, mode)
-            out[variant] = (ik, pm, p, pre)
+            out[v] = (ik, pm, p, pre)
     finally:
         s.close()
-    a, b = out[SWEEP_TWO_PASS], out[SWEEP_FUSED]
+    a, b = out[SWEEP_TWO_PASS], out[variant]
     assert a[0] == b[0]
     assert a[1] == b[1]
     assert np.array_equal(a[2].view(np.uint64), b[2].view(np.uint64))
@@ -41,14 +44,15 @@ def test_fused_equals_two_pass(shape, mode, itmax, zeta, hip_lib):
 
 
 @pytest.mark.parametrize("mode", [SOR_REDBLACK, SOR_MIXED])
-def test_fused_resident_steps_equal_two_pass(mode, hip_lib):
+@pytest.mark.parametrize("variant", [SWEEP_FUSED, SWEEP_GRAPH])
+def test_resident_steps_equal_two_pass(mode, variant, hip_lib):
     g = Grid(48, 40, 36)
     u0, v0, w0 = noise_ic(g, 1e-3, seed=12)
     res = {}
-    for variant in (SWEEP_TWO_PASS, SWEEP_FUSED):
+    for var in (SWEEP_TWO_PASS, variant):
         s = ProjectionSolver(g, bc_codes())
         try:
-            s.set_sweep_variant(variant)
+            s.set_sweep_variant(var)
             u, v, w, p = u0.copy(order="F"), v0.copy(order="F"), w0.copy(order="F"), g.halo()
             s.boundary_conditions(u, v, w, p)
             s.configure(dt=5e-4, nu=1e-3, Cs=0.1, LES=1, scheme=QUICK_CENTRE, omega=1.5,
@@ -57,10 +61,41 @@ def test_fused_resident_steps_equal_two_pass(mode, hip_lib):
             ik, pm, cd, cl, _ = s.step(4)
             out = [g.halo() for _ in range(4)]
             s.download(*out)
-            res[variant] = (ik, pm, out)
+            res[var] = (ik, pm, out)
         finally:
             s.close()
-    a, b = res[SWEEP_TWO_PASS], res[SWEEP_FUSED]
+    a, b = res[SWEEP_TWO_PASS], res[variant]
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
     for x, y in zip(a[2], b[2]):
         assert np.array_equal(x.view(np.uint64), y.view(np.uint64))
+
+
+@pytest.mark.parametrize("mode", [SOR_REDBLACK, SOR_MIXED])
+def test_graph_replays_follow_changed_arguments(mode, hip_lib):
+    """One solver context, solves whose captured arguments change from call
+    to call (omega, itmax, zeta) and repeat: each must
+    match the stream-launched sweeps of a separate context."""
+    g = stretched_grid(30, 22, 26)
+    rng = np.random.default_rng(77)
+    us, vs, ws = (rand_halo(g, rng, 1e-2) for _ in range(3))
+    p0 = rand_halo(g, rng, 1e-3)
+    calls = [(1.5, 0.0, 40.0), (1.3, 0.0, 40.0), (1.5, 1e-8, 500.0), (1.5, 0.0, 40.0),
+             (1.5, 0.0, 130.0), (1.2, 0.0, 7.0), (1.2, 0.0, 7.0)]
+    res = {}
+    for v in (SWEEP_TWO_PASS, SWEEP_GRAPH):
+        s = ProjectionSolver(g)
+        try:
+            s.set_sweep_variant(v)
+            out = []
+            for omega, zeta, itmax in calls:
+                p = p0.copy(order="F")
+                pre = g.interior()
+                ik, pm, _ = s.gauss_seidel(us, vs, ws, p, pre, 1e-3, omega, zeta, itmax, mode)
+                out.append((ik, pm, p, pre))
+            res[v] = out
+        finally:
+            s.close()
+    for a, b in zip(res[SWEEP_TWO_PASS], res[SWEEP_GRAPH]):
+        assert a[0] == b[0] and a[1] == b[1]
+        assert np.array_equal(a[2].view(np.uint64), b[2].view(np.uint64))
+        assert np.array_equal(a[3].view(np.uint64), b[3].view(np.uint64))
