@@ -38,7 +38,8 @@ enum {
     NTUST_E_ARG = -2,
     NTUST_E_HIP = -3,
     NTUST_E_ALLOC = -4,
-    NTUST_E_STATE = -5           /* a recomputation did not reproduce the forward pass */
+    NTUST_E_STATE = -5,          /* a recomputation did not reproduce the forward pass */
+    NTUST_E_IO = -6              /* a file could not be written or read, or is malformed */
 };
 
 /* pressure-solver orderings (ntust_configure sor_mode) */
@@ -193,6 +194,51 @@ int ntust_state_download(double *u, double *v, double *w, double *p, double *pre
  * NULL): SOR sweep count, last max|dp|, cDrag, cLift.  Returns
  * NTUST_WARN_ITMAX if any step hit the sweep cap. */
 int ntust_step(int nsteps, int *ik, double *pchange, double *cdrag, double *clift);
+
+/* ---- output, steady check and restart (resident state) ----------------- */
+
+/* filereachtime (filer.f90:161-187): packs Qout(nx,ny,nz,5) = real*4 of
+ * pre, uc, vc, wc, ETA on the device and copies it (20 B per cell) into
+ * qout (the caller's real*4 array, or a library buffer when NULL).  With a
+ * path, appends the four records nblocks; nx ny nz; temp x4 (1.0,
+ * variables_module.f90:363); Qout as Fortran unformatted sequential records
+ * (4-byte length before and after each), byte for byte what the
+ * reference's write statements produce.  The caller names the file
+ * (the reference: num in I4.4 plus '.q') and counts num. */
+int ntust_output_q(const char *path, float *qout);
+
+/* Reads a one-block solution file of this grid into qout (the reading
+ * half of reading_variables, reading_data.f90:47-53).  Unlike the
+ * reference, which reads inx, iny, inz and ignores them, a file of another
+ * grid is refused (NTUST_E_ARG). */
+int ntust_read_q(const char *path, float *qout);
+
+/* reading_variables (reading_data.f90:56-83) on the resident state: u, v, w
+ * = 0.5*(Q(i)+Q(i+1)) in single precision for i, j, k = 1..n-1, p and pre =
+ * Q1, uc/vc/wc re-centred; then the boundary conditions, the call that
+ * follows it (main.f90:96-100).  The state must have been uploaded
+ * (ntust_state_upload): entries the reference leaves alone keep it. */
+int ntust_reading_variables(const float *qout);
+
+/* mesh.x of gridder.f90:355-378 from the cell centres Xs(1:nx), Ys(1:ny),
+ * Zs(1:nz) (variables_module.f90:319-323); host only. */
+int ntust_write_mesh_x(const char *path, const double *Xs, const double *Ys, const double *Zs);
+
+/* check_steady (check_steady.f90:1-30) on the resident wc: when istep !=
+ * isto, *velocity_difference = max |wc - last_velocity| over the interior
+ * (otherwise it is left as it is); then last_velocity = wc.  last_velocity
+ * is held on the device, zero until the first call, as the module array. */
+int ntust_check_steady(int istep, int isto, double *velocity_difference);
+
+/* Exact FP64 restart, replacing the single-precision .q restart of
+ * reading_variables: writes the resident u, v, w, p, ETA (halo), pre, uc,
+ * vc, wc and last_velocity (interior) with a header (grid, istep, time)
+ * and an FNV-1a hash of the payload.  A run continued from the file is
+ * bit-identical to the uninterrupted run.  Reading refuses another grid
+ * (NTUST_E_ARG) and a damaged file (NTUST_E_IO), in which case the
+ * resident state is unchanged. */
+int ntust_restart_write(const char *path, int istep, double time);
+int ntust_restart_read(const char *path, int *istep, double *time);
 
 /* ---- discrete adjoint of the resident step ------------------------------
  *

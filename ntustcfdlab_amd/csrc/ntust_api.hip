@@ -634,6 +634,64 @@ int copy3(double *du, double *dv, double *dw, const double *su, const double *sv
     return 0;
 }
 
+// ---- files ------------------------------------------------------------------
+
+// One Fortran unformatted sequential record: 4-byte length, the data, the
+// length again (the record layout gfortran and flang write by default).
+bool put_record(FILE *f, const void *data, size_t len)
+{
+    if (len > 0x7fffffffu) return false;
+    const int32_t n = (int32_t)len;
+    return fwrite(&n, 4, 1, f) == 1 && (len == 0 || fwrite(data, 1, len, f) == len) &&
+           fwrite(&n, 4, 1, f) == 1;
+}
+
+// Reads one record of exactly `len` bytes (false on any other length).
+bool get_record(FILE *f, void *data, size_t len)
+{
+    int32_t a = 0, b = 0;
+    if (fread(&a, 4, 1, f) != 1 || a < 0 || (size_t)a != len) return false;
+    if (len && fread(data, 1, len, f) != len) return false;
+    return fread(&b, 4, 1, f) == 1 && b == a;
+}
+
+// Exact FP64 restart file: a 64-byte header, then the payload arrays in
+// the order of restart_arrays(), each as raw little-endian doubles.  The
+// hash is FNV-1a over the payload's 64-bit words.
+struct RestartHeader {
+    char magic[8];            // "NTUSTRS1"
+    uint32_t version;         // 1
+    int32_t nx, ny, nz;
+    int32_t istep;
+    int32_t narrays;
+    double time;
+    uint64_t payload_bytes;
+    uint64_t hash;
+    uint8_t pad[8];
+};
+static_assert(sizeof(RestartHeader) == 64, "restart header layout");
+
+constexpr uint64_t FNV_OFFSET = 14695981039346656037ull, FNV_PRIME = 1099511628211ull;
+
+uint64_t fnv_words(uint64_t h, const double *a, size_t n)
+{
+    for (size_t q = 0; q < n; q++) {
+        uint64_t w;
+        memcpy(&w, a + q, 8);
+        h = (h ^ w) * FNV_PRIME;
+    }
+    return h;
+}
+
+// The resident state a step continues from bit for bit (u, v, w, p, ETA;
+// halo), the outputs of the last step (pre, uc, vc, wc) and the steady
+// check's previous wc (interior).
+struct RestartArray { const char *name; bool halo; };
+constexpr RestartArray restart_arrays[] = {
+    {"u", true},    {"v", true},    {"w", true},    {"p", true},  {"eta", true},
+    {"pre", false}, {"uc", false},  {"vc", false},  {"wc", false}, {"last_velocity", false}};
+constexpr int N_RESTART = sizeof(restart_arrays) / sizeof(restart_arrays[0]);
+
 }  // namespace
 
 // =====================================================================
@@ -1481,6 +1539,225 @@ int ntust_prof_read(double *ms, long long *launches)
 {
     if (ms) *ms = C.prof_ms;
     if (launches) *launches = C.prof_launches;
+    return 0;
+}
+
+// ---- output, steady check, restart --------------------------------------
+
+int ntust_output_q(const char *path, float *qout)
+{
+    NEED_INIT();
+    StepBufs b;
+    TRY(step_buffers(b, false));
+    const size_t nq = 5 * (size_t)C.ni;
+    NEED(qd, "qout", (nq + 1) / 2, false);
+    float *q = (float *)qd;
+    launch_pack_q(C.g, b.pre, b.uc, b.vc, b.wc, b.eta, q, C.st);
+    HIPCHK(hipGetLastError());
+    std::vector<float> own;
+    float *h = qout;
+    if (!h) {
+        own.resize(nq);
+        h = own.data();
+    }
+    HIPCHK(hipMemcpyAsync(h, q, nq * sizeof(float), hipMemcpyDeviceToHost, C.st));
+    TRY(sync());
+    if (!path) return 0;
+    FILE *f = fopen(path, "ab");   // position='append' (filer.f90:177)
+    if (!f) return fail(NTUST_E_IO, "cannot open %s for appending", path);
+    const int32_t nb = 1, dims[3] = {C.nx, C.ny, C.nz};
+    const float temp[4] = {1.0f, 1.0f, 1.0f, 1.0f};   // variables_module.f90:363
+    bool ok = put_record(f, &nb, 4) && put_record(f, dims, 12) && put_record(f, temp, 16) &&
+              put_record(f, h, nq * sizeof(float));
+    ok = (fclose(f) == 0) && ok;
+    if (!ok) return fail(NTUST_E_IO, "writing %s failed (a record over 2 GiB?)", path);
+    return 0;
+}
+
+int ntust_read_q(const char *path, float *qout)
+{
+    NEED_INIT();
+    if (!path || !qout) return fail(NTUST_E_ARG, "NULL argument");
+    FILE *f = fopen(path, "rb");
+    if (!f) return fail(NTUST_E_IO, "cannot open %s", path);
+    int32_t nb = 0, dims[3] = {0, 0, 0};
+    float temp[4];
+    bool ok = get_record(f, &nb, 4) && get_record(f, dims, 12);
+    if (ok && (dims[0] != C.nx || dims[1] != C.ny || dims[2] != C.nz)) {
+        fclose(f);
+        return fail(NTUST_E_ARG, "%s holds a %dx%dx%d grid, the library %dx%dx%d", path,
+                    dims[0], dims[1], dims[2], C.nx, C.ny, C.nz);
+    }
+    ok = ok && get_record(f, temp, 16) &&
+         get_record(f, qout, 5 * (size_t)C.ni * sizeof(float));
+    fclose(f);
+    if (!ok) return fail(NTUST_E_IO, "%s is not a one-block PLOT3D solution file", path);
+    return 0;
+}
+
+int ntust_reading_variables(const float *qout)
+{
+    NEED_INIT();
+    if (!qout) return fail(NTUST_E_ARG, "NULL Qout");
+    StepBufs b;
+    TRY(step_buffers(b, false));
+    const size_t nq = 5 * (size_t)C.ni;
+    NEED(qd, "qout", (nq + 1) / 2, false);
+    float *q = (float *)qd;
+    HIPCHK(hipMemcpyAsync(q, qout, nq * sizeof(float), hipMemcpyHostToDevice, C.st));
+    launch_unpack_q(C.g, q, b.u, b.v, b.w, b.p, b.pre, b.uc, b.vc, b.wc, C.st);
+    // main.f90:100, the call that follows reading_variables
+    launch_boundary_conditions(C.g, C.bc, b.u, b.v, b.w, b.p, C.st);
+    launch_copy_shell(C.g, b.u, b.v, b.w, b.us, b.vs, b.ws, b.u2, b.v2, b.w2, C.st);
+    HIPCHK(hipGetLastError());
+    return sync();
+}
+
+int ntust_write_mesh_x(const char *path, const double *Xs, const double *Ys, const double *Zs)
+{
+    NEED_INIT();
+    if (!path || !Xs || !Ys || !Zs) return fail(NTUST_E_ARG, "NULL argument");
+    const size_t ni = (size_t)C.ni;
+    std::vector<float> xyz(3 * ni);
+    size_t c = 0;
+    for (int k = 0; k < C.nz; k++)
+        for (int j = 0; j < C.ny; j++)
+            for (int i = 0; i < C.nx; i++, c++) {
+                xyz[c] = (float)Xs[i];
+                xyz[c + ni] = (float)Ys[j];
+                xyz[c + 2 * ni] = (float)Zs[k];
+            }
+    FILE *f = fopen(path, "wb");
+    if (!f) return fail(NTUST_E_IO, "cannot open %s", path);
+    const int32_t nb = 1, dims[3] = {C.nx, C.ny, C.nz};
+    bool ok = put_record(f, &nb, 4) && put_record(f, dims, 12) &&
+              put_record(f, xyz.data(), xyz.size() * sizeof(float));
+    ok = (fclose(f) == 0) && ok;
+    if (!ok) return fail(NTUST_E_IO, "writing %s failed (a record over 2 GiB?)", path);
+    return 0;
+}
+
+int ntust_check_steady(int istep, int isto, double *velocity_difference)
+{
+    NEED_INIT();
+    if (!velocity_difference) return fail(NTUST_E_ARG, "NULL velocity_difference");
+    StepBufs b;
+    TRY(step_buffers(b, false));
+    NEED(last, "last_velocity", (size_t)C.ni, true);
+    NEED(vm, "steady_max", 1, false);
+    const int compare = istep != isto;   // check_steady.f90:6
+    unsigned long long *vmax = (unsigned long long *)vm;
+    if (compare) HIPCHK(hipMemsetAsync(vmax, 0, sizeof *vmax, C.st));
+    launch_steady(C.g, b.wc, last, compare, vmax, C.st);
+    HIPCHK(hipGetLastError());
+    double m = 0.0;
+    if (compare) HIPCHK(hipMemcpyAsync(&m, vmax, sizeof m, hipMemcpyDeviceToHost, C.st));
+    TRY(sync());
+    if (compare) *velocity_difference = m;
+    return 0;
+}
+
+int ntust_restart_write(const char *path, int istep, double time)
+{
+    NEED_INIT();
+    if (!path) return fail(NTUST_E_ARG, "NULL path");
+    StepBufs b;
+    TRY(step_buffers(b, false));
+    NEED(last, "last_velocity", (size_t)C.ni, true);
+    (void)last;
+    RestartHeader hd{};
+    memcpy(hd.magic, "NTUSTRS1", 8);
+    hd.version = 1;
+    hd.nx = C.nx;
+    hd.ny = C.ny;
+    hd.nz = C.nz;
+    hd.istep = istep;
+    hd.narrays = N_RESTART;
+    hd.time = time;
+    hd.hash = FNV_OFFSET;
+    FILE *f = fopen(path, "wb");
+    if (!f) return fail(NTUST_E_IO, "cannot open %s", path);
+    bool ok = fwrite(&hd, sizeof hd, 1, f) == 1;   // rewritten with the hash below
+    std::vector<double> h((size_t)C.nh);
+    for (int a = 0; a < N_RESTART && ok; a++) {
+        const size_t n = restart_arrays[a].halo ? (size_t)C.nh : (size_t)C.ni;
+        const double *d = buf(restart_arrays[a].name, n, false);
+        if (!d) {
+            fclose(f);
+            return fail(NTUST_E_ALLOC, "device buffer %s", restart_arrays[a].name);
+        }
+        if (d2h(h.data(), d, n) < 0 || sync() < 0) {
+            fclose(f);
+            return NTUST_E_HIP;
+        }
+        hd.hash = fnv_words(hd.hash, h.data(), n);
+        hd.payload_bytes += n * sizeof(double);
+        ok = fwrite(h.data(), sizeof(double), n, f) == n;
+    }
+    ok = ok && fseek(f, 0, SEEK_SET) == 0 && fwrite(&hd, sizeof hd, 1, f) == 1;
+    ok = (fclose(f) == 0) && ok;
+    if (!ok) return fail(NTUST_E_IO, "writing %s failed", path);
+    return 0;
+}
+
+int ntust_restart_read(const char *path, int *istep, double *time)
+{
+    NEED_INIT();
+    if (!path) return fail(NTUST_E_ARG, "NULL path");
+    FILE *f = fopen(path, "rb");
+    if (!f) return fail(NTUST_E_IO, "cannot open %s", path);
+    RestartHeader hd{};
+    const size_t want = (size_t)5 * C.nh + (size_t)5 * C.ni;
+    if (fread(&hd, sizeof hd, 1, f) != 1 || memcmp(hd.magic, "NTUSTRS1", 8) != 0 ||
+        hd.version != 1 || hd.narrays != N_RESTART) {
+        fclose(f);
+        return fail(NTUST_E_IO, "%s is not a version-1 restart file", path);
+    }
+    if (hd.nx != C.nx || hd.ny != C.ny || hd.nz != C.nz) {
+        fclose(f);
+        return fail(NTUST_E_ARG, "%s holds a %dx%dx%d grid, the library %dx%dx%d", path, hd.nx,
+                    hd.ny, hd.nz, C.nx, C.ny, C.nz);
+    }
+    if (hd.payload_bytes != want * sizeof(double)) {
+        fclose(f);
+        return fail(NTUST_E_IO, "%s: payload of %llu bytes, expected %zu", path,
+                    (unsigned long long)hd.payload_bytes, want * sizeof(double));
+    }
+    // first pass: the whole payload must hash right before the state changes
+    std::vector<double> h((size_t)C.nh);
+    uint64_t hash = FNV_OFFSET;
+    bool ok = true;
+    for (int a = 0; a < N_RESTART && ok; a++) {
+        const size_t n = restart_arrays[a].halo ? (size_t)C.nh : (size_t)C.ni;
+        ok = fread(h.data(), sizeof(double), n, f) == n;
+        if (ok) hash = fnv_words(hash, h.data(), n);
+    }
+    if (!ok || hash != hd.hash) {
+        fclose(f);
+        return fail(NTUST_E_IO, "%s: %s", path, ok ? "payload hash mismatch" : "file truncated");
+    }
+    StepBufs b;
+    TRY(step_buffers(b, false));
+    NEED(last, "last_velocity", (size_t)C.ni, true);
+    (void)last;
+    ok = fseek(f, (long)sizeof hd, SEEK_SET) == 0;
+    for (int a = 0; a < N_RESTART && ok; a++) {
+        const size_t n = restart_arrays[a].halo ? (size_t)C.nh : (size_t)C.ni;
+        double *d = buf(restart_arrays[a].name, n, false);
+        ok = d && fread(h.data(), sizeof(double), n, f) == n;
+        if (ok && (h2d(d, h.data(), n) < 0 || sync() < 0)) {
+            fclose(f);
+            return NTUST_E_HIP;
+        }
+    }
+    fclose(f);
+    if (!ok) return fail(NTUST_E_IO, "%s: reading the payload failed", path);
+    // the copies of u, v, w the step keeps (as ntust_state_upload)
+    launch_copy_shell(C.g, b.u, b.v, b.w, b.us, b.vs, b.ws, b.u2, b.v2, b.w2, C.st);
+    HIPCHK(hipGetLastError());
+    TRY(sync());
+    if (istep) *istep = hd.istep;
+    if (time) *time = hd.time;
     return 0;
 }
 

@@ -253,6 +253,14 @@ void launch_adj_predict(const KGrid &g, int upwind, const double *u, const doubl
 void launch_adj_smagorinsky(const KGrid &g, const double *u, const double *v, const double *w,
                             const double *filt, const double *lvisc, double *lu, double *lv,
                             double *lw, hipStream_t st);
+// output and restart (k_output.hip)
+void launch_pack_q(const KGrid &g, const double *pre, const double *uc, const double *vc,
+                   const double *wc, const double *eta, float *q, hipStream_t st);
+void launch_steady(const KGrid &g, const double *wc, double *last, int compare,
+                   unsigned long long *vmax, hipStream_t st);
+void launch_unpack_q(const KGrid &g, const float *q, double *u, double *v, double *w, double *p,
+                     double *pre, double *uc, double *vc, double *wc, hipStream_t st);
+
 // kinetic energy 0.5 sum (u^2+v^2+w^2) iDx iDy iDz over the interior: block
 // partial sums into part[], returns their count
 int launch_kinetic_energy(const KGrid &g, const double *u, const double *v, const double *w,

@@ -8,9 +8,17 @@
 ! through the drop-in shims of hotpath.f90 (mode 0), or the same step fused
 ! on the device with the state resident in HBM (mode 1).
 !
-! usage: ntust_driver OUT NX NY NZ RE DT NSTEP SCHEME LES CS OMEGA ZETA ITMAX SOR MODE [IC]
+! usage: ntust_driver OUT NX NY NZ RE DT NSTEP SCHEME LES CS OMEGA ZETA ITMAX SOR MODE
+!                     [IC [ISTO]]
 !   SCHEME 0 QUICK_centre, 1 upwind; SOR 0 lexicographic, 1 red-black;
-!   IC: optional raw file with u, v, w (halo arrays, real*8) as initial state.
+!   IC: optional raw file with u, v, w (halo arrays, real*8) as initial state
+!   ('-' for none).
+!   ISTO > 0: the output of main.f90:113-118,315-328 -- the solution file
+!   NNNN.q at the start and every ISTO steps, and check_steady after each
+!   but the first -- written into the working directory, in mode 0 by
+!   Fortran I/O from the host arrays as the solver's own routines do, in
+!   mode 1 by the library from the resident state; the VelocityDifference
+!   values go to vdiff.bin (stream: count int32, values real*8).
 ! OUT receives (stream, native endianness): nstep, ik(nstep) int32,
 !   cDrag(nstep), cLift(nstep), u, v, w, p (halo arrays), pre, uc, vc, wc.
 program ntust_driver
@@ -18,9 +26,9 @@ program ntust_driver
    use ntust_hip
    implicit none
    character(len=512) :: out, icfile, arg
-   integer :: scheme, sor, mode, s, lun
+   integer :: scheme, sor, mode, s, n, lun, nout
    integer(c_int), allocatable :: iks(:)
-   real*8, allocatable :: cds(:), cls(:), pcs(:)
+   real*8, allocatable :: cds(:), cls(:), pcs(:), vds(:)
    real*8 :: t0, t1
 
    call get_command_argument(1, out)
@@ -30,6 +38,9 @@ program ntust_driver
    sor = iarg(14); mode = iarg(15)
    icfile = ''
    if (command_argument_count() >= 16) call get_command_argument(16, icfile)
+   if (icfile == '-') icfile = ''
+   isto = 0
+   if (command_argument_count() >= 17) isto = iarg(17)
 
    nu = 1./Re                      ! reading_data.f90:38
    call allocate_fields()
@@ -44,12 +55,14 @@ program ntust_driver
    end if
    call boundary_conditions()      ! main.f90:100
 
-   allocate (iks(nstep), cds(nstep), cls(nstep), pcs(nstep))
+   allocate (iks(nstep), cds(nstep), cls(nstep), pcs(nstep), vds(nstep + 1))
+   nout = 0
    call cpu_time(t0)
    ! the SOR ordering the gauss_seidel shim uses
    call ntust_check(ntust_configure(dt, nu, Cs, LES, scheme, omega, zeta, itmax, sor, lx, 1), &
                     'ntust_configure')
    if (mode == 0) then
+      if (isto > 0) call output_host()   ! main.f90:117
       do istep = 1, nstep
          call CalculateSmagorinskyViscosity()
          if (scheme == 0) then
@@ -64,10 +77,36 @@ program ntust_driver
          call boundary_conditions()
          time = time + dt
          iks(istep) = ik; cds(istep) = cDrag; cls(istep) = cLift
+         if (isto > 0) then
+            if (mod(istep, isto) == 0) then   ! main.f90:317-328
+               VelocityDifference = 1024
+               call output_host()
+               call steady_host()
+               nout = nout + 1; vds(nout) = VelocityDifference
+            end if
+         end if
       end do
    else
       call ntust_check(ntust_state_upload(u, v, w, p, ETA), 'ntust_state_upload')
-      call ntust_check(ntust_step(nstep, iks, pcs, cds, cls), 'ntust_step')
+      if (isto > 0) then
+         call output_resident()
+         s = 1
+         do while (s <= nstep)
+            n = min(isto, nstep - s + 1)
+            call ntust_check(ntust_step(n, iks(s:), pcs(s:), cds(s:), cls(s:)), 'ntust_step')
+            istep = s + n - 1
+            if (mod(istep, isto) == 0) then
+               VelocityDifference = 1024
+               call output_resident()
+               call ntust_check(ntust_check_steady(istep, isto, VelocityDifference), &
+                                'ntust_check_steady')
+               nout = nout + 1; vds(nout) = VelocityDifference
+            end if
+            s = s + n
+         end do
+      else
+         call ntust_check(ntust_step(nstep, iks, pcs, cds, cls), 'ntust_step')
+      end if
       call ntust_check(ntust_state_download(u, v, w, p, pre, uc, vc, wc), 'ntust_state_download')
    end if
    call cpu_time(t1)
@@ -77,9 +116,58 @@ program ntust_driver
    write (lun) int(nstep, c_int)
    write (lun) iks, cds, cls, u, v, w, p, pre, uc, vc, wc
    close (lun)
+   if (isto > 0) then
+      open (newunit=lun, file='vdiff.bin', access='stream', form='unformatted', &
+            status='replace')
+      write (lun) int(nout, c_int), vds(1:nout)
+      close (lun)
+   end if
    call ntust_teardown()
 
 contains
+
+   ! the solution file from the host arrays, by Fortran I/O in the form of
+   ! filer.f90:161-187 (appended to NNNN.q, num counts the files)
+   subroutine output_host()
+      Qout(:, :, :, 1) = pre
+      Qout(:, :, :, 2) = uc
+      Qout(:, :, :, 3) = vc
+      Qout(:, :, :, 4) = wc
+      Qout(:, :, :, 5) = ETA(1:nx, 1:ny, 1:nz)
+      write (filename, '(I4.4)') num
+      open (newunit=lun, file=trim(filename)//'.q', position='append', form='unformatted')
+      write (lun) nblocks
+      write (lun) nx, ny, nz
+      write (lun) temp, temp, temp, temp
+      write (lun) Qout
+      close (lun)
+      num = num + 1
+   end subroutine output_host
+
+   ! the same file from the resident state: Qout packed on the device
+   subroutine output_resident()
+      write (filename, '(I4.4)') num
+      call ntust_check(ntust_output_q(trim(filename)//'.q'//c_null_char, Qout), &
+                       'ntust_output_q')
+      num = num + 1
+   end subroutine output_resident
+
+   ! check_steady.f90:1-30 on the host arrays
+   subroutine steady_host()
+      integer :: i, j, k
+      if (istep /= isto) then
+         VelocityDifference = 0
+         do k = 1, nz
+            do j = 1, ny
+               do i = 1, nx
+                  if (abs(wc(i, j, k) - last_velocity(i, j, k)) > VelocityDifference) &
+                     VelocityDifference = abs(wc(i, j, k) - last_velocity(i, j, k))
+               end do
+            end do
+         end do
+      end if
+      last_velocity(1:nx, 1:ny, 1:nz) = wc
+   end subroutine steady_host
 
    integer function iarg(n)
       integer, intent(in) :: n

@@ -8,7 +8,7 @@ module ntust_hip
    use iso_c_binding
    implicit none
 
-   integer(c_int), parameter :: NTUST_WARN_ITMAX = 1
+   integer(c_int), parameter :: NTUST_WARN_ITMAX = 1, NTUST_E_IO = -6
    integer(c_int), parameter :: NTUST_SOR_LEXICOGRAPHIC = 0, NTUST_SOR_REDBLACK = 1, &
                                 NTUST_SOR_MIXED = 2
    integer(c_int), parameter :: NTUST_QUICK_CENTRE = 0, NTUST_UPWIND = 1
@@ -154,6 +154,56 @@ module ntust_hip
             bind(C, name="ntust_set_sweep_variant")
          import :: c_int
          integer(c_int), value :: variant
+      end function
+
+      ! output, steady check and restart on the resident state; paths are
+      ! C strings (trim(name)//c_null_char)
+      integer(c_int) function ntust_output_q(path, qout) bind(C, name="ntust_output_q")
+         import :: c_int, c_char, c_float
+         character(kind=c_char), dimension(*), intent(in) :: path
+         real(c_float) :: qout(*)
+      end function
+
+      integer(c_int) function ntust_read_q(path, qout) bind(C, name="ntust_read_q")
+         import :: c_int, c_char, c_float
+         character(kind=c_char), dimension(*), intent(in) :: path
+         real(c_float) :: qout(*)
+      end function
+
+      integer(c_int) function ntust_reading_variables(qout) &
+            bind(C, name="ntust_reading_variables")
+         import :: c_int, c_float
+         real(c_float), intent(in) :: qout(*)
+      end function
+
+      integer(c_int) function ntust_write_mesh_x(path, Xs, Ys, Zs) &
+            bind(C, name="ntust_write_mesh_x")
+         import :: c_int, c_char, c_double
+         character(kind=c_char), dimension(*), intent(in) :: path
+         real(c_double), intent(in) :: Xs(*), Ys(*), Zs(*)
+      end function
+
+      integer(c_int) function ntust_check_steady(istep, isto, vdiff) &
+            bind(C, name="ntust_check_steady")
+         import :: c_int, c_double
+         integer(c_int), value :: istep, isto
+         real(c_double), intent(inout) :: vdiff
+      end function
+
+      integer(c_int) function ntust_restart_write(path, istep, time) &
+            bind(C, name="ntust_restart_write")
+         import :: c_int, c_char, c_double
+         character(kind=c_char), dimension(*), intent(in) :: path
+         integer(c_int), value :: istep
+         real(c_double), value :: time
+      end function
+
+      integer(c_int) function ntust_restart_read(path, istep, time) &
+            bind(C, name="ntust_restart_read")
+         import :: c_int, c_char, c_double
+         character(kind=c_char), dimension(*), intent(in) :: path
+         integer(c_int), intent(out) :: istep
+         real(c_double), intent(out) :: time
       end function
 
       ! members of the Krylov pressure path (solver_1st_member.f90,

@@ -37,6 +37,16 @@ module variables
    ! grid: nodes (-1:n+3) and metrics (-1:n+2)
    real*8, allocatable, dimension(:) :: X, Y, Z, iDx, Dxs, iDy, Dys, iDz, Dzs
 
+   ! output (variables_module.f90:201-205,223,355-365): solution array,
+   ! file counter, steady check
+   integer, parameter :: nblocks = 1
+   real, allocatable, dimension(:, :, :, :) :: Qout
+   real :: temp = 1.0
+   integer :: num = 0, isto = 100
+   character(len=20) :: filename
+   real*8 :: VelocityDifference
+   real*8, allocatable, dimension(:, :, :) :: last_velocity
+
 contains
 
    subroutine allocate_fields()
@@ -48,6 +58,8 @@ contains
       allocate (uc(nx, ny, nz), vc(nx, ny, nz), wc(nx, ny, nz), pre(nx, ny, nz), &
                 Viseff(nx, ny, nz), source=0.0d0)
       allocate (dudx(nx, ny, nz, 3), dvdx(nx, ny, nz, 3), dwdx(nx, ny, nz, 3), source=0.0d0)
+      allocate (Qout(nx, ny, nz, 5), source=0.0)
+      allocate (last_velocity(-1:nx + 2, -1:ny + 2, -1:nz + 2), source=0.0d0)
       allocate (X(-1:nx + 3), Y(-1:ny + 3), Z(-1:nz + 3))
       allocate (iDx(-1:nx + 2), Dxs(-1:nx + 2), iDy(-1:ny + 2), Dys(-1:ny + 2), &
                 iDz(-1:nz + 2), Dzs(-1:nz + 2))

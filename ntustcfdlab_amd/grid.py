@@ -125,6 +125,13 @@ class Grid:
     def metrics(self):
         return (self.iDx, self.Dxs, self.iDy, self.Dys, self.iDz, self.Dzs)
 
+    def centres(self):
+        """Cell centres Xs(1:nx), Ys, Zs: (X(i+1) + X(i)) / 2.0
+        (gridder.f90:173-183, 343-353)."""
+        def c(N, n):
+            return np.ascontiguousarray((N[3:n + 3] + N[2:n + 2]) / 2.0)
+        return c(self.X, self.nx), c(self.Y, self.ny), c(self.Z, self.nz)
+
 
 def bc_codes(**over) -> np.ndarray:
     """The 18 BC codes, default = the reference's cavity

@@ -26,6 +26,7 @@ SOR_LEXICOGRAPHIC, SOR_REDBLACK, SOR_MIXED = 0, 1, 2
 QUICK_CENTRE, UPWIND = 0, 1
 WARN_ITMAX = 1
 E_STATE = -5
+E_IO = -6
 SWEEP_TWO_PASS, SWEEP_FUSED, SWEEP_GRAPH = 0, 1, 2
 PROF_KERNELS = {0: ("k_rb_sweep<C>", "two-pass FP64 half-sweep"),
                 1: ("k_rb_sweep_fused<double>", "fused FP64 sweep"),
@@ -33,6 +34,7 @@ PROF_KERNELS = {0: ("k_rb_sweep<C>", "two-pass FP64 half-sweep"),
                 3: ("k_rb_sweep_fused<float>", "fused FP32 sweep")}
 
 _D = ct.POINTER(ct.c_double)
+_F = ct.POINTER(ct.c_float)
 _I = ct.POINTER(ct.c_int)
 _lib = None
 
@@ -85,6 +87,13 @@ def load_library() -> ct.CDLL:
         "ntust_poisson_rhs": [_D, _D, _D, d, _D],
         "ntust_pressure_field": [_D, _D, _D],
         "ntust_prof_kernel": [_D],
+        "ntust_output_q": [ct.c_char_p, _F],
+        "ntust_read_q": [ct.c_char_p, _F],
+        "ntust_reading_variables": [_F],
+        "ntust_write_mesh_x": [ct.c_char_p, _D, _D, _D],
+        "ntust_check_steady": [i, i, _D],
+        "ntust_restart_write": [ct.c_char_p, i, d],
+        "ntust_restart_read": [ct.c_char_p, _I, _D],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -277,6 +286,50 @@ class ProjectionSolver:
         ke = ct.c_double()
         _check(self.lib.ntust_kinetic_energy(ct.byref(ke)))
         return ke.value
+
+    # ---- output, steady check, restart (resident state) ----
+
+    def _q(self):
+        g = self.grid
+        return np.zeros((g.nx, g.ny, g.nz, 5), np.float32, order="F")
+
+    def output_q(self, path=None) -> np.ndarray:
+        """filereachtime (filer.f90:161-187): Qout from the resident state;
+        with a path, the PLOT3D records are appended to it."""
+        q = self._q()
+        _check(self.lib.ntust_output_q(None if path is None else str(path).encode(),
+                                       q.ctypes.data_as(_F)))
+        return q
+
+    def read_q(self, path) -> np.ndarray:
+        q = self._q()
+        _check(self.lib.ntust_read_q(str(path).encode(), q.ctypes.data_as(_F)))
+        return q
+
+    def reading_variables(self, q: np.ndarray):
+        """reading_data.f90:47-83 on the resident state, then the boundary
+        conditions (main.f90:96-100)."""
+        if q.dtype != np.float32 or not q.flags.f_contiguous or q.size != 5 * self.grid.cells:
+            raise TypeError("Qout must be float32 (nx,ny,nz,5) in Fortran order")
+        _check(self.lib.ntust_reading_variables(q.ctypes.data_as(_F)))
+
+    def write_mesh_x(self, path, Xs, Ys, Zs):
+        _check(self.lib.ntust_write_mesh_x(str(path).encode(), _p(Xs), _p(Ys), _p(Zs)))
+
+    def check_steady(self, istep: int, isto: int, velocity_difference: float) -> float:
+        """check_steady.f90:1-30; returns the new VelocityDifference (the
+        given one when istep == isto)."""
+        d = ct.c_double(velocity_difference)
+        _check(self.lib.ntust_check_steady(int(istep), int(isto), ct.byref(d)))
+        return d.value
+
+    def restart_write(self, path, istep: int, time: float):
+        _check(self.lib.ntust_restart_write(str(path).encode(), int(istep), float(time)))
+
+    def restart_read(self, path):
+        st, t = ct.c_int(), ct.c_double()
+        _check(self.lib.ntust_restart_read(str(path).encode(), ct.byref(st), ct.byref(t)))
+        return st.value, t.value
 
     def checkpoint_reserve(self, n: int):
         _check(self.lib.ntust_checkpoint_reserve(int(n)))

@@ -14,6 +14,8 @@
 #include "ntust_oracle.h"
 
 #include <math.h>
+#include <stdint.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -967,4 +969,135 @@ int or_step(const or_grid *g, const or_params *prm, or_fields *f,
     or_boundary_conditions(g, prm->bc, f->u, f->v, f->w, f->p, f->u1, f->v1,
                            f->w1, f->u2, f->v2, f->w2, f->us, f->vs, f->ws);
     return ik;
+}
+
+/* ------------------------------------------------------------------ */
+/* output, steady check and the .q restart reader                      */
+/* ------------------------------------------------------------------ */
+
+/* filer.f90:163-171: Qout(:,:,:,1..5) = pre, uc, vc, wc, ETA, each value
+ * assigned to a default real (round to nearest even). */
+void or_pack_q(const or_grid *g, const double *pre, const double *uc,
+               const double *vc, const double *wc, const double *ETA,
+               float *qout)
+{
+    GRID_LOCALS(g);
+    const size_t ni = (size_t)nx * ny * nz;
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                const size_t c = I3(i, j, k);
+                qout[c] = (float)pre[c];
+                qout[c + ni] = (float)uc[c];
+                qout[c + 2 * ni] = (float)vc[c];
+                qout[c + 3 * ni] = (float)wc[c];
+                qout[c + 4 * ni] = (float)ETA[H(i, j, k)];
+            }
+}
+
+/* One unformatted sequential record: 4-byte length, data, 4-byte length. */
+static int put_record(FILE *f, const void *data, size_t len)
+{
+    if (len > 0x7fffffffu) return -1;
+    const int32_t n = (int32_t)len;
+    if (fwrite(&n, 4, 1, f) != 1) return -1;
+    if (len && fwrite(data, 1, len, f) != len) return -1;
+    if (fwrite(&n, 4, 1, f) != 1) return -1;
+    return 0;
+}
+
+/* filer.f90:173-182: appended records nblocks; nx ny nz; temp x4; Qout */
+int or_write_q(const char *path, int nx, int ny, int nz, const float *qout)
+{
+    FILE *f = fopen(path, "ab");
+    if (!f) return -1;
+    const int32_t nb = 1, dims[3] = {nx, ny, nz};
+    const float temp[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+    int rc = put_record(f, &nb, 4);
+    if (!rc) rc = put_record(f, dims, 12);
+    if (!rc) rc = put_record(f, temp, 16);
+    if (!rc) rc = put_record(f, qout, (size_t)5 * nx * ny * nz * sizeof(float));
+    if (fclose(f)) rc = -1;
+    return rc;
+}
+
+/* gridder.f90:355-378: Xout(i,j,k) = Xs(i) (Y, Z alike) as default reals,
+ * written as nblocks; nx ny nz; one record with X, Y, Z.  Xs(1:nx). */
+int or_write_mesh_x(const char *path, int nx, int ny, int nz, const double *Xs,
+                    const double *Ys, const double *Zs)
+{
+    const size_t ni = (size_t)nx * ny * nz;
+    float *xyz = malloc(3 * ni * sizeof(float));
+    if (!xyz) return -1;
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                const size_t c = I3(i, j, k);
+                xyz[c] = (float)Xs[i - 1];
+                xyz[c + ni] = (float)Ys[j - 1];
+                xyz[c + 2 * ni] = (float)Zs[k - 1];
+            }
+    FILE *f = fopen(path, "wb");
+    int rc = f ? 0 : -1;
+    const int32_t nb = 1, dims[3] = {nx, ny, nz};
+    if (!rc) rc = put_record(f, &nb, 4);
+    if (!rc) rc = put_record(f, dims, 12);
+    if (!rc) rc = put_record(f, xyz, 3 * ni * sizeof(float));
+    if (f && fclose(f)) rc = -1;
+    free(xyz);
+    return rc;
+}
+
+/* check_steady.f90:1-30.  When istep /= isto, *vdiff = max |wc - last|
+ * (a NaN difference never compares greater, so it never wins); otherwise
+ * *vdiff is left as it is.  Then last = wc.  last: interior array. */
+void or_check_steady(const or_grid *g, int istep, int isto, const double *wc,
+                     double *last, double *vdiff)
+{
+    GRID_LOCALS(g);
+    const size_t ni = (size_t)nx * ny * nz;
+    if (istep != isto) {
+        double m = 0.0;
+        for (size_t c = 0; c < ni; c++)
+            if (fabs(wc[c] - last[c]) > m) m = fabs(wc[c] - last[c]);
+        *vdiff = m;
+    }
+    memcpy(last, wc, ni * sizeof(double));
+}
+
+/* reading_data.f90:47-83 reading_variables, from the Qout array of a .q
+ * file.  u, v, w take 0.5*(Q(i)+Q(i+1)) evaluated in single precision over
+ * i, j, k = 1..n-1 for all three components; p (interior) and pre take Q1;
+ * uc, vc, wc = 0.5*(u(i)+u(i-1)) in double precision.  Entries the loops
+ * do not reach keep their values. */
+void or_reading_variables(const or_grid *g, const float *qout, double *u,
+                          double *v, double *w, double *p, double *pre,
+                          double *uc, double *vc, double *wc)
+{
+    GRID_LOCALS(g);
+    const size_t ni = (size_t)nx * ny * nz;
+    const float *q1 = qout, *q2 = qout + ni, *q3 = qout + 2 * ni, *q4 = qout + 3 * ni;
+    for (int k = 1; k <= nz - 1; k++)
+        for (int j = 1; j <= ny - 1; j++)
+            for (int i = 1; i <= nx - 1; i++) {
+                const float a = 0.5f * (q2[I3(i, j, k)] + q2[I3(i + 1, j, k)]);
+                const float b = 0.5f * (q3[I3(i, j, k)] + q3[I3(i, j + 1, k)]);
+                const float c = 0.5f * (q4[I3(i, j, k)] + q4[I3(i, j, k + 1)]);
+                u[H(i, j, k)] = (double)a;
+                v[H(i, j, k)] = (double)b;
+                w[H(i, j, k)] = (double)c;
+            }
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                p[H(i, j, k)] = (double)q1[I3(i, j, k)];
+                pre[I3(i, j, k)] = p[H(i, j, k)];
+            }
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                uc[I3(i, j, k)] = 0.5 * (u[H(i, j, k)] + u[H(i - 1, j, k)]);
+                vc[I3(i, j, k)] = 0.5 * (v[H(i, j, k)] + v[H(i, j - 1, k)]);
+                wc[I3(i, j, k)] = 0.5 * (w[H(i, j, k)] + w[H(i, j, k - 1)]);
+            }
 }

@@ -160,6 +160,29 @@ typedef struct {
 int or_step(const or_grid *g, const or_params *prm, or_fields *f,
             double *cd, double *cl, double *pmax);
 
+/* filer.f90:163-171: Qout(nx,ny,nz,5) = real(pre, uc, vc, wc, ETA) */
+void or_pack_q(const or_grid *g, const double *pre, const double *uc,
+               const double *vc, const double *wc, const double *ETA,
+               float *qout);
+
+/* filer.f90:173-182: appends the records nblocks; nx ny nz; temp x4 (1.0,
+ * variables_module.f90:363); Qout, each as a 4-byte length, the data and
+ * the length again (Fortran unformatted sequential).  0 or -1. */
+int or_write_q(const char *path, int nx, int ny, int nz, const float *qout);
+
+/* gridder.f90:355-378: mesh.x from the cell centres Xs(1:nx), Ys, Zs */
+int or_write_mesh_x(const char *path, int nx, int ny, int nz, const double *Xs,
+                    const double *Ys, const double *Zs);
+
+/* check_steady.f90:1-30 (last: interior array) */
+void or_check_steady(const or_grid *g, int istep, int isto, const double *wc,
+                     double *last, double *vdiff);
+
+/* reading_data.f90:47-83 reading_variables from a Qout array */
+void or_reading_variables(const or_grid *g, const float *qout, double *u,
+                          double *v, double *w, double *p, double *pre,
+                          double *uc, double *vc, double *wc);
+
 #ifdef __cplusplus
 }
 #endif

@@ -142,6 +142,40 @@ class Oracle:
         _lib.or_poisson_coeffs(ct.byref(self.g), _p(coef), jc)
         return coef, list(jc)
 
+    # filer.f90:163-171
+    def pack_q(self, pre, uc, vc, wc, eta):
+        g = self.grid
+        q = np.zeros((g.nx, g.ny, g.nz, 5), np.float32, order="F")
+        _lib.or_pack_q(ct.byref(self.g), _p(pre), _p(uc), _p(vc), _p(wc), _p(eta),
+                       q.ctypes.data_as(ct.POINTER(ct.c_float)))
+        return q
+
+    # filer.f90:173-182 (appends to path)
+    def write_q(self, path, q):
+        assert q.dtype == np.float32 and q.flags.f_contiguous
+        g = self.grid
+        rc = _lib.or_write_q(str(path).encode(), g.nx, g.ny, g.nz,
+                             q.ctypes.data_as(ct.POINTER(ct.c_float)))
+        assert rc == 0, f"or_write_q({path}) failed"
+
+    # gridder.f90:355-378
+    def write_mesh_x(self, path, Xs, Ys, Zs):
+        g = self.grid
+        rc = _lib.or_write_mesh_x(str(path).encode(), g.nx, g.ny, g.nz, _p(Xs), _p(Ys), _p(Zs))
+        assert rc == 0, f"or_write_mesh_x({path}) failed"
+
+    # check_steady.f90:1-30; returns the new VelocityDifference
+    def check_steady(self, istep, isto, wc, last, vdiff):
+        d = ct.c_double(vdiff)
+        _lib.or_check_steady(ct.byref(self.g), istep, isto, _p(wc), _p(last), ct.byref(d))
+        return d.value
+
+    # reading_data.f90:47-83
+    def reading_variables(self, q, u, v, w, p, pre, uc, vc, wc):
+        assert q.dtype == np.float32 and q.flags.f_contiguous
+        _lib.or_reading_variables(ct.byref(self.g), q.ctypes.data_as(ct.POINTER(ct.c_float)),
+                                  _p(u), _p(v), _p(w), _p(p), _p(pre), _p(uc), _p(vc), _p(wc))
+
 
 class OracleRun:
     """Whole-state time stepping with or_step (main.f90:139-259 order)."""

@@ -76,3 +76,58 @@ def test_fortran_driver_matches_oracle(name, mode, tmp_path, hip_lib):
     assert np.array_equal(cd.view(np.uint64), np.array([x[2] for x in rows]).view(np.uint64))
     for n in ("u", "v", "w", "p", "pre", "uc", "vc", "wc"):
         assert np.array_equal(out[n].view(np.uint64), ref.f[n].view(np.uint64)), n
+
+
+@pytest.mark.parametrize("mode", [0, 1], ids=["host-fortran-io", "resident-library"])
+def test_fortran_driver_output_files(mode, tmp_path, hip_lib):
+    """The output of main.f90:113-118,315-328 (a solution file at the start
+    and every isto steps, check_steady after each): mode 0 writes the files
+    by Fortran I/O from the host arrays, mode 1 through ntust_output_q and
+    ntust_check_steady from the resident state.  Both must equal the
+    oracle's files byte for byte and its VelocityDifference sequence."""
+    assert DRIVER.exists(), "Fortran driver not built"
+    c = CASES["les-quick-rb"]
+    n, steps, isto = c["n"], 6, 2
+    g = Grid(n, n, n)
+    ref = OracleRun(g, dt=c["dt"], nu=1.0 / c["Re"], Cs=c["Cs"], omega=1.5, zeta=c["zeta"],
+                    itmax=c["itmax"], LES=c["LES"], scheme=c["scheme"], sor=c["sor"],
+                    bc=bc_codes())
+    u, v, w = noise_ic(g, c["noise"], seed=11)
+    ref.f["u"][...], ref.f["v"][...], ref.f["w"][...] = u, v, w
+    ic = tmp_path / "ic.bin"
+    ic.write_bytes(b"".join(a.tobytes(order="F") for a in (u, v, w)))
+    ref.apply_bc()
+    run = tmp_path / "run"
+    run.mkdir()
+    args = [str(DRIVER), str(tmp_path / "out.bin"), *(str(n),) * 3, repr(c["Re"]), repr(c["dt"]),
+            str(steps), str(c["scheme"]), str(c["LES"]), repr(c["Cs"]), "1.5", repr(c["zeta"]),
+            repr(c["itmax"]), str(c["sor"]), str(mode), str(ic), str(isto)]
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120, cwd=run)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+    from oracle import Oracle
+    o = Oracle(g)
+    f = ref.f
+    want = run.parent / "want"
+    want.mkdir()
+    last = g.interior()
+    vds = []
+
+    def out(num):
+        o.write_q(want / f"{num:04d}.q", o.pack_q(f["pre"], f["uc"], f["vc"], f["wc"], f["ETA"]))
+
+    out(0)
+    for istep in range(1, steps + 1):
+        ref.step()
+        if istep % isto == 0:
+            out(istep // isto)
+            vds.append(o.check_steady(istep, isto, f["wc"], last, 1024.0))
+    names = sorted(p.name for p in want.iterdir())
+    assert names == [f"{k:04d}.q" for k in range(steps // isto + 1)]
+    for name in names:
+        assert (run / name).read_bytes() == (want / name).read_bytes(), name
+    raw = (run / "vdiff.bin").read_bytes()
+    assert int(np.frombuffer(raw[:4], np.int32)[0]) == len(vds)
+    got = np.frombuffer(raw[4:], np.float64)
+    assert np.array_equal(got.view(np.uint64), np.array(vds).view(np.uint64))
+    assert vds[0] == 1024.0 and vds[1] != 1024.0
