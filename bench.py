@@ -16,10 +16,11 @@ FP32 sweeps inside FP64 defect correction; the headline is the latter when its
 velocity is within 1e-6 (relative) of the FP64 solver's.
 
 Further legs in the same JSON line: the same cavity at 256^3 (the size the
->= 70 % HBM and >= 2x FP32-refine targets refer to), the discrete adjoint on
-a 96^3 Re=400 cavity (cost relative to the forward, central-difference
-check), and the CPU restatement timed on a bounded sample.  One JSON line on
-stdout.
+>= 70 % HBM and >= 2x FP32-refine targets refer to), the airfoil's ETA ray
+casting on a 512x384x128 grid (the per-step cost of the pitching mode), the
+discrete adjoint on a 96^3 Re=400 cavity (cost relative to the forward,
+central-difference check), and the CPU restatement timed on a bounded
+sample.  One JSON line on stdout.
 """
 from __future__ import annotations
 
@@ -168,6 +169,54 @@ def adjoint_leg(hip, hiptime, n, steps, fd_dirs):
     }
 
 
+def airfoil_leg(hip, calls, cpu):
+    """RayCasting (airfoil.f90:580-892) of a 5000-vertex NACA 0012 at 8 degrees
+    on the 512x384x128 grid of the large airfoil configuration (y, z
+    clustered at the airfoil with tanh stretching, domain 8 x 8): the cost
+    the pitching mode pays every step (dynamic_AOA + RayCasting,
+    main.f90:144-155).  Resident ETA, host clock around each synchronous
+    call; the CPU restatement of the same cast beside it (1 thread)."""
+    from ntustcfdlab_amd.airfoil import naca4_polygon
+    from ntustcfdlab_amd.grid import Grid, tanh_nodes
+    nx, ny, nz = 512, 384, 128
+    g = Grid(nx, ny, nz, ly=8.0, lz=8.0, Y=tanh_nodes(ny, 8.0, 0.5, 2.5),
+             Z=tanh_nodes(nz, 8.0, 0.5, 2.5))
+    az0, ay0 = naca4_polygon(poly=5000)
+    s = hip.ProjectionSolver(g)
+    try:
+        for _ in range(2):
+            s.ray_casting(az0, ay0, 8.0, 4.0, 4.0)
+        times = []
+        for c in range(calls):
+            t0 = time.perf_counter()
+            s.ray_casting(az0, ay0, 8.0 + 0.01 * c, 4.0, 4.0)
+            times.append(time.perf_counter() - t0)
+        eta = g.halo()
+        s.ray_casting(az0, ay0, 8.0, 4.0, 4.0, ETA=eta)
+    finally:
+        s.close()
+    ms = 1e3 * float(np.median(times))
+    out = {"workload": f"NACA 0012, 5000 vertices, AOA 8, nsub 50, grid {nx}x{ny}x{nz} "
+                       "(tanh-clustered y, z)",
+           "calls": calls, "ms_per_cast": round(ms, 3),
+           "cast_cells_per_s": round(g.cells / (ms * 1e-3) / 1e6, 1),
+           "eta_sum": float(eta[2:-2, 2:-2, 2:-2].sum())}
+    if cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        from oracle import Oracle
+        o = Oracle(g)
+        az, ay = o.airfoil_transform(az0, ay0, 8.0, 4.0, 4.0)
+        ref = g.halo()
+        t0 = time.perf_counter()
+        o.ray_casting(az, ay, g.Y, g.Z, 50, ref)
+        cpu_s = time.perf_counter() - t0
+        out["cpu_restatement_ms"] = round(1e3 * cpu_s, 1)
+        out["cpu_cores"] = 1
+        out["bitwise_equal_to_cpu"] = bool(np.array_equal(ref.view(np.uint64),
+                                                           eta.view(np.uint64)))
+    return out
+
+
 def timed_run(s, hip, hiptime, state, sor, steps, warmup):
     """Upload the state, W warm-up steps, then K steps bracketed by HIP
     events on the library stream.  Returns (ms, per-step sweeps, prof, final)."""
@@ -282,6 +331,10 @@ def main():
     ap.add_argument("--fd-directions", type=int, default=2)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-256", action="store_true", help="skip the 256^3 leg")
+    ap.add_argument("--no-airfoil", action="store_true",
+                    help="skip the airfoil ray-casting leg")
+    ap.add_argument("--only-airfoil", action="store_true",
+                    help="run the airfoil ray-casting leg alone and print its JSON")
     ap.add_argument("--sweep", choices=["two-pass", "graph", "fused"], default="two-pass",
                     help="red-black sweep launches (bit-identical results): two-pass "
                          "kernels from the stream, the same replayed from a HIP graph, "
@@ -296,6 +349,10 @@ def main():
 
     from ntustcfdlab_amd import hip, hiptime
 
+    if a.only_airfoil:
+        print(json.dumps({"airfoil_ray_casting": airfoil_leg(hip, 10,
+                                                             not a.no_cpu_baseline)}))
+        return
     if a.only_adjoint:
         print(json.dumps({"adjoint": adjoint_leg(hip, hiptime, a.adjoint_size,
                                                  a.adjoint_steps, a.fd_directions)}))
@@ -332,6 +389,9 @@ def main():
             "roofline": big["roofline"],
             "mixed_speedup_vs_fp64": big["mixed_speedup_vs_fp64"],
             "mixed_vs_fp64_rel_velocity_err": big["mixed_vs_fp64_rel_velocity_err"]}
+    if not a.no_airfoil and not a.only:
+        print("# airfoil geometry", file=sys.stderr, flush=True)
+        out["airfoil_ray_casting"] = airfoil_leg(hip, 10, not a.no_cpu_baseline)
     if not a.no_adjoint:
         print("# adjoint", file=sys.stderr, flush=True)
         out["adjoint"] = adjoint_leg(hip, hiptime, a.adjoint_size, a.adjoint_steps,

@@ -240,6 +240,36 @@ int ntust_check_steady(int istep, int isto, double *velocity_difference);
 int ntust_restart_write(const char *path, int istep, double time);
 int ntust_restart_read(const char *path, int *istep, double *time);
 
+/* ---- airfoil geometry ---------------------------------------------------- */
+
+/* RayCasting (airfoil.f90:580-892): the airfoil's volume fraction ETA.
+ * az0, ay0: the polygon as the NACA table holds it (chord along z from 0
+ * to 1, poly vertices, the last equal to the first; the reference reads
+ * NACA_filename with list-directed input).  It is shifted to the quarter
+ * chord, rotated by -AOA degrees -- ay is rotated with the already rotated
+ * az, so the shape is sheared by cos(AOA) as in the reference -- and
+ * placed at (zc, yc) (variables_module.f90:144-146).  Yn, Zn: the node
+ * arrays Y(-1:ny+3), Z(-1:nz+3) of the gridder.  Each (z,y) cell centre
+ * gets the even-odd crossing parity, each cell holding a polygon vertex the
+ * inside-fraction of its nsub x nsub sub-cells (nSubGrids = 50, at most
+ * 256), and the plane is extruded along x into the interior of ETA.
+ * Computed on the device into the resident ETA; with ETA non-NULL (the
+ * caller's halo array) the caller's array is uploaded first and copied back
+ * after, so its ghosts keep their values.  A bound the reference's box
+ * search (airfoil.f90:614-654) does not find is 0, the module variable's
+ * first value, so a polygon wholly outside the grid gives ETA = 0 (the
+ * unit cavity of the reference, airfoil at (4, 4)); a polygon straddling
+ * the lower edge of the grid, for which the reference indexes outside its
+ * arrays, is refused with NTUST_E_ARG. */
+int ntust_ray_casting(const double *az0, const double *ay0, int poly, double AOA, double zc,
+                      double yc, const double *Yn, const double *Zn, int nsub, double *ETA);
+
+/* dynamic_AOA (airfoil.f90:311-321): AOA = AOA1 + 10 |sin(2 PI f (time -
+ * dt (StartDynamic - 1)))| with f = 2/PI and PI the single-precision
+ * literal 3.14159265359 widened, as the reference evaluates it.  Host
+ * only; the caller then calls ntust_ray_casting (main.f90:144-155). */
+int ntust_dynamic_aoa(double AOA1, double time, double dt, double StartDynamic, double *AOA);
+
 /* ---- discrete adjoint of the resident step ------------------------------
  *
  * No counterpart in the reference: gradients of an objective of the flow

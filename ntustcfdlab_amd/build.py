@@ -20,7 +20,8 @@ LIB = PKG / "libntust_hip.so"
 ORACLE_DIR = ROOT / "oracle"
 
 HIP_SOURCES = ["k_les.hip", "k_predict.hip", "k_sor.hip", "k_sor_mixed.hip", "k_sor_fused.hip",
-               "k_project.hip", "k_output.hip", "k_bc.hip", "k_adjoint.hip", "ntust_api.hip"]
+               "k_project.hip", "k_output.hip", "k_ray.hip", "k_bc.hip", "k_adjoint.hip",
+               "ntust_api.hip"]
 
 # -ffp-contract=off: no fused multiply-add unless the source writes fma(),
 # which keeps the exact-mode kernels bit-identical to the CPU restatement.
@@ -83,7 +84,8 @@ def build_fortran() -> Path | None:
     out = FSRC / "ntust_driver"
     moddir = PKG / "_build" / "fmod"
     moddir.mkdir(parents=True, exist_ok=True)
-    srcs = [FSRC / f for f in ("ntust_hip.f90", "variables.f90", "hotpath.f90", "driver.f90")]
+    srcs = [FSRC / f for f in ("ntust_hip.f90", "variables.f90", "hotpath.f90",
+                                "airfoil_gpu.f90", "driver.f90")]
     if not all(s.exists() for s in srcs):
         return None
     _run([flang, "-O2", "-module-dir", str(moddir), *map(str, srcs), "-o", str(out),

@@ -1761,4 +1761,89 @@ int ntust_restart_read(const char *path, int *istep, double *time)
     return 0;
 }
 
+// ---- airfoil geometry -------------------------------------------------------
+
+int ntust_ray_casting(const double *az0, const double *ay0, int poly, double AOA, double zc,
+                      double yc, const double *Yn, const double *Zn, int nsub, double *ETA)
+{
+    NEED_INIT();
+    if (!az0 || !ay0 || !Yn || !Zn) return fail(NTUST_E_ARG, "NULL argument");
+    if (poly < 2) return fail(NTUST_E_ARG, "the polygon needs at least 2 vertices (%d)", poly);
+    if (nsub < 1 || nsub > RAY_NSUB_MAX)
+        return fail(NTUST_E_ARG, "nsub %d outside 1..%d", nsub, RAY_NSUB_MAX);
+    // airfoil.f90:597-611 on the host (poly vertices): quarter-chord shift,
+    // rotation by -AOA degrees in which ay uses the rotated az, placement
+    std::vector<double> az((size_t)poly), ay((size_t)poly);
+    const double PI = 4.0 * atan(1.0);
+    for (int i = 0; i < poly; i++) {
+        double z = az0[i] - 0.25, y = ay0[i];
+        z = cos((-AOA) * PI / 180) * z - sin((-AOA) * PI / 180) * y;
+        y = sin((-AOA) * PI / 180) * z + cos((-AOA) * PI / 180) * y;
+        az[i] = (z + zc) + 0.25;
+        ay[i] = y + yc;
+    }
+    // the bounding-box search of airfoil.f90:614-654 over nodes 1..n
+    double ymin = ay[0], ymax = ay[0], zmin = az[0], zmax = az[0];
+    for (int i = 1; i < poly; i++) {
+        ymin = ay[i] < ymin ? ay[i] : ymin;
+        ymax = ay[i] > ymax ? ay[i] : ymax;
+        zmin = az[i] < zmin ? az[i] : zmin;
+        zmax = az[i] > zmax ? az[i] : zmax;
+    }
+    // a bound not found keeps the module variable's first value, 0; a
+    // polygon wholly outside the grid then gives ETA = 0 (the reference's
+    // unit cavity, where the airfoil sits at (4, 4))
+    int A = -1, B = -1, Cb = -1, E = -1;
+    for (int j = 1; j <= C.ny && A < 0; j++) if (Yn[j + 1] > ymin) A = j - 1;
+    for (int j = 1; j <= C.ny && B < 0; j++) if (Yn[j + 1] > ymax) B = j;
+    for (int k = 1; k <= C.nz && Cb < 0; k++) if (Zn[k + 1] > zmin) Cb = k - 1;
+    for (int k = 1; k <= C.nz && E < 0; k++) if (Zn[k + 1] > zmax) E = k;
+    A = A < 0 ? 0 : A;
+    B = B < 0 ? 0 : B;
+    Cb = Cb < 0 ? 0 : Cb;
+    E = E < 0 ? 0 : E;
+    const bool boxed = E - 1 >= Cb && B - 1 >= A;
+    if (boxed && (A < 1 || Cb < 1))
+        return fail(NTUST_E_ARG,
+                    "the airfoil (z %g..%g, y %g..%g) straddles the lower edge of the grid: "
+                    "the reference's box loops would index outside its arrays",
+                    zmin, zmax, ymin, ymax);
+    StepBufs b;
+    TRY(step_buffers(b, false));
+    const size_t n2 = (size_t)C.nz * C.ny;
+    NEED(daz, "ray_az", (size_t)poly, false);
+    NEED(day, "ray_ay", (size_t)poly, false);
+    NEED(dy, "ray_Y", (size_t)C.ny + 5, false);
+    NEED(dz, "ray_Z", (size_t)C.nz + 5, false);
+    NEED(e1, "ray_eta1", n2, false);
+    NEED(pt, "ray_points", n2 / 2 + 1, false);
+    NEED(cn, "ray_counts", n2 / 2 + 1, false);
+    TRY(h2d(daz, az.data(), (size_t)poly));
+    TRY(h2d(day, ay.data(), (size_t)poly));
+    TRY(h2d(dy, Yn, (size_t)C.ny + 5));
+    TRY(h2d(dz, Zn, (size_t)C.nz + 5));
+    HIPCHK(hipMemsetAsync(pt, 0, n2 * sizeof(int), C.st));
+    HIPCHK(hipMemsetAsync(cn, 0, n2 * sizeof(int), C.st));
+    if (ETA) TRY(h2d(b.eta, ETA, C.nh));
+    // an empty box: no vertex cell, no sub-grid (k_ray_subcells launches none)
+    RayGeom r{C.nx, C.ny, C.nz, poly, A, boxed ? B : A, Cb, boxed ? E : Cb, nsub, dy, dz, daz,
+              day};
+    launch_ray_casting(C.g, r, (int *)pt, (int *)cn, e1, b.eta, C.st);
+    HIPCHK(hipGetLastError());
+    if (ETA) TRY(d2h(ETA, b.eta, C.nh));
+    // the host vectors stay alive until the copies have run
+    return sync();
+}
+
+int ntust_dynamic_aoa(double AOA1, double time, double dt, double StartDynamic, double *AOA)
+{
+    if (!AOA) return fail(NTUST_E_ARG, "NULL AOA");
+    // airfoil.f90:311-321: PI is the default-real literal 3.14159265359
+    // widened to real*8; reduce_frequency = 2.0 (variables_module.f90:148)
+    const double PI = (double)3.14159265359f, reduce_frequency = 2.0;
+    const double frequency = reduce_frequency / PI;
+    *AOA = AOA1 + 10 * fabs(sin(2 * PI * frequency * (time - dt * (StartDynamic - 1))));
+    return 0;
+}
+
 }  // extern "C"

@@ -253,6 +253,20 @@ void launch_adj_predict(const KGrid &g, int upwind, const double *u, const doubl
 void launch_adj_smagorinsky(const KGrid &g, const double *u, const double *v, const double *w,
                             const double *filt, const double *lvisc, double *lu, double *lv,
                             double *lw, hipStream_t st);
+// airfoil volume fraction (k_ray.hip, airfoil.f90:580-892).  Yn, Zn: node
+// coordinates with element j at Yn[j+1] (j = -1..n+3); az, ay: the rotated
+// and placed polygon; A..B-1 (y) and C..E-1 (z): the cells of its bounding
+// box, as the reference's bound search finds them.
+constexpr int RAY_NSUB_MAX = 256;
+struct RayGeom {
+    int nx, ny, nz, poly;
+    int A, B, C, E;
+    int nsub;
+    const double *Yn, *Zn, *az, *ay;
+};
+void launch_ray_casting(const KGrid &g, const RayGeom &r, int *points, int *counts,
+                        double *eta1, double *eta, hipStream_t st);
+
 // output and restart (k_output.hip)
 void launch_pack_q(const KGrid &g, const double *pre, const double *uc, const double *vc,
                    const double *wc, const double *eta, float *q, hipStream_t st);

@@ -9,7 +9,7 @@
 ! on the device with the state resident in HBM (mode 1).
 !
 ! usage: ntust_driver OUT NX NY NZ RE DT NSTEP SCHEME LES CS OMEGA ZETA ITMAX SOR MODE
-!                     [IC [ISTO]]
+!                     [IC [ISTO [NACA AOA1 START L POLY]]]
 !   SCHEME 0 QUICK_centre, 1 upwind; SOR 0 lexicographic, 1 red-black;
 !   IC: optional raw file with u, v, w (halo arrays, real*8) as initial state
 !   ('-' for none).
@@ -19,6 +19,12 @@
 !   Fortran I/O from the host arrays as the solver's own routines do, in
 !   mode 1 by the library from the resident state; the VelocityDifference
 !   values go to vdiff.bin (stream: count int32, values real*8).
+!   NACA: the airfoil table ('-' for none): RayCasting at start-up with
+!   AOA = AOA1 (main.f90:109-110), and from step START on dynamic_AOA and
+!   RayCasting before each step (main.f90:144-155), through the shims of
+!   airfoil_gpu.f90 in mode 0 and on the resident ETA in mode 1; L: the
+!   y and z extent of the grid (the airfoil sits at y = z = 4); POLY: the
+!   table's vertex count.
 ! OUT receives (stream, native endianness): nstep, ik(nstep) int32,
 !   cDrag(nstep), cLift(nstep), u, v, w, p (halo arrays), pre, uc, vc, wc.
 program ntust_driver
@@ -26,7 +32,8 @@ program ntust_driver
    use ntust_hip
    implicit none
    character(len=512) :: out, icfile, arg
-   integer :: scheme, sor, mode, s, n, lun, nout
+   integer :: scheme, sor, mode, s, n, lun, nout, q
+   logical :: airfoil
    integer(c_int), allocatable :: iks(:)
    real*8, allocatable :: cds(:), cls(:), pcs(:), vds(:)
    real*8 :: t0, t1
@@ -41,6 +48,13 @@ program ntust_driver
    if (icfile == '-') icfile = ''
    isto = 0
    if (command_argument_count() >= 17) isto = iarg(17)
+   NACA_filename = '-'
+   if (command_argument_count() >= 22) then
+      call get_command_argument(18, NACA_filename)
+      AOA1 = rarg(19); StartDynamic = rarg(20)
+      ly = rarg(21); lz = ly; poly = iarg(22)
+   end if
+   airfoil = NACA_filename /= '-'
 
    nu = 1./Re                      ! reading_data.f90:38
    call allocate_fields()
@@ -54,6 +68,11 @@ program ntust_driver
       close (lun)
    end if
    call boundary_conditions()      ! main.f90:100
+   if (airfoil) then               ! main.f90:109-110
+      allocate (az(poly), ay(poly))
+      AOA = AOA1
+      call RayCasting()
+   end if
 
    allocate (iks(nstep), cds(nstep), cls(nstep), pcs(nstep), vds(nstep + 1))
    nout = 0
@@ -64,6 +83,10 @@ program ntust_driver
    if (mode == 0) then
       if (isto > 0) call output_host()   ! main.f90:117
       do istep = 1, nstep
+         if (airfoil .and. istep >= StartDynamic) then   ! main.f90:144-155
+            call dynamic_AOA()
+            call RayCasting()
+         end if
          call CalculateSmagorinskyViscosity()
          if (scheme == 0) then
             call discretisation_QUICK_centre()
@@ -88,13 +111,30 @@ program ntust_driver
       end do
    else
       call ntust_check(ntust_state_upload(u, v, w, p, ETA), 'ntust_state_upload')
-      if (isto > 0) then
-         call output_resident()
-         s = 1
-         do while (s <= nstep)
-            n = min(isto, nstep - s + 1)
-            call ntust_check(ntust_step(n, iks(s:), pcs(s:), cds(s:), cls(s:)), 'ntust_step')
-            istep = s + n - 1
+      if (isto > 0) call output_resident()
+      ! runs of steps end at an output step, and from StartDynamic on every
+      ! step re-casts the airfoil at the pitching angle
+      s = 1
+      do while (s <= nstep)
+         n = nstep - s + 1
+         if (isto > 0) n = min(n, isto - mod(s - 1, isto))
+         if (airfoil) then
+            if (s >= StartDynamic) then
+               n = 1
+               call ntust_check(ntust_dynamic_aoa(AOA1, time, dt, StartDynamic, AOA), &
+                                'ntust_dynamic_aoa')
+               call ntust_check(ntust_ray_casting(az, ay, poly, AOA, zc, yc, Y, Z, nSubGrids, &
+                                                  c_null_ptr), 'ntust_ray_casting')
+            else if (StartDynamic <= nstep) then
+               n = min(n, int(StartDynamic) - s)
+            end if
+         end if
+         call ntust_check(ntust_step(n, iks(s:), pcs(s:), cds(s:), cls(s:)), 'ntust_step')
+         do q = 1, n
+            time = time + dt
+         end do
+         istep = s + n - 1
+         if (isto > 0) then
             if (mod(istep, isto) == 0) then
                VelocityDifference = 1024
                call output_resident()
@@ -102,11 +142,9 @@ program ntust_driver
                                 'ntust_check_steady')
                nout = nout + 1; vds(nout) = VelocityDifference
             end if
-            s = s + n
-         end do
-      else
-         call ntust_check(ntust_step(nstep, iks, pcs, cds, cls), 'ntust_step')
-      end if
+         end if
+         s = s + n
+      end do
       call ntust_check(ntust_state_download(u, v, w, p, pre, uc, vc, wc), 'ntust_state_download')
    end if
    call cpu_time(t1)

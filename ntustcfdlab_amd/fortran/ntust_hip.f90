@@ -156,6 +156,24 @@ module ntust_hip
          integer(c_int), value :: variant
       end function
 
+      ! airfoil geometry: ETA is type(c_ptr) (c_loc(ETA), or c_null_ptr for
+      ! the resident ETA only)
+      integer(c_int) function ntust_ray_casting(az0, ay0, poly, AOA, zc, yc, Yn, Zn, nsub, &
+                                                ETA) bind(C, name="ntust_ray_casting")
+         import :: c_int, c_double, c_ptr
+         real(c_double), intent(in) :: az0(*), ay0(*), Yn(*), Zn(*)
+         integer(c_int), value :: poly, nsub
+         real(c_double), value :: AOA, zc, yc
+         type(c_ptr), value :: ETA
+      end function
+
+      integer(c_int) function ntust_dynamic_aoa(AOA1, time, dt, StartDynamic, AOA) &
+            bind(C, name="ntust_dynamic_aoa")
+         import :: c_int, c_double
+         real(c_double), value :: AOA1, time, dt, StartDynamic
+         real(c_double), intent(out) :: AOA
+      end function
+
       ! output, steady check and restart on the resident state; paths are
       ! C strings (trim(name)//c_null_char)
       integer(c_int) function ntust_output_q(path, qout) bind(C, name="ntust_output_q")

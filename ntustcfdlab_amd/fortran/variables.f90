@@ -37,6 +37,14 @@ module variables
    ! grid: nodes (-1:n+3) and metrics (-1:n+2)
    real*8, allocatable, dimension(:) :: X, Y, Z, iDx, Dxs, iDy, Dys, iDz, Dzs
 
+   ! airfoil (variables_module.f90:142-179): placement, pitching, polygon
+   real*8, parameter :: xc = 1.0, yc = 4.0, zc = 4.0, reduce_frequency = 2.0
+   real*8 :: AOA = 0, AOA1 = 5.D0, frequency, StartDynamic = 1000000
+   integer :: poly = 5000
+   integer, parameter :: nSubGrids = 50
+   character(len=512) :: NACA_filename = 'NACA0012.DAT'
+   real*8, allocatable, dimension(:) :: az, ay
+
    ! output (variables_module.f90:201-205,223,355-365): solution array,
    ! file counter, steady check
    integer, parameter :: nblocks = 1

@@ -94,6 +94,8 @@ def load_library() -> ct.CDLL:
         "ntust_check_steady": [i, i, _D],
         "ntust_restart_write": [ct.c_char_p, i, d],
         "ntust_restart_read": [ct.c_char_p, _I, _D],
+        "ntust_ray_casting": [_D, _D, i, d, d, d, _D, _D, i, _D],
+        "ntust_dynamic_aoa": [d, d, d, d, _D],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -105,6 +107,15 @@ def load_library() -> ct.CDLL:
     lib.ntust_stream.argtypes = []
     _lib = lib
     return lib
+
+
+def dynamic_aoa(AOA1: float, time: float, dt: float, StartDynamic: float) -> float:
+    """dynamic_AOA (airfoil.f90:311-321), evaluated by the library."""
+    lib = load_library()
+    out = ct.c_double()
+    _check(lib.ntust_dynamic_aoa(float(AOA1), float(time), float(dt), float(StartDynamic),
+                                 ct.byref(out)))
+    return out.value
 
 
 def _p(a):
@@ -330,6 +341,22 @@ class ProjectionSolver:
         st, t = ct.c_int(), ct.c_double()
         _check(self.lib.ntust_restart_read(str(path).encode(), ct.byref(st), ct.byref(t)))
         return st.value, t.value
+
+    # ---- airfoil geometry ----
+
+    def ray_casting(self, az0, ay0, AOA: float, zc: float, yc: float, nsub: int = 50,
+                    ETA: np.ndarray | None = None):
+        """RayCasting (airfoil.f90:580-892) on the device: the resident ETA,
+        and ETA (a halo array) in place when given.  az0, ay0: the polygon
+        table as read (see airfoil.py)."""
+        az0 = np.ascontiguousarray(az0, np.float64)
+        ay0 = np.ascontiguousarray(ay0, np.float64)
+        if az0.shape != ay0.shape:
+            raise ValueError("az0 and ay0 differ in length")
+        g = self.grid
+        _check(self.lib.ntust_ray_casting(_p(az0), _p(ay0), len(az0), float(AOA), float(zc),
+                                          float(yc), _p(np.ascontiguousarray(g.Y)),
+                                          _p(np.ascontiguousarray(g.Z)), int(nsub), _p(ETA)))
 
     def checkpoint_reserve(self, n: int):
         _check(self.lib.ntust_checkpoint_reserve(int(n)))

@@ -1101,3 +1101,155 @@ void or_reading_variables(const or_grid *g, const float *qout, double *u,
                 wc[I3(i, j, k)] = 0.5 * (w[H(i, j, k)] + w[H(i, j, k - 1)]);
             }
 }
+
+/* ------------------------------------------------------------------ */
+/* airfoil.f90: RayCasting                                             */
+/* ------------------------------------------------------------------ */
+
+/* airfoil.f90:597-611.  The rotation is not rigid: ay is rotated with the
+ * already rotated az (:604-605). */
+void or_airfoil_transform(int poly, const double *az0, const double *ay0,
+                          double AOA, double zc, double yc, double *az,
+                          double *ay)
+{
+    const double PI = 4.0 * atan(1.0);   /* :594 */
+    for (int i = 0; i < poly; i++) {
+        double z = az0[i], y = ay0[i];
+        z = z - 0.25;
+        z = cos((-AOA) * PI / 180) * z - sin((-AOA) * PI / 180) * y;
+        y = sin((-AOA) * PI / 180) * z + cos((-AOA) * PI / 180) * y;
+        z = z + zc;
+        y = y + yc;
+        z = z + 0.25;
+        az[i] = z;
+        ay[i] = y;
+    }
+}
+
+/* One edge of the crossing-number test (:695-760 for cells, :788-860 for
+ * sub-cells): the edge from vertex m to m+1 (0-based here) against the
+ * row through yrow and the point at zcen, with (Yn, Zn) the node the
+ * slope test uses (the cell's or sub-cell's lower corner, not its
+ * centre).  strict: the sub-cell form (zcen > max) instead of >=.  The
+ * "nudge" of :701-702 adds 0.d01 = 0 and is left out. */
+static int ray_edge(const double *az, const double *ay, int m, double yrow,
+                    double zcen, double Yn, double Zn, int strict)
+{
+    int c = 0;
+    for (int blk = 0; blk < 2; blk++) {
+        int P, Q;
+        if (blk == 0) {
+            if (!(ay[m] >= yrow && ay[m + 1] <= yrow)) continue;
+            P = m + 1;
+            Q = m;
+        } else {
+            if (!(ay[m] <= yrow && ay[m + 1] >= yrow)) continue;
+            P = m;
+            Q = m + 1;
+        }
+        const double zmax = az[m] > az[m + 1] ? az[m] : az[m + 1];
+        const double zmin = az[m] < az[m + 1] ? az[m] : az[m + 1];
+        if (strict ? zcen > zmax : zcen >= zmax) {
+            /* + 0 */
+        } else if (zcen < zmin) {
+            c += 1;
+        } else {
+            const double m_pa = (Yn - ay[P]) / (Zn - az[P]);
+            const double m_ab = (ay[Q] - ay[P]) / (az[Q] - az[P]);
+            if (m_pa > m_ab) c += 1;
+        }
+    }
+    return c;
+}
+
+int or_ray_casting(const or_grid *g, int poly, const double *az,
+                   const double *ay, const double *Yn, const double *Zn,
+                   int nsub, double *ETA, double *eta1_out, int *points_out)
+{
+    GRID_LOCALS(g);
+#define YN(j) Yn[(j) + 1]
+#define ZN(k) Zn[(k) + 1]
+    double ymin = ay[0], ymax = ay[0], zmin = az[0], zmax = az[0];
+    for (int i = 1; i < poly; i++) {           /* minval / maxval */
+        if (ay[i] < ymin) ymin = ay[i];
+        if (ay[i] > ymax) ymax = ay[i];
+        if (az[i] < zmin) zmin = az[i];
+        if (az[i] > zmax) zmax = az[i];
+    }
+    /* :614-654; a bound the search does not find keeps the module
+     * variable's value, 0 on the first call */
+    int A = 0, B = 0, C = 0, E = 0;
+    for (int j = 1; j <= ny; j++) if (YN(j) > ymin) { A = j - 1; break; }
+    for (int j = 1; j <= ny; j++) if (YN(j) > ymax) { B = j; break; }
+    for (int k = 1; k <= nz; k++) if (ZN(k) > zmin) { C = k - 1; break; }
+    for (int k = 1; k <= nz; k++) if (ZN(k) > zmax) { E = k; break; }
+    /* with both ranges non-empty the loops below touch points(C..E-1,
+     * A..B-1), outside the array when A or C is 0 */
+    const int boxed = E - 1 >= C && B - 1 >= A;
+    if (boxed && (A < 1 || C < 1)) return -1;
+
+    const size_t n2 = (size_t)nz * ny;
+    int *points = calloc(n2, sizeof(int)), *inter = calloc(n2, sizeof(int));
+    double *eta1 = malloc(n2 * sizeof(double));
+#define P2(k, j) ((size_t)((k) - 1) + (size_t)nz * (size_t)((j) - 1))
+    for (int m = 0; m < poly && boxed; m++)    /* :659-673 */
+        for (int k = C; k <= E - 1; k++)
+            if (az[m] >= ZN(k) && az[m] < ZN(k + 1))
+                for (int j = A; j <= B - 1; j++)
+                    if (ay[m] >= YN(j) && ay[m] < YN(j + 1)) points[P2(k, j)] = 1;
+
+    for (int m = 0; m < poly - 1; m++)         /* :684-766 */
+        for (int j = 1; j <= ny; j++) {
+            const double yrow = (YN(j) + YN(j + 1)) / 2.0;
+            if (!((ay[m] >= yrow && ay[m + 1] <= yrow) || (ay[m] <= yrow && ay[m + 1] >= yrow)))
+                continue;
+            for (int k = 1; k <= nz; k++)
+                inter[P2(k, j)] += ray_edge(az, ay, m, yrow, (ZN(k) + ZN(k + 1)) / 2.0,
+                                            YN(j), ZN(k), 0);
+        }
+    for (size_t c = 0; c < n2; c++) eta1[c] = (inter[c] % 2 == 0) ? 0.0 : 1.0;
+
+    double *ZZ = malloc((nsub + 1) * sizeof(double)), *YY = malloc((nsub + 1) * sizeof(double));
+    for (int k = C; k <= E - 1 && boxed; k++)  /* :776-875 */
+        for (int j = A; j <= B - 1; j++) {
+            if (points[P2(k, j)] <= 0) continue;
+            for (int l = 1; l <= nsub + 1; l++) {
+                ZZ[l - 1] = ZN(k) + (double)(l - 1) * (ZN(k + 1) - ZN(k)) / (double)nsub;
+                YY[l - 1] = YN(j) + (double)(l - 1) * (YN(j + 1) - YN(j)) / (double)nsub;
+            }
+            long total = 0;
+            for (int l = 0; l < nsub; l++) {
+                const double yrow = (YY[l] + YY[l + 1]) / 2.0;
+                for (int n = 0; n < nsub; n++) {
+                    const double zcen = (ZZ[n] + ZZ[n + 1]) / 2.0;
+                    int cnt = 0;
+                    for (int m = 0; m < poly - 1; m++)
+                        cnt += ray_edge(az, ay, m, yrow, zcen, YY[l], ZZ[n], 1);
+                    total += cnt % 2;
+                }
+            }
+            eta1[P2(k, j)] = (double)total / (double)nsub / (double)nsub;
+        }
+    for (int k = 1; k <= nz; k++)              /* :879-883 */
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) ETA[H(i, j, k)] = eta1[P2(k, j)];
+    if (eta1_out) memcpy(eta1_out, eta1, n2 * sizeof(double));
+    if (points_out) memcpy(points_out, points, n2 * sizeof(int));
+    free(ZZ);
+    free(YY);
+    free(points);
+    free(inter);
+    free(eta1);
+    return 0;
+#undef P2
+#undef YN
+#undef ZN
+}
+
+/* airfoil.f90:311-321 dynamic_AOA; PI is the default-real literal widened */
+double or_dynamic_aoa(double AOA1, double time, double dt, double StartDynamic)
+{
+    const double PI = (double)3.14159265359f, reduce_frequency = 2.0;
+    const double frequency = reduce_frequency / PI;
+    return AOA1 + 10 * fabs(sin(2 * PI * frequency * (time - dt * (StartDynamic - 1))));
+}

@@ -183,6 +183,30 @@ void or_reading_variables(const or_grid *g, const float *qout, double *u,
                           double *v, double *w, double *p, double *pre,
                           double *uc, double *vc, double *wc);
 
+/* airfoil.f90:597-611: polygon shifted to the quarter chord, rotated by
+ * -AOA degrees with ay using the rotated az (not rigid), placed at (zc, yc) */
+void or_airfoil_transform(int poly, const double *az0, const double *ay0,
+                          double AOA, double zc, double yc, double *az,
+                          double *ay);
+
+/* airfoil.f90:613-883 RayCasting on the transformed polygon (az, ay; poly
+ * vertices, the last equal to the first): crossing-number parity of each
+ * (z,y) cell centre, nsub x nsub sub-cell fraction in the cells holding a
+ * vertex, extruded along x into ETA (interior of a halo array).  Yn, Zn:
+ * node coordinates, element j at j+1 (j = -1..n+3).  eta1_out (nz,ny)
+ * and points_out (nz,ny) may be NULL.  A bound of the box search
+ * (:614-654) that is not found is 0, the module variable's first value: a
+ * polygon outside the grid then gives ETA = 0 as in the reference.
+ * Returns -1 (ETA untouched) when the box loops would index row or column
+ * 0 of points (a polygon straddling the lower edge of the grid), which
+ * the reference does out of bounds. */
+int or_ray_casting(const or_grid *g, int poly, const double *az,
+                   const double *ay, const double *Yn, const double *Zn,
+                   int nsub, double *ETA, double *eta1_out, int *points_out);
+
+/* airfoil.f90:311-321 dynamic_AOA (reduce_frequency = 2.0) */
+double or_dynamic_aoa(double AOA1, double time, double dt, double StartDynamic);
+
 #ifdef __cplusplus
 }
 #endif

@@ -46,6 +46,13 @@ def _load():
 
 
 _lib = _load()
+_lib.or_dynamic_aoa.restype = ct.c_double
+
+
+def dynamic_aoa(AOA1, time, dt, StartDynamic):
+    """airfoil.f90:311-321"""
+    return _lib.or_dynamic_aoa(ct.c_double(AOA1), ct.c_double(time), ct.c_double(dt),
+                               ct.c_double(StartDynamic))
 
 
 def _p(a):
@@ -169,6 +176,26 @@ class Oracle:
         d = ct.c_double(vdiff)
         _lib.or_check_steady(ct.byref(self.g), istep, isto, _p(wc), _p(last), ct.byref(d))
         return d.value
+
+    # airfoil.f90:597-611
+    def airfoil_transform(self, az0, ay0, AOA, zc, yc):
+        az0, ay0 = np.ascontiguousarray(az0, np.float64), np.ascontiguousarray(ay0, np.float64)
+        az, ay = np.zeros_like(az0), np.zeros_like(ay0)
+        _lib.or_airfoil_transform(len(az0), _p(az0), _p(ay0), ct.c_double(AOA), ct.c_double(zc),
+                                  ct.c_double(yc), _p(az), _p(ay))
+        return az, ay
+
+    # airfoil.f90:613-883 on a transformed polygon; returns (ETA_1, points)
+    # (nz, ny) or None when the polygon is not inside the grid
+    def ray_casting(self, az, ay, Yn, Zn, nsub, ETA):
+        g = self.grid
+        az, ay = np.ascontiguousarray(az, np.float64), np.ascontiguousarray(ay, np.float64)
+        eta1 = np.zeros((g.nz, g.ny), order="F")
+        pts = np.zeros((g.nz, g.ny), np.int32, order="F")
+        rc = _lib.or_ray_casting(ct.byref(self.g), len(az), _p(az), _p(ay),
+                                 _p(np.ascontiguousarray(Yn)), _p(np.ascontiguousarray(Zn)),
+                                 int(nsub), _p(ETA), _p(eta1), pts.ctypes.data_as(_I))
+        return None if rc != 0 else (eta1, pts)
 
     # reading_data.f90:47-83
     def reading_variables(self, q, u, v, w, p, pre, uc, vc, wc):

@@ -131,3 +131,56 @@ def test_fortran_driver_output_files(mode, tmp_path, hip_lib):
     got = np.frombuffer(raw[4:], np.float64)
     assert np.array_equal(got.view(np.uint64), np.array(vds).view(np.uint64))
     assert vds[0] == 1024.0 and vds[1] != 1024.0
+
+
+@pytest.mark.parametrize("mode", [0, 1], ids=["per-call-shims", "resident"])
+def test_fortran_driver_airfoil_pitching(mode, tmp_path, hip_lib):
+    """RayCasting at start-up and, from StartDynamic on, dynamic_AOA +
+    RayCasting before every step (main.f90:109-110, 144-155): in mode 0 the
+    airfoil_gpu.f90 shims with the reference's names, in mode 1 the
+    resident ETA.  The state, sweep counts and force coefficients must
+    equal the oracle's, whose ETA comes from its own RayCasting."""
+    from ntustcfdlab_amd.airfoil import naca4_polygon
+    from oracle import Oracle, dynamic_aoa
+    assert DRIVER.exists(), "Fortran driver not built"
+    nx, n, L, steps, start, isto, poly = 6, 24, 8.0, 6, 4.0, 3, 1000
+    az0, ay0 = naca4_polygon(poly=poly)
+    table = tmp_path / "naca.dat"
+    table.write_text("".join(f"{float(z)!r} {float(y)!r}\n" for z, y in zip(az0, ay0)))
+    dt, Re = 5e-4, 1000.0
+    g = Grid(nx, n, n, ly=L, lz=L)
+    ref = OracleRun(g, dt=dt, nu=1.0 / Re, Cs=0.1, omega=1.5, zeta=1e-9, itmax=300.0, LES=1,
+                    scheme=0, sor=1, bc=bc_codes())
+    u, v, w = noise_ic(g, 1e-2, seed=17)
+    ref.f["u"][...], ref.f["v"][...], ref.f["w"][...] = u, v, w
+    ref.apply_bc()
+    o = Oracle(g)
+
+    def cast(aoa):
+        az, ay = o.airfoil_transform(az0, ay0, aoa, 4.0, 4.0)
+        assert o.ray_casting(az, ay, g.Y, g.Z, 50, ref.f["ETA"]) is not None
+
+    cast(5.0)
+    assert ref.f["ETA"].any()
+    rows, time = [], 0.0
+    for istep in range(1, steps + 1):
+        if istep >= start:
+            cast(dynamic_aoa(5.0, time, dt, start))
+        rows.append(ref.step())
+        time = time + dt
+    ic = tmp_path / "ic.bin"
+    ic.write_bytes(b"".join(a.tobytes(order="F") for a in (u, v, w)))
+    run = tmp_path / "run"
+    run.mkdir()
+    args = [str(DRIVER), str(tmp_path / "out.bin"), str(nx), str(n), str(n), repr(Re), repr(dt),
+            str(steps), "0", "1", "0.1", "1.5", "1e-9", "300.0", "1", str(mode), str(ic),
+            str(isto), str(table), "5.0", repr(start), repr(L), str(poly)]
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120, cwd=run)
+    assert r.returncode == 0, r.stdout + r.stderr
+    ik, cd, cl, out = read_out(tmp_path / "out.bin", g, steps)
+    assert list(ik) == [x[0] for x in rows]
+    assert np.array_equal(cd.view(np.uint64), np.array([x[2] for x in rows]).view(np.uint64))
+    assert np.array_equal(cl.view(np.uint64), np.array([x[3] for x in rows]).view(np.uint64))
+    assert np.all(cd != 0)
+    for name in ("u", "v", "w", "p", "pre", "uc", "vc", "wc"):
+        assert np.array_equal(out[name].view(np.uint64), ref.f[name].view(np.uint64)), name
