@@ -217,6 +217,36 @@ def airfoil_leg(hip, calls, cpu):
     return out
 
 
+def krylov_leg(hip, n):
+    """solver_BICG3 (solver.f90:1-144) on the solver_1st_member matrix of a
+    uniform n^3 grid: unpreconditioned BiCGSTAB to ||r||/||b|| <= 1e-8 from
+    x = 0 for a seeded right-hand side; the per-iteration time and the
+    algorithmic bandwidth (256 B per unknown and iteration: 7 coefficient
+    columns read twice, and p, r, r2, v, s, t, x read or written)."""
+    from ntustcfdlab_amd.grid import Grid
+    g = Grid(n, n, n)
+    s = hip.ProjectionSolver(g)
+    try:
+        coef, jcoef = s.solver_1st_member()
+        b = np.random.default_rng(7).standard_normal(g.cells)
+        x = np.zeros(g.cells)
+        s.solver_bicg3(coef, jcoef, b, x, 1e-8, 3)       # warm-up, allocations
+        x[:] = 0.0
+        t0 = time.perf_counter()
+        it, nrm = s.solver_bicg3(coef, jcoef, b, x, 1e-8, 20000)
+        wall = time.perf_counter() - t0
+    finally:
+        s.close()
+    # the wall includes the H2D of coef, b, x and the D2H of x (72 B per
+    # unknown over PCIe); reported as is
+    ms_it = 1e3 * wall / max(it, 1)
+    return {"workload": f"BiCGSTAB on the {n}^3 Poisson matrix (solver_1st_member), "
+                        "eps 1e-8, x0 = 0",
+            "iterations": it, "norm": nrm, "wall_ms": round(1e3 * wall, 2),
+            "ms_per_iteration_incl_transfers": round(ms_it, 4),
+            "algorithmic_GBs_incl_transfers": round(256.0 * g.cells / (ms_it * 1e-3) / 1e9, 1)}
+
+
 def timed_run(s, hip, hiptime, state, sor, steps, warmup):
     """Upload the state, W warm-up steps, then K steps bracketed by HIP
     events on the library stream.  Returns (ms, per-step sweeps, prof, final)."""
@@ -335,6 +365,8 @@ def main():
                     help="skip the airfoil ray-casting leg")
     ap.add_argument("--only-airfoil", action="store_true",
                     help="run the airfoil ray-casting leg alone and print its JSON")
+    ap.add_argument("--only-krylov", action="store_true",
+                    help="run the BiCGSTAB leg alone and print its JSON")
     ap.add_argument("--sweep", choices=["two-pass", "graph", "fused"], default="two-pass",
                     help="red-black sweep launches (bit-identical results): two-pass "
                          "kernels from the stream, the same replayed from a HIP graph, "
@@ -349,6 +381,9 @@ def main():
 
     from ntustcfdlab_amd import hip, hiptime
 
+    if a.only_krylov:
+        print(json.dumps({"krylov_bicgstab": krylov_leg(hip, 128)}))
+        return
     if a.only_airfoil:
         print(json.dumps({"airfoil_ray_casting": airfoil_leg(hip, 10,
                                                              not a.no_cpu_baseline)}))

@@ -170,6 +170,25 @@ int ntust_poisson_rhs(const double *u_star, const double *v_star, const double *
  * the order of the unknowns, one rank). */
 int ntust_pressure_field(const double *x, double *p, double *pre);
 
+/* matmul_ell (solver.f90:146-190): x = A y for the DIA matrix of
+ * solver_1st_member, coef(n,7) (column-major) and jcoef(m), m <= 4
+ * (jcoef(2..4) the offsets 1, nx, nx*ny).  Each row's terms are added in the
+ * reference's order (diagonal, +1, -1, +nx, -nx, +nxny, -nxny): bit-identical
+ * to a serial evaluation. */
+int ntust_matmul_ell(double *x, const double *coef, const int *jcoef, const double *y, int n,
+                     int m);
+
+/* solver_BICG3 (solver.f90:1-144): unpreconditioned BiCGSTAB for A x = b
+ * with the same matrix arguments, from the initial x, while
+ * ||r|| / ||b|| > eps and fewer than itmax iterations.  x is overwritten by
+ * the solution; *iterations and *norm get the final j and ||r|| / ||b||
+ * (what the reference prints).  The work vectors p, r, r2, v, ss, t and
+ * the MPI arguments of the reference are internal here.  Inner products
+ * are deterministic tree sums, so iterates are reproducible run to run
+ * but not bit-identical to the serial loop. */
+int ntust_solver_bicg3(const double *coef, const int *jcoef, const double *b, double *x, int n,
+                       int m, double eps, int itmax, int *iterations, double *norm);
+
 /* ---- device-resident time stepping ------------------------------------ */
 
 /* Step parameters (variables_module.f90:9-23,277; main.f90:44-60):

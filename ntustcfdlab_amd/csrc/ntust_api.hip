@@ -1761,6 +1761,88 @@ int ntust_restart_read(const char *path, int *istep, double *time)
     return 0;
 }
 
+// ---- Krylov path ----------------------------------------------------------
+
+// jcoef(2..m) as offsets, checked against n
+static int dia_offsets(const int *jcoef, int n, int m, long col[3])
+{
+    if (!jcoef) return fail(NTUST_E_ARG, "NULL jcoef");
+    if (m < 1 || m > 4) return fail(NTUST_E_ARG, "m = %d: the DIA layout has 1..4 offsets", m);
+    for (int d = 0; d < 3; d++) {
+        col[d] = d + 2 <= m ? jcoef[d + 1] : 0;
+        if (d + 2 <= m && (col[d] < 1 || col[d] >= n))
+            return fail(NTUST_E_ARG, "jcoef(%d) = %ld outside 1..n-1", d + 2, col[d]);
+    }
+    return 0;
+}
+
+int ntust_matmul_ell(double *x, const double *coef, const int *jcoef, const double *y, int n,
+                     int m)
+{
+    NEED_INIT();
+    if (!x || !coef || !y || n < 1) return fail(NTUST_E_ARG, "NULL array or n < 1");
+    long col[3];
+    TRY(dia_offsets(jcoef, n, m, col));
+    NEED(dc, "kry_coef", 7 * (size_t)n, false);
+    NEED(dy, "kry_y", (size_t)n, false);
+    NEED(dx, "kry_x", (size_t)n, false);
+    TRY(h2d(dc, coef, 7 * (size_t)n));
+    TRY(h2d(dy, y, (size_t)n));
+    launch_dia_matvec(dc, col, m, n, dy, dx, C.st);
+    HIPCHK(hipGetLastError());
+    TRY(d2h(x, dx, (size_t)n));
+    return sync();
+}
+
+int ntust_solver_bicg3(const double *coef, const int *jcoef, const double *b, double *x, int n,
+                       int m, double eps, int itmax, int *iterations, double *norm)
+{
+    NEED_INIT();
+    if (!coef || !b || !x || n < 1) return fail(NTUST_E_ARG, "NULL array or n < 1");
+    long col[3];
+    TRY(dia_offsets(jcoef, n, m, col));
+    NEED(dc, "kry_coef", 7 * (size_t)n, false);
+    NEED(db, "kry_b", (size_t)n, false);
+    NEED(dx, "kry_x", (size_t)n, false);
+    BicgVectors w;
+    double **vec[6] = {&w.p, &w.r, &w.r2, &w.v, &w.ss, &w.t};
+    const char *nm[6] = {"kry_p", "kry_r", "kry_r2", "kry_v", "kry_s", "kry_t"};
+    for (int q = 0; q < 6; q++) {
+        *vec[q] = buf(nm[q], (size_t)n, false);
+        if (!*vec[q]) return fail(NTUST_E_ALLOC, "device allocation of %s failed", nm[q]);
+    }
+    NEED(part, "kry_part", 2 * (size_t)KRY_MAX_BLOCKS, false);
+    NEED(sd, "kry_state", (sizeof(BicgState) + 7) / 8, false);
+    w.part = part;
+    BicgState *st = (BicgState *)sd;
+    BicgState h{};
+    h.eps = eps;
+    h.itmax = itmax;
+    TRY(h2d(dc, coef, 7 * (size_t)n));
+    TRY(h2d(db, b, (size_t)n));
+    TRY(h2d(dx, x, (size_t)n));
+    HIPCHK(hipMemcpyAsync(st, &h, sizeof h, hipMemcpyHostToDevice, C.st));
+    launch_bicg_setup(dc, col, m, n, db, dx, w, st, C.st);
+    HIPCHK(hipGetLastError());
+    // iterations in batches; the kernels of iterations past the stopping
+    // rule return at once
+    HIPCHK(hipMemcpyAsync(&h, st, sizeof h, hipMemcpyDeviceToHost, C.st));
+    TRY(sync());
+    int batch = 8;
+    while (!h.stop) {
+        for (int q = 0; q < batch; q++) launch_bicg_iteration(dc, col, m, n, dx, w, st, C.st);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(&h, st, sizeof h, hipMemcpyDeviceToHost, C.st));
+        TRY(sync());
+        batch = batch < 64 ? 2 * batch : 64;
+    }
+    TRY(d2h(x, dx, (size_t)n));
+    TRY(sync());
+    if (iterations) *iterations = h.j;
+    if (norm) *norm = h.norm;
+    return 0;
+}
+
 // ---- airfoil geometry -------------------------------------------------------
 
 int ntust_ray_casting(const double *az0, const double *ay0, int poly, double AOA, double zc,

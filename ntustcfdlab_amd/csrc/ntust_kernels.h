@@ -267,6 +267,24 @@ struct RayGeom {
 void launch_ray_casting(const KGrid &g, const RayGeom &r, int *points, int *counts,
                         double *eta1, double *eta, hipStream_t st);
 
+// Krylov path (k_krylov.hip, solver.f90).  coef: the n x 7 DIA columns;
+// col: jcoef(2..4); m: jcoef entries used (1..4).
+constexpr int KRY_MAX_BLOCKS = 1024;
+struct BicgState {
+    double rho1, rho2, alpha, omega, norm0, norm, eps;
+    int j, itmax, stop;
+};
+struct BicgVectors {
+    double *p, *r, *r2, *v, *ss, *t;
+    double *part;   // 2 * KRY_MAX_BLOCKS partial sums
+};
+void launch_dia_matvec(const double *coef, const long *col, int m, long n, const double *y,
+                       double *x, hipStream_t st);
+void launch_bicg_setup(const double *coef, const long *col, int m, long n, const double *b,
+                       const double *x, BicgVectors &w, BicgState *st, hipStream_t s);
+void launch_bicg_iteration(const double *coef, const long *col, int m, long n, double *x,
+                           BicgVectors &w, BicgState *st, hipStream_t s);
+
 // output and restart (k_output.hip)
 void launch_pack_q(const KGrid &g, const double *pre, const double *uc, const double *vc,
                    const double *wc, const double *eta, float *q, hipStream_t st);

@@ -27,6 +27,11 @@
 !   table's vertex count.
 ! OUT receives (stream, native endianness): nstep, ik(nstep) int32,
 !   cDrag(nstep), cLift(nstep), u, v, w, p (halo arrays), pre, uc, vc, wc.
+! MODE 2: the Krylov path instead of time steps -- solver_1st_member,
+!   solver_BICG3 (tolerance ZETA, cap ITMAX) from x = 0 for the right-hand
+!   side b(nx*ny*nz) read from IC, then matmul_ell of the solution, all
+!   through the shims of hotpath.f90 (solver_BICG3 prints the iteration
+!   count and norm as the reference does); OUT receives x and A x.
 program ntust_driver
    use variables
    use ntust_hip
@@ -62,6 +67,11 @@ program ntust_driver
    call uniform_grid(ny, ly, Y, iDy, Dys)
    call uniform_grid(nz, lz, Z, iDz, Dzs)
    call ntust_setup()
+   if (mode == 2) then
+      call krylov()
+      call ntust_teardown()
+      stop
+   end if
    if (len_trim(icfile) > 0) then
       open (newunit=lun, file=trim(icfile), access='stream', form='unformatted', status='old')
       read (lun) u, v, w
@@ -189,6 +199,27 @@ contains
                        'ntust_output_q')
       num = num + 1
    end subroutine output_resident
+
+   ! the Krylov path through the solver's own call sequence
+   subroutine krylov()
+      integer :: nd, dc(40)
+      integer :: jc(7)
+      real*8, allocatable :: cf(:, :), b(:), x(:), ax(:), wk(:, :)
+      nd = nx*ny*nz
+      allocate (cf(nd, 7), b(nd), x(nd), ax(nd), wk(nd, 6))
+      call solver_1st_member(cf, jc, nx, ny, nz, nd, 7, iDx, Dxs, iDy, Dys, iDz, Dzs)
+      open (newunit=lun, file=trim(icfile), access='stream', form='unformatted', status='old')
+      read (lun) b
+      close (lun)
+      x = 0
+      dc = 0
+      call solver_BICG3(cf, jc, b, x, nd, 4, zeta, wk(:, 1), wk(:, 2), wk(:, 3), wk(:, 4), &
+                        wk(:, 5), wk(:, 6), int(itmax), 0, 1, nz, dc)
+      call matmul_ell(ax, cf, jc, x, nd, 4)
+      open (newunit=lun, file=trim(out), access='stream', form='unformatted', status='replace')
+      write (lun) x, ax
+      close (lun)
+   end subroutine krylov
 
    ! check_steady.f90:1-30 on the host arrays
    subroutine steady_host()

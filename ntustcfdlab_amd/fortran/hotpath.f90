@@ -160,3 +160,43 @@ subroutine solver_1st_member(coef, jcoef, nx, ny, nz, ndim, mdim, iDx, Dxs, iDy,
    call ntust_check(ntust_poisson_coeffs(coef, jc), 'solver_1st_member')
    jcoef(1:4) = jc
 end subroutine solver_1st_member
+
+! solver.f90:1-144 solver_BICG3 with the reference's arguments; the work
+! vectors (p, r, r2, v, ss, t) and the MPI arguments are not used: the
+! iteration runs on the GPU.  Prints what the reference prints on rank 0.
+subroutine solver_BICG3(coef, jcoef, b, x, n, m, eps, p, r, r2, v, ss, t, itmax, myid, &
+                        iDIMstart, iDIMend, DIMcount)
+   use ntust_hip
+   implicit none
+   integer, intent(in) :: n, m, itmax
+   double precision, intent(in) :: eps
+   integer, dimension(m), intent(inout) :: jcoef
+   double precision, dimension(n, *), intent(inout) :: coef
+   double precision, dimension(n), intent(inout) :: b, x, p, r, r2, v, ss, t
+   integer :: myid, iDIMstart, iDIMend
+   integer, dimension(1:40) :: DIMcount
+   integer(c_int) :: it
+   real(c_double) :: nrm
+   call ntust_check(ntust_solver_bicg3(coef, jcoef, b, x, n, m, eps, itmax, it, nrm), &
+                    'solver_BICG3')
+   if (myid == 0) then
+      print *, '      '
+      if (it >= itmax) then
+         print *, 'non convergence =', it, nrm
+      else
+         print *, '  Iterations BICG3 =', it, nrm
+      end if
+   end if
+end subroutine solver_BICG3
+
+! solver.f90:146-190 matmul_ell: x = A y on the GPU, bit for bit
+subroutine matmul_ell(x, coef, jcoef, y, n, m)
+   use ntust_hip
+   implicit none
+   integer, intent(in) :: n, m
+   double precision, dimension(n, *), intent(in) :: coef
+   integer, dimension(m), intent(in) :: jcoef
+   double precision, dimension(n), intent(in) :: y
+   double precision, dimension(n), intent(out) :: x
+   call ntust_check(ntust_matmul_ell(x, coef, jcoef, y, n, m), 'matmul_ell')
+end subroutine matmul_ell

@@ -233,6 +233,28 @@ module ntust_hip
          integer(c_int), intent(out) :: jcoef(4)
       end function
 
+      integer(c_int) function ntust_matmul_ell(x, coef, jcoef, y, n, m) &
+            bind(C, name="ntust_matmul_ell")
+         import :: c_int, c_double
+         real(c_double), intent(out) :: x(*)
+         real(c_double), intent(in) :: coef(*), y(*)
+         integer(c_int), intent(in) :: jcoef(*)
+         integer(c_int), value :: n, m
+      end function
+
+      integer(c_int) function ntust_solver_bicg3(coef, jcoef, b, x, n, m, eps, itmax, &
+                                                 iterations, norm) &
+            bind(C, name="ntust_solver_bicg3")
+         import :: c_int, c_double
+         real(c_double), intent(in) :: coef(*), b(*)
+         integer(c_int), intent(in) :: jcoef(*)
+         real(c_double), intent(inout) :: x(*)
+         integer(c_int), value :: n, m, itmax
+         real(c_double), value :: eps
+         integer(c_int), intent(out) :: iterations
+         real(c_double), intent(out) :: norm
+      end function
+
       integer(c_int) function ntust_poisson_rhs(u_star, v_star, w_star, dt, b) &
             bind(C, name="ntust_poisson_rhs")
          import :: c_int, c_double

@@ -95,6 +95,8 @@ def load_library() -> ct.CDLL:
         "ntust_restart_write": [ct.c_char_p, i, d],
         "ntust_restart_read": [ct.c_char_p, _I, _D],
         "ntust_ray_casting": [_D, _D, i, d, d, d, _D, _D, i, _D],
+        "ntust_matmul_ell": [_D, _D, _I, _D, i, i],
+        "ntust_solver_bicg3": [_D, _I, _D, _D, i, i, d, i, _I, _D],
         "ntust_dynamic_aoa": [d, d, d, d, _D],
     }
     for name, args in sig.items():
@@ -341,6 +343,28 @@ class ProjectionSolver:
         st, t = ct.c_int(), ct.c_double()
         _check(self.lib.ntust_restart_read(str(path).encode(), ct.byref(st), ct.byref(t)))
         return st.value, t.value
+
+    # ---- Krylov path (solver.f90) ----
+
+    def matmul_ell(self, coef, jcoef, y):
+        """matmul_ell (solver.f90:146-190): returns A y."""
+        y = np.ascontiguousarray(y, np.float64)
+        x = np.zeros_like(y)
+        jc = np.ascontiguousarray(jcoef, np.int32)
+        _check(self.lib.ntust_matmul_ell(_p(x), _p(np.asfortranarray(coef)),
+                                         jc.ctypes.data_as(_I), _p(y), len(y), len(jc)))
+        return x
+
+    def solver_bicg3(self, coef, jcoef, b, x, eps: float, itmax: int):
+        """solver_BICG3 (solver.f90:1-144): x (initial guess) is overwritten by
+        the solution; returns (iterations, norm)."""
+        jc = np.ascontiguousarray(jcoef, np.int32)
+        it, nrm = ct.c_int(), ct.c_double()
+        _check(self.lib.ntust_solver_bicg3(_p(np.asfortranarray(coef)), jc.ctypes.data_as(_I),
+                                           _p(np.ascontiguousarray(b, np.float64)), _p(x),
+                                           len(x), len(jc), float(eps), int(itmax),
+                                           ct.byref(it), ct.byref(nrm)))
+        return it.value, nrm.value
 
     # ---- airfoil geometry ----
 

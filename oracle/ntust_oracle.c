@@ -1253,3 +1253,71 @@ double or_dynamic_aoa(double AOA1, double time, double dt, double StartDynamic)
     const double frequency = reduce_frequency / PI;
     return AOA1 + 10 * fabs(sin(2 * PI * frequency * (time - dt * (StartDynamic - 1))));
 }
+
+/* ------------------------------------------------------------------ */
+/* solver.f90: BiCGSTAB on the DIA matrix                              */
+/* ------------------------------------------------------------------ */
+
+/* solver.f90:146-190 matmul_ell: x = coef(:,1) y, then for j = 2..m the
+ * upper diagonal coef(:,j+a) y(i+col) and the lower coef(:,j+b) y(i-col),
+ * col = jcoef(j), (a,b) = (0,1), (1,2), (2,3) for j = 2, 3, 4: per row the
+ * terms are added in the order diag, +1, -1, +nx, -nx, +nxny, -nxny. */
+void or_matmul_ell(double *x, const double *coef, const int *jcoef,
+                   const double *y, int n, int m)
+{
+    for (int i = 0; i < n; i++) x[i] = coef[i] * y[i];
+    int a = 0, b = 0;
+    for (int j = 2; j <= m; j++) {
+        const int col = jcoef[j - 1];
+        if (j == 2) { a = 0; b = 1; }
+        else if (j == 3) { a = 1; b = 2; }
+        else if (j == 4) { a = 2; b = 3; }
+        const double *up = coef + (size_t)(j + a - 1) * n, *lo = coef + (size_t)(j + b - 1) * n;
+        for (int i = 0; i < n - col; i++) x[i] = x[i] + up[i] * y[i + col];
+        for (int i = col; i < n; i++) x[i] = x[i] + lo[i] * y[i - col];
+    }
+}
+
+static double or_scal(const double *x, const double *y, int n)
+{
+    double s = 0.0;
+    for (int i = 0; i < n; i++) s = s + x[i] * y[i];
+    return s;
+}
+
+/* solver.f90:1-144 solver_BICG3 (BiCGSTAB, no preconditioner), serial:
+ * the reductions sum left to right.  x: initial guess in, solution out.
+ * Returns the iteration count j; *norm gets ||r|| / ||b||. */
+int or_solver_bicg3(const double *coef, const int *jcoef, const double *b,
+                    double *x, int n, int m, double eps, int itmax, double *normout)
+{
+    double *p = calloc(n, sizeof(double)), *r = malloc(n * sizeof(double));
+    double *r2 = malloc(n * sizeof(double)), *v = calloc(n, sizeof(double));
+    double *ss = malloc(n * sizeof(double)), *t = malloc(n * sizeof(double));
+    const double norm0 = sqrt(or_scal(b, b, n));
+    or_matmul_ell(p, coef, jcoef, x, n, m);
+    for (int i = 0; i < n; i++) r[i] = b[i] - p[i];
+    for (int i = 0; i < n; i++) r2[i] = r[i];
+    double rho1 = 1, alpha = 1, omega = 1, rho2, beta;
+    for (int i = 0; i < n; i++) { v[i] = 0; p[i] = 0; }
+    double norm = sqrt(or_scal(r, r, n)) / norm0;
+    int j = 0;
+    while (norm > eps && j < itmax) {
+        j = j + 1;
+        rho2 = or_scal(r2, r, n);
+        beta = (rho2 / rho1) * (alpha / omega);
+        for (int i = 0; i < n; i++) p[i] = r[i] + beta * (p[i] - omega * v[i]);
+        or_matmul_ell(v, coef, jcoef, p, n, m);
+        alpha = rho2 / or_scal(r2, v, n);
+        for (int i = 0; i < n; i++) ss[i] = r[i] - alpha * v[i];
+        or_matmul_ell(t, coef, jcoef, ss, n, m);
+        omega = or_scal(t, ss, n) / or_scal(t, t, n);
+        for (int i = 0; i < n; i++) x[i] = x[i] + alpha * p[i] + omega * ss[i];
+        for (int i = 0; i < n; i++) r[i] = ss[i] - omega * t[i];
+        rho1 = rho2;
+        norm = sqrt(or_scal(r, r, n)) / norm0;
+    }
+    if (normout) *normout = norm;
+    free(p); free(r); free(r2); free(v); free(ss); free(t);
+    return j;
+}

@@ -207,6 +207,15 @@ int or_ray_casting(const or_grid *g, int poly, const double *az,
 /* airfoil.f90:311-321 dynamic_AOA (reduce_frequency = 2.0) */
 double or_dynamic_aoa(double AOA1, double time, double dt, double StartDynamic);
 
+/* solver.f90:146-190 matmul_ell: x = A y for the DIA matrix coef(n,7),
+ * jcoef(m) (m = 4: offsets 0, 1, nx, nx*ny) */
+void or_matmul_ell(double *x, const double *coef, const int *jcoef,
+                   const double *y, int n, int m);
+
+/* solver.f90:1-144 solver_BICG3, serial sums; returns the iteration count */
+int or_solver_bicg3(const double *coef, const int *jcoef, const double *b,
+                    double *x, int n, int m, double eps, int itmax, double *normout);
+
 #ifdef __cplusplus
 }
 #endif

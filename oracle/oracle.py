@@ -197,6 +197,28 @@ class Oracle:
                                  int(nsub), _p(ETA), _p(eta1), pts.ctypes.data_as(_I))
         return None if rc != 0 else (eta1, pts)
 
+    # solver.f90:146-190
+    @staticmethod
+    def matmul_ell(coef, jcoef, y):
+        n = len(y)
+        x = np.zeros(n)
+        jc = np.ascontiguousarray(jcoef, np.int32)
+        _lib.or_matmul_ell(_p(x), _p(np.asfortranarray(coef)), jc.ctypes.data_as(_I),
+                           _p(np.ascontiguousarray(y, np.float64)), n, len(jc))
+        return x
+
+    # solver.f90:1-144; returns (x, iterations, norm)
+    @staticmethod
+    def solver_bicg3(coef, jcoef, b, x0, eps, itmax):
+        x = np.array(x0, dtype=np.float64)
+        jc = np.ascontiguousarray(jcoef, np.int32)
+        nrm = ct.c_double()
+        _lib.or_solver_bicg3.restype = ct.c_int
+        it = _lib.or_solver_bicg3(_p(np.asfortranarray(coef)), jc.ctypes.data_as(_I),
+                                  _p(np.ascontiguousarray(b, np.float64)), _p(x), len(x),
+                                  len(jc), ct.c_double(eps), int(itmax), ct.byref(nrm))
+        return x, it, nrm.value
+
     # reading_data.f90:47-83
     def reading_variables(self, q, u, v, w, p, pre, uc, vc, wc):
         assert q.dtype == np.float32 and q.flags.f_contiguous

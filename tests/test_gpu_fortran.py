@@ -184,3 +184,37 @@ def test_fortran_driver_airfoil_pitching(mode, tmp_path, hip_lib):
     assert np.all(cd != 0)
     for name in ("u", "v", "w", "p", "pre", "uc", "vc", "wc"):
         assert np.array_equal(out[name].view(np.uint64), ref.f[name].view(np.uint64)), name
+
+
+def test_fortran_driver_krylov_path(tmp_path, hip_lib):
+    """solver_1st_member, solver_BICG3 and matmul_ell through the shims of
+    hotpath.f90 with the reference's argument lists: the solution equals the
+    library's own call bit for bit, the printed count and norm are the
+    reference's messages, and A x is the restatement's product of x."""
+    from ntustcfdlab_amd.hip import ProjectionSolver
+    from oracle import Oracle
+    assert DRIVER.exists(), "Fortran driver not built"
+    n, eps, itmax = 14, 1e-9, 2000
+    g = Grid(n, n, n)
+    b = np.random.default_rng(4).standard_normal(g.cells)
+    ic = tmp_path / "b.bin"
+    ic.write_bytes(b.tobytes())
+    args = [str(DRIVER), str(tmp_path / "out.bin"), *(str(n),) * 3, "1000.0", "5e-4", "1",
+            "0", "0", "0.1", "1.5", repr(eps), repr(float(itmax)), "1", "2", str(ic)]
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = np.frombuffer((tmp_path / "out.bin").read_bytes(), np.float64)
+    x, ax = raw[:g.cells], raw[g.cells:]
+    line = [ln for ln in r.stdout.splitlines() if "Iterations BICG3" in ln]
+    assert len(line) == 1, r.stdout
+    it, nrm = line[0].split("=")[1].split()
+    s = ProjectionSolver(g)
+    try:
+        coef, jcoef = s.solver_1st_member()
+        xl = np.zeros(g.cells)
+        itl, nrml = s.solver_bicg3(coef, jcoef, b, xl, eps, itmax)
+    finally:
+        s.close()
+    assert int(it) == itl and float(nrm) == pytest.approx(nrml, rel=1e-12) and nrml <= eps
+    assert np.array_equal(x.view(np.uint64), xl.view(np.uint64))
+    assert np.array_equal(ax.view(np.uint64), Oracle.matmul_ell(coef, jcoef, x).view(np.uint64))
