@@ -184,7 +184,10 @@ __global__ void k_rb_scatter(KGrid g, SplitGrid s, const double *__restrict__ p0
 // before the gate value is waited on (no per-cell branch, indices clamped
 // into the arrays), so a wavefront pays one memory latency; only the stores
 // depend on the gate and on the cell being interior.  Offsets are 32-bit.
-template <int C>
+// SPEC: the speculative form -- no gate, no control block: the sweep runs
+// and records its maximum; the host decides afterwards whether the
+// reference's loop would have run it (and replays from a checkpoint if not).
+template <int C, bool SPEC>
 __global__ void __launch_bounds__(256)
 k_rb_sweep(KGrid g, SplitGrid s, const double *__restrict__ rx,
            const double *__restrict__ ry, const double *__restrict__ rz,
@@ -196,9 +199,13 @@ k_rb_sweep(KGrid g, SplitGrid s, const double *__restrict__ rx,
     const int tid = threadIdx.x + threadIdx.y * blockDim.x;
     const int blin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
     const int lane = tid & 63;
-    const int t = sweep_index<C>(targ, ctl);
-    double gate = lane < SLOT_BUCKETS ? load_flag(&slots[slot_index(t - 1, lane)]) : 0.0;
-    const int stopped = load_flag(&ctl->stop);
+    const int t = SPEC ? targ : sweep_index<C>(targ, ctl);
+    double gate = 0.0;
+    int stopped = 0;
+    if (!SPEC) {
+        gate = lane < SLOT_BUCKETS ? load_flag(&slots[slot_index(t - 1, lane)]) : 0.0;
+        stopped = load_flag(&ctl->stop);
+    }
 
     // natural block order: measured faster for this FP64 kernel at 256^3
     // than the XCD-slab order the FP32 kernel uses
@@ -253,17 +260,19 @@ k_rb_sweep(KGrid g, SplitGrid s, const double *__restrict__ rx,
         if (pc > pcm) pcm = pc;   // a NaN change never raises the max (GS.f90:61)
         nv[r] = fma(omega, d, vo[r]);
     }
-    // every wavefront of the block reads the same buckets: the decision is
-    // uniform over the block, so returning here cannot split a barrier
-    gate = wave_max(gate);
-    const bool active = !stopped && ((double)(t - 1) < itmax) && (gate > zeta);
-    if (!active) {
-        if (blin == 0 && tid == 0) ctl->stop = 1;
-        return;
-    }
-    if (blin == 0) {
-        if (C == 0) clear_next_buckets(slots, t, tid);
-        if (tid == 0) pass_sweep_index<C>(t, ctl);
+    if (!SPEC) {
+        // every wavefront of the block reads the same buckets: the decision
+        // is uniform over the block, so returning here cannot split a barrier
+        gate = wave_max(gate);
+        const bool active = !stopped && ((double)(t - 1) < itmax) && (gate > zeta);
+        if (!active) {
+            if (blin == 0 && tid == 0) ctl->stop = 1;
+            return;
+        }
+        if (blin == 0) {
+            if (C == 0) clear_next_buckets(slots, t, tid);
+            if (tid == 0) pass_sweep_index<C>(t, ctl);
+        }
     }
 #pragma unroll
     for (int r = 0; r < RB_KPT; r++)
@@ -277,7 +286,7 @@ k_rb_sweep(KGrid g, SplitGrid s, const double *__restrict__ rx,
         double *bk = &slots[slot_index(t, blin % SLOT_BUCKETS)];
         if (mm > 0.0 && mm > load_flag(bk))
             atomicMax((unsigned long long *)bk, (unsigned long long)__double_as_longlong(mm));
-        if (C == 1 && blin == 0) ctl->ik = t;
+        if (!SPEC && C == 1 && blin == 0) ctl->ik = t;
     }
 }
 
@@ -407,10 +416,55 @@ void launch_rb_sweep(const KGrid &g, const SplitGrid &s, const double *rx, const
 {
     const int nq = (g.nx + 1) / 2;
     dim3 blk(64, 4), grd((nq + 63) / 64, (g.ny + 3) / 4, (g.nz + RB_KPT - 1) / RB_KPT);
-    hipLaunchKernelGGL(k_rb_sweep<0>, grd, blk, 0, st, g, s, rx, ry, rz, p0, p1, b0, omega,
-                       zeta, itmax, t, slots, ctl);
-    hipLaunchKernelGGL(k_rb_sweep<1>, grd, blk, 0, st, g, s, rx, ry, rz, p1, p0, b1, omega,
-                       zeta, itmax, t, slots, ctl);
+    hipLaunchKernelGGL((k_rb_sweep<0, false>), grd, blk, 0, st, g, s, rx, ry, rz, p0, p1, b0,
+                       omega, zeta, itmax, t, slots, ctl);
+    hipLaunchKernelGGL((k_rb_sweep<1, false>), grd, blk, 0, st, g, s, rx, ry, rz, p1, p0, b1,
+                       omega, zeta, itmax, t, slots, ctl);
+}
+
+void launch_rb_sweep_spec(const KGrid &g, const SplitGrid &s, const double *rx, const double *ry,
+                          const double *rz, double *p0, double *p1, const double *b0,
+                          const double *b1, double omega, int t, double *slots, hipStream_t st)
+{
+    const int nq = (g.nx + 1) / 2;
+    dim3 blk(64, 4), grd((nq + 63) / 64, (g.ny + 3) / 4, (g.nz + RB_KPT - 1) / RB_KPT);
+    hipLaunchKernelGGL((k_rb_sweep<0, true>), grd, blk, 0, st, g, s, rx, ry, rz, p0, p1, b0,
+                       omega, 0.0, 0.0, t, slots, nullptr);
+    hipLaunchKernelGGL((k_rb_sweep<1, true>), grd, blk, 0, st, g, s, rx, ry, rz, p1, p0, b1,
+                       omega, 0.0, 0.0, t, slots, nullptr);
+}
+
+// zero the buckets of sweeps t0 .. t0+n-1
+__global__ void __launch_bounds__(256)
+k_slots_clear(double *slots, int t0, int n)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n * SLOT_BUCKETS) slots[slot_index(t0 + q / SLOT_BUCKETS, q % SLOT_BUCKETS)] = 0.0;
+}
+
+// m[s] = the maximum change of sweep t0+s (over its buckets)
+__global__ void __launch_bounds__(256)
+k_slots_max(double *slots, int t0, int n, double *m)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    double v = 0.0;
+    for (int b = 0; b < SLOT_BUCKETS; b++) {
+        const double x = load_flag(&slots[slot_index(t0 + q, b)]);
+        v = x > v ? x : v;
+    }
+    m[q] = v;
+}
+
+void launch_slots_clear(double *slots, int t0, int n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_slots_clear, dim3((n * SLOT_BUCKETS + 255) / 256), dim3(256), 0, st,
+                       slots, t0, n);
+}
+
+void launch_slots_max(double *slots, int t0, int n, double *m, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_slots_max, dim3((n + 255) / 256), dim3(256), 0, st, slots, t0, n, m);
 }
 
 }  // namespace ntust

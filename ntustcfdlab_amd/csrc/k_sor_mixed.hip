@@ -21,7 +21,9 @@
 
 namespace ntust {
 
-template <int C>
+// SPEC: the speculative form (no gate, no control block, no dirty flag),
+// as for the FP64 sweep
+template <int C, bool SPEC>
 __global__ void __launch_bounds__(256)
 k_rb_sweep32(KGrid g, SplitGrid s, FMetric fm, float *__restrict__ e_own,
              const float *__restrict__ e_oth, const float *__restrict__ r_own, float omega,
@@ -32,9 +34,13 @@ k_rb_sweep32(KGrid g, SplitGrid s, FMetric fm, float *__restrict__ e_own,
     const int tid = threadIdx.x + threadIdx.y * blockDim.x;
     const int blin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
     const int lane = tid & 63;
-    const int t = sweep_index<C>(targ, ctl);
-    double gate = lane < SLOT_BUCKETS ? load_flag(&slots[slot_index(t - 1, lane)]) : 0.0;
-    const int stopped = load_flag(&ctl->stop);
+    const int t = SPEC ? targ : sweep_index<C>(targ, ctl);
+    double gate = 0.0;
+    int stopped = 0;
+    if (!SPEC) {
+        gate = lane < SLOT_BUCKETS ? load_flag(&slots[slot_index(t - 1, lane)]) : 0.0;
+        stopped = load_flag(&ctl->stop);
+    }
 
     // same thread layout and load-first structure as the FP64 sweep
     int bx, by, bz;
@@ -89,18 +95,20 @@ k_rb_sweep32(KGrid g, SplitGrid s, FMetric fm, float *__restrict__ e_own,
         if (pc > pcm) pcm = pc;
         nv[r] = fmaf(omega, d, vo[r]);
     }
-    gate = wave_max(gate);
-    const bool active = !stopped && ((double)(t - 1) < itmax) && (gate > zeta);
-    if (!active) {
-        if (blin == 0 && tid == 0) {
-            ctl->stop = 1;
-            if (first) *dirty = 0;   // no sweep of this segment runs
+    if (!SPEC) {
+        gate = wave_max(gate);
+        const bool active = !stopped && ((double)(t - 1) < itmax) && (gate > zeta);
+        if (!active) {
+            if (blin == 0 && tid == 0) {
+                ctl->stop = 1;
+                if (first) *dirty = 0;   // no sweep of this segment runs
+            }
+            return;
         }
-        return;
-    }
-    if (blin == 0) {
-        if (C == 0) clear_next_buckets(slots, t, tid);
-        if (tid == 0) pass_sweep_index<C>(t, ctl);
+        if (blin == 0) {
+            if (C == 0) clear_next_buckets(slots, t, tid);
+            if (tid == 0) pass_sweep_index<C>(t, ctl);
+        }
     }
 #pragma unroll
     for (int r = 0; r < RB_KPT; r++)
@@ -114,7 +122,7 @@ k_rb_sweep32(KGrid g, SplitGrid s, FMetric fm, float *__restrict__ e_own,
         double *bk = &slots[slot_index(t, blin % SLOT_BUCKETS)];
         if (mm > 0.0 && mm > load_flag(bk))
             atomicMax((unsigned long long *)bk, (unsigned long long)__double_as_longlong(mm));
-        if (blin == 0) {
+        if (!SPEC && blin == 0) {
             *dirty = 1;
             if (C == 1) ctl->ik = t;
         }
@@ -122,11 +130,12 @@ k_rb_sweep32(KGrid g, SplitGrid s, FMetric fm, float *__restrict__ e_own,
 }
 
 // p0 += e on the interior, e = 0; skipped when no sweep of the segment ran
+// (dirty NULL: always)
 __global__ void __launch_bounds__(256)
 k_rb_absorb(KGrid g, SplitGrid s, double *__restrict__ p0, double *__restrict__ p1,
             float *__restrict__ e0, float *__restrict__ e1, const int *dirty)
 {
-    if (!load_flag(dirty)) return;
+    if (dirty && !load_flag(dirty)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const int k = blockIdx.z + 1;
@@ -178,10 +187,22 @@ void launch_rb_sweep32(const KGrid &g, const SplitGrid &s, const FMetric &fm, fl
 {
     const int nq = (g.nx + 1) / 2;
     dim3 blk(64, 4), grd((nq + 63) / 64, (g.ny + 3) / 4, (g.nz + RB_KPT - 1) / RB_KPT);
-    hipLaunchKernelGGL(k_rb_sweep32<0>, grd, blk, 0, st, g, s, fm, e0, e1, r0, omega, zeta,
+    hipLaunchKernelGGL((k_rb_sweep32<0, false>), grd, blk, 0, st, g, s, fm, e0, e1, r0, omega, zeta,
                        itmax, t, slots, ctl, dirty, first);
-    hipLaunchKernelGGL(k_rb_sweep32<1>, grd, blk, 0, st, g, s, fm, e1, e0, r1, omega, zeta,
+    hipLaunchKernelGGL((k_rb_sweep32<1, false>), grd, blk, 0, st, g, s, fm, e1, e0, r1, omega, zeta,
                        itmax, t, slots, ctl, dirty, 0);
+}
+
+void launch_rb_sweep32_spec(const KGrid &g, const SplitGrid &s, const FMetric &fm, float *e0,
+                            float *e1, const float *r0, const float *r1, float omega, int t,
+                            double *slots, hipStream_t st)
+{
+    const int nq = (g.nx + 1) / 2;
+    dim3 blk(64, 4), grd((nq + 63) / 64, (g.ny + 3) / 4, (g.nz + RB_KPT - 1) / RB_KPT);
+    hipLaunchKernelGGL((k_rb_sweep32<0, true>), grd, blk, 0, st, g, s, fm, e0, e1, r0, omega, 0.0,
+                       0.0, t, slots, nullptr, nullptr, 0);
+    hipLaunchKernelGGL((k_rb_sweep32<1, true>), grd, blk, 0, st, g, s, fm, e1, e0, r1, omega, 0.0,
+                       0.0, t, slots, nullptr, nullptr, 0);
 }
 
 void launch_rb_absorb(const KGrid &g, const SplitGrid &s, double *p0, double *p1, float *e0,

@@ -99,3 +99,47 @@ def test_graph_replays_follow_changed_arguments(mode, hip_lib):
         assert a[0] == b[0] and a[1] == b[1]
         assert np.array_equal(a[2].view(np.uint64), b[2].view(np.uint64))
         assert np.array_equal(a[3].view(np.uint64), b[3].view(np.uint64))
+
+
+@pytest.mark.parametrize("mode", [SOR_REDBLACK, SOR_MIXED])
+def test_speculative_first_batch_and_replay(mode, hip_lib):
+    """The two-pass solve runs its first batch (7/8 of the previous solve's
+    sweep count) without the device gate and replays from the initial state
+    when the loop would have stopped inside it.  A sequence of solves whose
+    counts drop (a replay), hold (none) and rise (the gated tail) must give
+    the same bits as the gated graph replay, solve by solve."""
+    g = stretched_grid(26, 22, 20)
+    rng = np.random.default_rng(31)
+    us, vs, ws = (rand_halo(g, rng, 1e-2) for _ in range(3))
+    p0 = rand_halo(g, rng, 1e-3)
+    probe = ProjectionSolver(g)
+    try:
+        probe.set_sweep_variant(SWEEP_GRAPH)
+        p = p0.copy(order="F")
+        _, pm300, _ = probe.gauss_seidel(us, vs, ws, p, g.interior(), 1e-3, 1.5, 0.0, 300.0,
+                                         mode)
+    finally:
+        probe.close()
+    z = pm300 * (1.0 + 1e-9)
+    calls = [(0.0, 700.0), (z, 700.0), (z, 700.0), (0.0, 700.0), (z, 700.0), (0.0, 90.0)]
+    res = {}
+    for v in (SWEEP_TWO_PASS, SWEEP_GRAPH):
+        s = ProjectionSolver(g)
+        try:
+            s.set_sweep_variant(v)
+            out = []
+            for zeta, itmax in calls:
+                p = p0.copy(order="F")
+                pre = g.interior()
+                ik, pm, _ = s.gauss_seidel(us, vs, ws, p, pre, 1e-3, 1.5, zeta, itmax, mode)
+                out.append((ik, pm, p, pre))
+            res[v] = out
+        finally:
+            s.close()
+    iks = [r[0] for r in res[SWEEP_TWO_PASS]]
+    assert iks[0] == 700 and iks[3] == 700 and iks[5] == 90
+    assert 250 <= iks[1] <= 310 and iks[1] < 0.875 * 700     # replayed
+    for a, b in zip(res[SWEEP_TWO_PASS], res[SWEEP_GRAPH]):
+        assert a[0] == b[0] and a[1] == b[1]
+        assert np.array_equal(a[2].view(np.uint64), b[2].view(np.uint64))
+        assert np.array_equal(a[3].view(np.uint64), b[3].view(np.uint64))
