@@ -265,7 +265,10 @@ k_rb_sweep_stream(KGrid g, SplitGrid s, SweepMetric<T> m, const T *__restrict__ 
         double mm = red[0];
         for (int w = 1; w < NT / 64; w++) mm = red[w] > mm ? red[w] : mm;
         double *bk = &slots[slot_index(t, blin % SLOT_BUCKETS)];
-        if (mm > 0.0 && mm > load_flag(bk))
+        // a posted atomic (no return, no uncached pre-read of the bucket,
+        // which held every block open for a memory round trip: 12 % of an
+        // FP32 launch at 256^3)
+        if (mm > 0.0)
             atomicMax((unsigned long long *)bk, (unsigned long long)__double_as_longlong(mm));
         if (blin == 0) {
             ctl->ik = t;
