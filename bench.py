@@ -17,7 +17,8 @@ velocity is within 1e-6 (relative) of the FP64 solver's.
 
 Further legs in the same JSON line: the same cavity at 256^3 (the size the
 >= 70 % HBM and >= 2x FP32-refine targets refer to), the airfoil's ETA ray
-casting on a 512x384x128 grid (the per-step cost of the pitching mode), the
+casting on a 512x384x128 grid (the per-step cost of the pitching mode) and the
+airfoil LES on that grid (both solvers), the
 discrete adjoint on a 96^3 Re=400 cavity (cost relative to the forward,
 central-difference check), and the CPU restatement timed on a bounded
 sample.  One JSON line on stdout.
@@ -217,6 +218,75 @@ def airfoil_leg(hip, calls, cpu):
     return out
 
 
+def airfoil_les_leg(hip, hiptime, steps, warmup):
+    """The large airfoil configuration, forward: NACA 0012 (5000 vertices) at
+    8 degrees, Re = 5000, QUICK, Smagorinsky LES Cs = 0.1, dt = 5e-4, on
+    512 x 384 x 128 cells (span x, normal y, chord z; y and z tanh-clustered
+    at the airfoil, domain 1 x 8 x 8), ETA cast on the device, SOR omega 1.5,
+    zeta 1e-10, itmax 3000.  Uniform inflow w = 1 through the back face,
+    Neumann outflow at the front, Neumann tangential components on the other
+    faces (this leg's own choice: the reference ships only the cavity's
+    boundary conditions).  Both solvers from the same state; C_D, C_L of the
+    last step from the force integral."""
+    from ntustcfdlab_amd.airfoil import naca4_polygon
+    from ntustcfdlab_amd.grid import Grid, bc_codes, tanh_nodes
+    nx, ny, nz = 512, 384, 128
+    g = Grid(nx, ny, nz, lx=1.0, ly=8.0, lz=8.0, Y=tanh_nodes(ny, 8.0, 0.5, 2.5),
+             Z=tanh_nodes(nz, 8.0, 0.5, 2.5))
+    bc = bc_codes(BackWall_w="Dirichlet", FrontWall_u="Neumann", FrontWall_v="Neumann",
+                  FrontWall_w="Neumann", NorthWall_u="Neumann", NorthWall_w="Neumann",
+                  SouthWall_u="Neumann", SouthWall_w="Neumann", WestWall_v="Neumann",
+                  WestWall_w="Neumann", EastWall_v="Neumann", EastWall_w="Neumann")
+    s = hip.ProjectionSolver(g, bc)
+
+    def configure_sor(sor):
+        s.configure(dt=5e-4, nu=1.0 / 5000.0, Cs=0.1, LES=1, scheme=hip.QUICK_CENTRE, omega=1.5,
+                    zeta=1e-10, itmax=3000.0, sor=sor, lx=1.0, force_every=1)
+    s.configure_sor = configure_sor
+    configure_sor(hip.SOR_REDBLACK)
+    u, v, w, p = g.halo(), g.halo(), g.halo(), g.halo()
+    w[...] = 1.0
+    s.boundary_conditions(u, v, w, p)
+    eta = g.halo()
+    az0, ay0 = naca4_polygon(poly=5000)
+    s.ray_casting(az0, ay0, 8.0, 4.0, 4.0, ETA=eta)
+    state = (u, v, w, p, eta)
+    out = {"workload": "NACA 0012 at 8 deg, Re 5000, LES Cs 0.1, 512x384x128 tanh-clustered, "
+                       "inflow w = 1 along z", "steps": steps, "warmup": warmup, "solvers": {}}
+    finals = {}
+    try:
+        for name, sor in (("fp64", hip.SOR_REDBLACK), ("mixed", hip.SOR_MIXED)):
+            print(f"# airfoil LES {name}", file=sys.stderr, flush=True)
+            s.configure_sor(sor)
+            s.upload(*state)
+            if warmup:
+                s.step(warmup)
+            e0, e1 = hiptime.Event(), hiptime.Event()
+            hiptime.device_synchronize()
+            e0.record(s.stream())
+            ik, pm, cd, cl, rc = s.step(steps)
+            e1.record(s.stream())
+            ms = e0.elapsed_ms(e1)
+            hiptime.device_synchronize()
+            final = [g.halo() for _ in range(3)]
+            s.download(*final)
+            finals[name] = final
+            out["solvers"][name] = {"value": round(g.cells * steps / (ms / 1e3) / 1e6, 3),
+                                    "ms_per_step": round(ms / steps, 3),
+                                    "sweeps_per_step": round(float(np.mean(ik)), 1),
+                                    "cD": float(cd[-1]), "cL": float(cl[-1])}
+    finally:
+        s.close()
+    ref = finals["fp64"]
+    vel = max(np.abs(x).max() for x in ref)
+    out["mixed_vs_fp64_rel_velocity_err"] = float(
+        max(np.abs(x - y).max() for x, y in zip(finals["mixed"], ref)) / vel)
+    out["mixed_speedup_vs_fp64"] = round(out["solvers"]["mixed"]["value"] /
+                                         out["solvers"]["fp64"]["value"], 3)
+    out["solid_cells"] = float(eta[2:-2, 2:-2, 2:-2].sum())
+    return out
+
+
 def krylov_leg(hip, n):
     """solver_BICG3 (solver.f90:1-144) on the solver_1st_member matrix of a
     uniform n^3 grid: unpreconditioned BiCGSTAB to ||r||/||b|| <= 1e-8 from
@@ -385,8 +455,8 @@ def main():
         print(json.dumps({"krylov_bicgstab": krylov_leg(hip, 128)}))
         return
     if a.only_airfoil:
-        print(json.dumps({"airfoil_ray_casting": airfoil_leg(hip, 10,
-                                                             not a.no_cpu_baseline)}))
+        print(json.dumps({"airfoil_ray_casting": airfoil_leg(hip, 10, not a.no_cpu_baseline),
+                          "airfoil_les_512x384x128": airfoil_les_leg(hip, hiptime, 2, 1)}))
         return
     if a.only_adjoint:
         print(json.dumps({"adjoint": adjoint_leg(hip, hiptime, a.adjoint_size,
@@ -427,6 +497,7 @@ def main():
     if not a.no_airfoil and not a.only:
         print("# airfoil geometry", file=sys.stderr, flush=True)
         out["airfoil_ray_casting"] = airfoil_leg(hip, 10, not a.no_cpu_baseline)
+        out["airfoil_les_512x384x128"] = airfoil_les_leg(hip, hiptime, 2, 1)
     if not a.no_adjoint:
         print("# adjoint", file=sys.stderr, flush=True)
         out["adjoint"] = adjoint_leg(hip, hiptime, a.adjoint_size, a.adjoint_steps,
