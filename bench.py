@@ -348,7 +348,7 @@ def measure(hip, hiptime, size, steps, warmup, sweep, only, calibrate):
     g = Grid(w["nx"], w["ny"], w["nz"])
     s = hip.ProjectionSolver(g, bc_codes())
     s.set_sweep_variant({"fused": hip.SWEEP_FUSED, "two-pass": hip.SWEEP_TWO_PASS,
-                         "graph": hip.SWEEP_GRAPH}[sweep])
+                         "graph": hip.SWEEP_GRAPH, "tile": hip.SWEEP_TILE}[sweep])
     if calibrate:
         hiptime.copy_calibration()
 
@@ -437,10 +437,10 @@ def main():
                     help="run the airfoil ray-casting leg alone and print its JSON")
     ap.add_argument("--only-krylov", action="store_true",
                     help="run the BiCGSTAB leg alone and print its JSON")
-    ap.add_argument("--sweep", choices=["two-pass", "graph", "fused"], default="two-pass",
+    ap.add_argument("--sweep", choices=["two-pass", "graph", "fused", "tile"], default="two-pass",
                     help="red-black sweep launches (bit-identical results): two-pass "
                          "kernels from the stream, the same replayed from a HIP graph, "
-                         "or the fused one-launch sweep")
+                         "the z-streaming one-launch sweep, or the 3-D-tile one")
     ap.add_argument("--only-adjoint", action="store_true",
                     help="run the adjoint leg alone and print its JSON")
     a = ap.parse_args()

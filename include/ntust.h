@@ -54,9 +54,11 @@ enum {
     NTUST_SWEEP_TWO_PASS = 0,     /* one launch per colour, in place, from the
                                      stream (default) */
     NTUST_SWEEP_FUSED = 1,        /* one launch per sweep on LDS tiles */
-    NTUST_SWEEP_GRAPH = 2         /* the two-pass launches replayed from a captured
+    NTUST_SWEEP_GRAPH = 2,        /* the two-pass launches replayed from a captured
                                      HIP graph, 32 sweeps (FP64) or one FP32
                                      segment per replay */
+    NTUST_SWEEP_TILE = 3          /* one launch per sweep on 3-D tiles (red halo
+                                     recomputed in LDS, no plane loop) */
 };
 
 /* advection schemes (ntust_configure scheme) */
@@ -360,7 +362,8 @@ int ntust_prof_reset(void);
 int ntust_prof_read(double *ms, long long *launches);
 
 /* The timed kernel: returns 0 two-pass FP64, 1 fused FP64, 2 two-pass FP32,
- * 3 fused FP32 sweep; *bytes_per_cell gets its algorithmic bytes per grid
+ * 3 fused FP32 sweep, 4 tile FP64 sweep, 5 tile FP32 sweep; *bytes_per_cell
+ * gets its algorithmic bytes per grid
  * cell and launch (read p and b, write p). */
 int ntust_prof_kernel(double *bytes_per_cell);
 

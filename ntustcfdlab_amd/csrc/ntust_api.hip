@@ -264,7 +264,10 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
         return fail(NTUST_E_ARG, "unsupported SOR mode %d", mode);
 
     const size_t np = (size_t)C.sg.plane;
-    const bool fused = C.sweep_variant == NTUST_SWEEP_FUSED;
+    // one launch per sweep: the z-streaming or the 3-D-tile kernel, both
+    // reading one copy of the colour planes and writing the other
+    const bool tile = C.sweep_variant == NTUST_SWEEP_TILE;
+    const bool fused = C.sweep_variant == NTUST_SWEEP_FUSED || tile;
     NEED(P0, "sor_p0", np, true);
     NEED(P1, "sor_p1", np, true);
     NEED(B0, "sor_b0", np, true);
@@ -416,10 +419,10 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
                     const long tt = t + b;
                     if (fused) {
                         const bool rd = (tt - 1) & 1, wr = tt & 1;
-                        launch_rb_sweep32_fused(g, C.sg, fm, rd ? F0 : E0, rd ? F1 : E1,
-                                                wr ? F0 : E0, wr ? F1 : E1, R0, R1,
-                                                (float)omega, zeta, itmax, (int)tt, slots,
-                                                C.ctl, dirty + c, C.st);
+                        (tile ? launch_rb_sweep32_tile : launch_rb_sweep32_fused)(
+                            g, C.sg, fm, rd ? F0 : E0, rd ? F1 : E1, wr ? F0 : E0,
+                            wr ? F1 : E1, R0, R1, (float)omega, zeta, itmax, (int)tt, slots,
+                            C.ctl, dirty + c, C.st);
                     } else {
                         launch_rb_sweep32(g, C.sg, fm, E0, E1, R0, R1, (float)omega, zeta,
                                           itmax, (int)tt, slots, C.ctl, dirty + c,
@@ -523,9 +526,9 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
             const long tt = t + b;
             if (fused) {
                 const bool rd = (tt - 1) & 1, wr = tt & 1;
-                launch_rb_sweep_fused(g, C.sg, rx, ry, rz, rd ? Q0 : P0, rd ? Q1 : P1,
-                                      wr ? Q0 : P0, wr ? Q1 : P1, B0, B1, omega, zeta, itmax,
-                                      (int)tt, slots, C.ctl, C.st);
+                (tile ? launch_rb_sweep_tile : launch_rb_sweep_fused)(
+                    g, C.sg, rx, ry, rz, rd ? Q0 : P0, rd ? Q1 : P1, wr ? Q0 : P0,
+                    wr ? Q1 : P1, B0, B1, omega, zeta, itmax, (int)tt, slots, C.ctl, C.st);
             } else {
                 launch_rb_sweep(g, C.sg, rx, ry, rz, P0, P1, B0, B1, omega, zeta, itmax,
                                 (int)tt, slots, C.ctl, C.st);
@@ -1631,7 +1634,7 @@ int ntust_prof_reset(void)
 int ntust_set_sweep_variant(int variant)
 {
     if (variant != NTUST_SWEEP_TWO_PASS && variant != NTUST_SWEEP_FUSED &&
-        variant != NTUST_SWEEP_GRAPH)
+        variant != NTUST_SWEEP_GRAPH && variant != NTUST_SWEEP_TILE)
         return fail(NTUST_E_ARG, "unknown sweep variant %d", variant);
     C.sweep_variant = variant;
     return 0;
@@ -1642,8 +1645,10 @@ int ntust_prof_kernel(double *bytes_per_cell)
     // algorithmic bytes per grid cell of one timed launch: read p and b,
     // write p per sweep (24 B FP64, 12 B FP32); a two-pass launch is half
     const bool mixed = C.prm.sor == NTUST_SOR_MIXED;
-    const bool fused = C.sweep_variant == NTUST_SWEEP_FUSED;
+    const bool tile = C.sweep_variant == NTUST_SWEEP_TILE;
+    const bool fused = C.sweep_variant == NTUST_SWEEP_FUSED || tile;
     if (bytes_per_cell) *bytes_per_cell = (mixed ? 12.0 : 24.0) / (fused ? 1.0 : 2.0);
+    if (tile) return mixed ? 5 : 4;
     return mixed ? (fused ? 3 : 2) : (fused ? 1 : 0);
 }
 

@@ -194,6 +194,18 @@ void launch_rb_sweep32(const KGrid &g, const SplitGrid &s, const FMetric &fm, fl
 void launch_rb_sweep_spec(const KGrid &g, const SplitGrid &s, const double *rx, const double *ry,
                           const double *rz, double *p0, double *p1, const double *b0,
                           const double *b1, double omega, int t, double *slots, hipStream_t st);
+// one launch per sweep on 3-D tiles (k_sor_tile.hip); arguments as the
+// z-streaming fused sweeps
+void launch_rb_sweep_tile(const KGrid &g, const SplitGrid &s, const double *rx, const double *ry,
+                          const double *rz, const double *in0, const double *in1, double *out0,
+                          double *out1, const double *b0, const double *b1, double omega,
+                          double zeta, double itmax, int t, double *slots, SorControl *ctl,
+                          hipStream_t st);
+void launch_rb_sweep32_tile(const KGrid &g, const SplitGrid &s, const FMetric &fm,
+                            const float *in0, const float *in1, float *out0, float *out1,
+                            const float *r0, const float *r1, float omega, double zeta,
+                            double itmax, int t, double *slots, SorControl *ctl, int *dirty,
+                            hipStream_t st);
 void launch_rb_sweep32_spec(const KGrid &g, const SplitGrid &s, const FMetric &fm, float *e0,
                             float *e1, const float *r0, const float *r1, float omega, int t,
                             double *slots, hipStream_t st);

@@ -13,7 +13,7 @@ module ntust_hip
                                 NTUST_SOR_MIXED = 2
    integer(c_int), parameter :: NTUST_QUICK_CENTRE = 0, NTUST_UPWIND = 1
    integer(c_int), parameter :: NTUST_SWEEP_TWO_PASS = 0, NTUST_SWEEP_FUSED = 1, &
-                                NTUST_SWEEP_GRAPH = 2
+                                NTUST_SWEEP_GRAPH = 2, NTUST_SWEEP_TILE = 3
 
    interface
       integer(c_int) function ntust_init(nx, ny, nz, iDx, Dxs, iDy, Dys, iDz, Dzs) &

@@ -27,11 +27,13 @@ QUICK_CENTRE, UPWIND = 0, 1
 WARN_ITMAX = 1
 E_STATE = -5
 E_IO = -6
-SWEEP_TWO_PASS, SWEEP_FUSED, SWEEP_GRAPH = 0, 1, 2
+SWEEP_TWO_PASS, SWEEP_FUSED, SWEEP_GRAPH, SWEEP_TILE = 0, 1, 2, 3
 PROF_KERNELS = {0: ("k_rb_sweep<C>", "two-pass FP64 half-sweep"),
                 1: ("k_rb_sweep_fused<double>", "fused FP64 sweep"),
                 2: ("k_rb_sweep32<C>", "two-pass FP32 half-sweep"),
-                3: ("k_rb_sweep_fused<float>", "fused FP32 sweep")}
+                3: ("k_rb_sweep_fused<float>", "fused FP32 sweep"),
+                4: ("k_rb_sweep_tile<double>", "3-D-tile FP64 sweep"),
+                5: ("k_rb_sweep_tile<float>", "3-D-tile FP32 sweep")}
 
 _D = ct.POINTER(ct.c_double)
 _F = ct.POINTER(ct.c_float)
@@ -273,8 +275,8 @@ class ProjectionSolver:
         self.lib.ntust_prof_reset()
 
     def set_sweep_variant(self, variant: int):
-        """SWEEP_TWO_PASS (default), SWEEP_GRAPH or SWEEP_FUSED; the iterates are
-        bit-identical."""
+        """SWEEP_TWO_PASS (default), SWEEP_GRAPH, SWEEP_FUSED or SWEEP_TILE; the
+        iterates are bit-identical."""
         _check(self.lib.ntust_set_sweep_variant(int(variant)))
 
     def prof_kernel(self):

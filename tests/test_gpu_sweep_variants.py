@@ -11,7 +11,7 @@ import pytest
 from conftest import rand_halo, stretched_grid
 from ntustcfdlab_amd.grid import Grid, bc_codes, noise_ic
 from ntustcfdlab_amd.hip import (QUICK_CENTRE, SOR_MIXED, SOR_REDBLACK, SWEEP_FUSED,
-                                 SWEEP_GRAPH, SWEEP_TWO_PASS, ProjectionSolver)
+                                 SWEEP_GRAPH, SWEEP_TILE, SWEEP_TWO_PASS, ProjectionSolver)
 
 pytestmark = pytest.mark.gpu
 
@@ -19,7 +19,7 @@ pytestmark = pytest.mark.gpu
 @pytest.mark.parametrize("shape", [(40, 36, 44), (33, 9, 17), (64, 16, 8), (7, 5, 3), (70, 18, 90)])
 @pytest.mark.parametrize("mode", [SOR_REDBLACK, SOR_MIXED])
 @pytest.mark.parametrize("itmax,zeta", [(1.0, 0.0), (2.0, 0.0), (65.0, 0.0), (400.0, 1e-7)])
-@pytest.mark.parametrize("variant", [SWEEP_FUSED, SWEEP_GRAPH])
+@pytest.mark.parametrize("variant", [SWEEP_FUSED, SWEEP_GRAPH, SWEEP_TILE])
 def test_variant_equals_two_pass(shape, mode, itmax, zeta, variant, hip_lib):
     g = stretched_grid(*shape)
     s = ProjectionSolver(g)
@@ -44,7 +44,7 @@ def test_variant_equals_two_pass(shape, mode, itmax, zeta, variant, hip_lib):
 
 
 @pytest.mark.parametrize("mode", [SOR_REDBLACK, SOR_MIXED])
-@pytest.mark.parametrize("variant", [SWEEP_FUSED, SWEEP_GRAPH])
+@pytest.mark.parametrize("variant", [SWEEP_FUSED, SWEEP_GRAPH, SWEEP_TILE])
 def test_resident_steps_equal_two_pass(mode, variant, hip_lib):
     g = Grid(48, 40, 36)
     u0, v0, w0 = noise_ic(g, 1e-3, seed=12)
