@@ -17,11 +17,16 @@
 
 namespace ntust {
 
+// Branch-free: the in-array candidate is loaded from a clamped index and
+// selected, so all loads of a cell issue together (f >= -nxy always: the
+// most negative offset is the -z neighbour on the k = 1 plane).
 __device__ __forceinline__ double flat_at(const double *phi, const double *pre, long nxy,
                                           long f)
 {
-    if (f >= 0) return phi[f];
-    return pre ? pre[nxy + f] : 0.0;
+    const double a = phi[f >= 0 ? f : 0];
+    if (!pre) return f >= 0 ? a : 0.0;
+    const double b = pre[nxy + (f < 0 ? f : -1)];
+    return f >= 0 ? a : b;
 }
 
 // Gauss gradient of the flat view at cell (i,j,k); reference order of
@@ -52,9 +57,11 @@ k_gradient_flat(KGrid g, const double *__restrict__ phi, const double *__restric
                 const double *__restrict__ Dym, const double *__restrict__ Dzm, int k0,
                 int k1)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int k = blockIdx.z + k0;
+    int bx, by, bz;
+    xcd_band(bx, by, bz);
+    const int i = bx * blockDim.x + threadIdx.x + 1;
+    const int j = by * blockDim.y + threadIdx.y + 1;
+    const int k = bz + k0;
     if (i > g.nx || j > g.ny || k > k1) return;
     const long nxy = (long)g.nx * g.ny;
     const long f = iix(g, i, j, k);
@@ -77,12 +84,15 @@ k_smagorinsky(KGrid g, const double *__restrict__ u, const double *__restrict__ 
               double *__restrict__ dudx, double *__restrict__ dvdx,
               double *__restrict__ dwdx, int k0, int k1)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int k = blockIdx.z + k0;
+    int bx, by, bz;
+    xcd_band(bx, by, bz);
+    const int i = bx * blockDim.x + threadIdx.x + 1;
+    const int j = by * blockDim.y + threadIdx.y + 1;
+    const int k = bz + k0;
     if (i > g.nx || j > g.ny || k > k1) return;
     const long nxy = (long)g.nx * g.ny;
     const long f = iix(g, i, j, k);
+    const double fl = filt[f];   // issued with the gradient loads
     double ux, uy, uz, vx, vy, vz, wx, wy, wz;
     gauss_grad(u, pu, g.nx, nxy, f, g.Dxs[i], g.iDy[j], g.iDz[k], ux, uy, uz);
     gauss_grad(v, pv, g.nx, nxy, f, g.iDx[i], g.Dys[j], g.iDz[k], vx, vy, vz);
@@ -96,7 +106,7 @@ k_smagorinsky(KGrid g, const double *__restrict__ u, const double *__restrict__ 
     const double m = fabs(2.0 * (ux * ux) + 2.0 * (vy * vy) + 2.0 * (wz * wz) +
                           (uy + vx) * (uy + vx) + (uz + wx) * (uz + wx) +
                           (vz + wy) * (vz + wy));
-    visc[f] = filt[f] * sqrt(m);
+    visc[f] = fl * sqrt(m);
 }
 
 void launch_gradient_flat(const KGrid &g, const double *phi, const double *prebase,

@@ -116,17 +116,20 @@ __device__ __forceinline__ double quick_component(const KGrid &g, const double *
              + cnu * dt * (l2) / c2;
 }
 
-// blockIdx.z enumerates (component, k)
+// blockIdx.z enumerates (k, component), the component fastest, so the three
+// components of a plane run together and share its u, v, w in L2
 __global__ void __launch_bounds__(256)
 k_predict_quick(KGrid g, const double *__restrict__ u, const double *__restrict__ v,
                 const double *__restrict__ w, const double *__restrict__ visc, double dt,
                 double nu, double les, double *__restrict__ us, double *__restrict__ vs,
                 double *__restrict__ ws, int k0, int nk)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int comp = blockIdx.z / nk;
-    const int k = blockIdx.z - comp * nk + k0;
+    int bx, by, bz;
+    xcd_band(bx, by, bz);
+    const int i = bx * blockDim.x + threadIdx.x + 1;
+    const int j = by * blockDim.y + threadIdx.y + 1;
+    const int comp = bz % 3;
+    const int k = bz / 3 + k0;
     if (i > g.nx || j > g.ny) return;
     const double nut = visc ? visc[iix(g, i, j, k)] : 0.0;
     const double cnu = nu + nut * les;
@@ -194,10 +197,12 @@ k_predict_upwind(KGrid g, const double *__restrict__ u, const double *__restrict
                  double *__restrict__ us, double *__restrict__ vs, double *__restrict__ ws,
                  int wall_copy)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int comp = blockIdx.z / g.nz;
-    const int k = blockIdx.z - comp * g.nz + 1;
+    int bx, by, bz;
+    xcd_band(bx, by, bz);
+    const int i = bx * blockDim.x + threadIdx.x + 1;
+    const int j = by * blockDim.y + threadIdx.y + 1;
+    const int comp = bz % 3;
+    const int k = bz / 3 + 1;
     if (i > g.nx || j > g.ny) return;
     const long o = hix(g, i, j, k);
     if (comp == 0) {

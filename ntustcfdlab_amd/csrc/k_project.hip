@@ -42,9 +42,11 @@ k_project(KGrid g, Proj q, double *u1, double *v1, double *w1, double *u2, doubl
           double *uo, double *vo, double *wo, const double *u2g, const double *v2g,
           const double *w2g, int k0, int k1)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int k = blockIdx.z + k0;
+    int bx, by, bz;
+    xcd_band(bx, by, bz);
+    const int i = bx * blockDim.x + threadIdx.x + 1;
+    const int j = by * blockDim.y + threadIdx.y + 1;
+    const int k = bz + k0;
     if (i > g.nx || j > g.ny || k > k1) return;
     const long o = hix(g, i, j, k);
     const double dt = q.dt;

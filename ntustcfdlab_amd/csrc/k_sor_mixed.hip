@@ -14,9 +14,9 @@
 // per-sweep gating as the FP64 kernels.
 //
 // Traffic: an FP32 half-sweep streams one colour of e and r and reads the
-// other colour of e: 8 B per cell of the grid (16 per sweep, FP64: 32); a
-// correction reads p, e and writes p, e (24 B/cell) and the residual reads
-// p, b and writes r (20 B/cell).
+// other colour of e: 8 B per cell of the grid (16 per sweep, FP64: 32); the
+// correction of a segment (absorb and residual, k_rb_correct) reads p, e, b
+// and writes p, e, r: 24 + 22 B/cell over its two colour launches.
 #include "ntust_kernels.h"
 
 namespace ntust {
@@ -132,55 +132,127 @@ k_rb_sweep32(KGrid g, SplitGrid s, FMetric fm, float *__restrict__ e_own,
     }
 }
 
-// p0 += e on the interior, e = 0; skipped when no sweep of the segment ran
-// (dirty NULL: always)
+// The FP64 correction of a segment in colour-split form, two launches
+// (colour 0, then colour 1): each cell absorbs its correction, p = p + e,
+// e = 0, and gets its residual
+//   r = (sum_nb a_nb p_nb - b) - aP p
+// from the absorbed values of its neighbours, which are all of the other
+// colour.  During the colour-0 launch the colour-1 cells are not yet
+// absorbed, so their value is formed on the fly as p + e (the same
+// operation the absorb performs, so the same bits); during the colour-1
+// launch the colour-0 cells are already absorbed and read as they are.
+// The result is bit for bit that of an absorb over the grid followed by the
+// residual.  ABSORB = false: the residual alone (p and e untouched; e may be
+// null).  Thread layout of the sweeps (RB_KPT cells along k, other
+// colour's W/E rows shared for the +-k neighbours, XCD-slab block order).
+// Traffic per cell of the grid: 24 B (colour 0) + 22 B (colour 1) against
+// 24 + 28 for the two whole-grid kernels.
+template <int C, bool ABSORB>
 __global__ void __launch_bounds__(256)
-k_rb_absorb(KGrid g, SplitGrid s, double *__restrict__ p0, double *__restrict__ p1,
-            float *__restrict__ e0, float *__restrict__ e1, const int *dirty)
+k_rb_correct(KGrid g, SplitGrid s, const double *__restrict__ rx, const double *__restrict__ ry,
+             const double *__restrict__ rz, double *__restrict__ p_own,
+             const double *__restrict__ p_oth, float *__restrict__ e_own,
+             const float *__restrict__ e_oth, const double *__restrict__ b_own,
+             float *__restrict__ r_own, const int *dirty)
 {
+    constexpr int KPT = RB_KPT;
     if (dirty && !load_flag(dirty)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
-    if (i > g.nx || j > g.ny) return;
-    const long o = six(s, i, j, k);
-    if ((i + j + k) & 1) {
-        p1[o] = p1[o] + (double)e1[o];
-        e1[o] = 0.0f;
-    } else {
-        p0[o] = p0[o] + (double)e0[o];
-        e0[o] = 0.0f;
+    int bx, by, bz;
+    xcd_block(bx, by, bz);
+    const int q = bx * blockDim.x + threadIdx.x;
+    const int jr = by * blockDim.y + threadIdx.y + 1;
+    const int j = min(jr, g.ny);
+    const int k0 = bz * KPT + 1;
+    const int sh0 = (C + 1 + j + k0) & 1;
+    const int ia = min(2 * q + 1 + sh0, g.nx + 1), ib = min(2 * q + 2 - sh0, g.nx + 1);
+    const double axa = g.iDx[ia], rEa = rx[ia], rWa = rx[ia - 1];
+    const double axb = g.iDx[ib], rEb = rx[ib], rWb = rx[ib - 1];
+    const double ay = g.iDy[j], rN = ry[j], rS = ry[j - 1];
+    const unsigned mp = (unsigned)s.mp;
+    const unsigned q1 = (unsigned)min(q, (g.nx + 1) / 2);
+    // Other-colour values: during the colour-0 launch they are formed as
+    // p + e, but only for interior cells -- the absorb touches interior
+    // cells only, and a ghost entry (whose e is zero) is read as it is,
+    // since p + 0 would turn a -0 into +0.  Colour c of pair x in row
+    // (jj, kk) is cell i = 2x + (((jj+kk)&1)^c).  All loads are issued
+    // first and unconditionally; the interior selection follows, so the
+    // e loads are not serialised behind per-load branches.
+    auto interior = [&](int x, int jj, int kk) -> bool {
+        const int i = 2 * x + (((jj + kk) & 1) ^ 1);
+        return jj >= 1 && jj <= g.ny && kk >= 1 && kk <= g.nz && i >= 1 && i <= g.nx;
+    };
+    double vW[KPT + 2], vE[KPT + 2], vN[KPT], vS[KPT], vb[KPT], vp[KPT];
+    float eW[KPT + 2], eE[KPT + 2], eN[KPT], eS[KPT], ve[KPT];
+    unsigned at[KPT];
+#pragma unroll
+    for (int r = -1; r <= KPT; r++) {
+        const int k = min(k0 + r, g.nz + 1);
+        const unsigned row = mp * ((unsigned)j + (unsigned)s.ry * (unsigned)k);
+        vW[r + 1] = p_oth[row + q1];
+        vE[r + 1] = p_oth[row + q1 + 1];
+        if (ABSORB && C == 0) {
+            eW[r + 1] = e_oth[row + q1];
+            eE[r + 1] = e_oth[row + q1 + 1];
+        }
     }
-}
-
-// r = sum_nb a_nb p_nb - b - aP p with the FP64 coefficients and summation
-// of the FP64 red-black kernel, rounded to FP32
-__global__ void __launch_bounds__(256)
-k_rb_residual(KGrid g, SplitGrid s, const double *__restrict__ rx,
-              const double *__restrict__ ry, const double *__restrict__ rz,
-              const double *__restrict__ p0, const double *__restrict__ p1,
-              const double *__restrict__ b0, const double *__restrict__ b1,
-              float *__restrict__ r0, float *__restrict__ r1, const int *dirty)
-{
-    if (dirty && !load_flag(dirty)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
-    if (i > g.nx || j > g.ny) return;
-    const int c = (i + j + k) & 1;
-    const double *own = c ? p1 : p0, *oth = c ? p0 : p1;
-    const long o = six(s, i, j, k);
-    const long pl = (long)s.mp * s.ry;
-    const double ax = g.iDx[i], ay = g.iDy[j], az = g.iDz[k];
-    const double ayz = ay * az, axz = ax * az, axy = ax * ay;
-    const double rE = rx[i], rW = rx[i - 1], rN = ry[j], rS = ry[j - 1], rF = rz[k], rB = rz[k - 1];
-    const double t1 = fma(rE, oth[six(s, i + 1, j, k)], rW * oth[six(s, i - 1, j, k)]);
-    const double t2 = fma(rN, oth[o + s.mp], rS * oth[o - s.mp]);
-    const double t3 = fma(rF, oth[o + pl], rB * oth[o - pl]);
-    const double sum = fma(ayz, t1, fma(axz, t2, axy * t3));
-    const double aP = fma(ayz, rE + rW, fma(axz, rN + rS, axy * (rF + rB)));
-    const double r = (sum - (c ? b1 : b0)[o]) - aP * own[o];
-    (c ? r1 : r0)[o] = (float)r;
+#pragma unroll
+    for (int r = 0; r < KPT; r++) {
+        const int k = min(k0 + r, g.nz);
+        const unsigned row = mp * ((unsigned)j + (unsigned)s.ry * (unsigned)k);
+        const unsigned o = row + q1 + (unsigned)(sh0 ^ (r & 1));
+        at[r] = o;
+        vN[r] = p_oth[o + mp];
+        vS[r] = p_oth[o - mp];
+        if (ABSORB && C == 0) {
+            eN[r] = e_oth[o + mp];
+            eS[r] = e_oth[o - mp];
+        }
+        vb[r] = b_own[o];
+        vp[r] = p_own[o];
+        ve[r] = ABSORB ? e_own[o] : 0.0f;
+    }
+    if (ABSORB && C == 0) {
+#pragma unroll
+        for (int r = -1; r <= KPT; r++) {
+            const int k = min(k0 + r, g.nz + 1);
+            const double w = vW[r + 1] + (double)eW[r + 1], e = vE[r + 1] + (double)eE[r + 1];
+            vW[r + 1] = interior((int)q1, j, k) ? w : vW[r + 1];
+            vE[r + 1] = interior((int)q1 + 1, j, k) ? e : vE[r + 1];
+        }
+#pragma unroll
+        for (int r = 0; r < KPT; r++) {
+            const int k = min(k0 + r, g.nz);
+            const int x = (int)q1 + (sh0 ^ (r & 1));
+            const double n = vN[r] + (double)eN[r], sv = vS[r] + (double)eS[r];
+            vN[r] = interior(x, j + 1, k) ? n : vN[r];
+            vS[r] = interior(x, j - 1, k) ? sv : vS[r];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < KPT; r++) {
+        const int k = min(k0 + r, g.nz);
+        const bool odd = r & 1;
+        const bool sh = sh0 ^ (int)odd;
+        const double ax = odd ? axb : axa, rE = odd ? rEb : rEa, rW = odd ? rWb : rWa;
+        const double az = g.iDz[k], rF = rz[k], rB = rz[k - 1];
+        const double ayz = ay * az, axz = ax * az, axy = ax * ay;
+        const double pF = sh ? vE[r + 2] : vW[r + 2], pB = sh ? vE[r] : vW[r];
+        const double own = ABSORB ? vp[r] + (double)ve[r] : vp[r];
+        const double t1 = fma(rE, vE[r + 1], rW * vW[r + 1]);
+        const double t2 = fma(rN, vN[r], rS * vS[r]);
+        const double t3 = fma(rF, pF, rB * pB);
+        const double sum = fma(ayz, t1, fma(axz, t2, axy * t3));
+        const double aP = fma(ayz, rE + rW, fma(axz, rN + rS, axy * (rF + rB)));
+        const double res = (sum - vb[r]) - aP * own;
+        const int i = 2 * q + 1 + (int)sh;
+        if (jr <= g.ny && k0 + r <= g.nz && i <= g.nx) {
+            if (ABSORB) {
+                p_own[at[r]] = own;
+                e_own[at[r]] = 0.0f;
+            }
+            r_own[at[r]] = (float)res;
+        }
+    }
 }
 
 void launch_rb_sweep32(const KGrid &g, const SplitGrid &s, const FMetric &fm, float *e0,
@@ -208,21 +280,33 @@ void launch_rb_sweep32_spec(const KGrid &g, const SplitGrid &s, const FMetric &f
                        0.0, t, slots, nullptr, nullptr, 0);
 }
 
-void launch_rb_absorb(const KGrid &g, const SplitGrid &s, double *p0, double *p1, float *e0,
-                      float *e1, const int *dirty, hipStream_t st)
-{
-    dim3 blk(64, 4), grd((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz);
-    hipLaunchKernelGGL(k_rb_absorb, grd, blk, 0, st, g, s, p0, p1, e0, e1, dirty);
-}
-
 void launch_rb_residual(const KGrid &g, const SplitGrid &s, const double *rx, const double *ry,
                         const double *rz, const double *p0, const double *p1, const double *b0,
                         const double *b1, float *r0, float *r1, const int *dirty,
                         hipStream_t st)
 {
-    dim3 blk(64, 4), grd((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz);
-    hipLaunchKernelGGL(k_rb_residual, grd, blk, 0, st, g, s, rx, ry, rz, p0, p1, b0, b1, r0, r1,
-                       dirty);
+    const int nq = (g.nx + 1) / 2;
+    dim3 blk(64, 4), grd((nq + 63) / 64, (g.ny + 3) / 4, (g.nz + RB_KPT - 1) / RB_KPT);
+    // p is only read when ABSORB is false
+    double *q0 = const_cast<double *>(p0), *q1 = const_cast<double *>(p1);
+    hipLaunchKernelGGL((k_rb_correct<0, false>), grd, blk, 0, st, g, s, rx, ry, rz, q0, p1,
+                       nullptr, nullptr, b0, r0, dirty);
+    hipLaunchKernelGGL((k_rb_correct<1, false>), grd, blk, 0, st, g, s, rx, ry, rz, q1, p0,
+                       nullptr, nullptr, b1, r1, dirty);
+}
+
+// absorb + residual of a segment (k_rb_correct, both colours)
+void launch_rb_correct(const KGrid &g, const SplitGrid &s, const double *rx, const double *ry,
+                       const double *rz, double *p0, double *p1, float *e0, float *e1,
+                       const double *b0, const double *b1, float *r0, float *r1,
+                       const int *dirty, hipStream_t st)
+{
+    const int nq = (g.nx + 1) / 2;
+    dim3 blk(64, 4), grd((nq + 63) / 64, (g.ny + 3) / 4, (g.nz + RB_KPT - 1) / RB_KPT);
+    hipLaunchKernelGGL((k_rb_correct<0, true>), grd, blk, 0, st, g, s, rx, ry, rz, p0, p1, e0, e1,
+                       b0, r0, dirty);
+    hipLaunchKernelGGL((k_rb_correct<1, true>), grd, blk, 0, st, g, s, rx, ry, rz, p1, p0, e1, e0,
+                       b1, r1, dirty);
 }
 
 }  // namespace ntust

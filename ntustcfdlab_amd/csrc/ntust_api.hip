@@ -342,9 +342,8 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
                         launch_rb_sweep32_spec(g, C.sg, fm, E0, E1, R0, R1, (float)omega,
                                                (int)(tt + b), slots, C.st);
                     if (ev) (void)hipEventRecord(C.ev1, C.st);
-                    launch_rb_absorb(g, C.sg, P0, P1, E0, E1, nullptr, C.st);
-                    launch_rb_residual(g, C.sg, rx, ry, rz, P0, P1, B0, B1, R0, R1, nullptr,
-                                       C.st);
+                    launch_rb_correct(g, C.sg, rx, ry, rz, P0, P1, E0, E1, B0, B1, R0, R1,
+                                      nullptr, C.st);
                 }
             };
             launch_slots_clear(slots, 1, (int)nb1 + 1, C.st);
@@ -390,9 +389,8 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
                     if (r == 0) HIPCHK(hipEventRecord(C.ev0, C.st));
                     HIPCHK(hipGraphLaunch(C.graph32.exec, C.st));
                     if (r == 0) HIPCHK(hipEventRecord(C.ev1, C.st));
-                    launch_rb_absorb(g, C.sg, P0, P1, E0, E1, dirty, C.st);
-                    launch_rb_residual(g, C.sg, rx, ry, rz, P0, P1, B0, B1, R0, R1, dirty,
-                                       C.st);
+                    launch_rb_correct(g, C.sg, rx, ry, rz, P0, P1, E0, E1, B0, B1, R0, R1,
+                                      dirty, C.st);
                 }
                 HIPCHK(hipGetLastError());
                 HIPCHK(hipMemcpyAsync(&h, C.ctl, sizeof h, hipMemcpyDeviceToHost, C.st));
@@ -430,11 +428,14 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
                     }
                 }
                 if (!timed) HIPCHK(hipEventRecord(C.ev1, C.st));
-                if (fused)
+                if (fused) {
                     launch_rb_absorb_pp(g, C.sg, P0, P1, E0, E1, F0, F1, dirty + c, C.ctl, C.st);
-                else
-                    launch_rb_absorb(g, C.sg, P0, P1, E0, E1, dirty + c, C.st);
-                launch_rb_residual(g, C.sg, rx, ry, rz, P0, P1, B0, B1, R0, R1, dirty + c, C.st);
+                    launch_rb_residual(g, C.sg, rx, ry, rz, P0, P1, B0, B1, R0, R1, dirty + c,
+                                       C.st);
+                } else {
+                    launch_rb_correct(g, C.sg, rx, ry, rz, P0, P1, E0, E1, B0, B1, R0, R1,
+                                      dirty + c, C.st);
+                }
                 HIPCHK(hipGetLastError());
                 if (!timed) { timed = true; batch = n; }
                 t += n;

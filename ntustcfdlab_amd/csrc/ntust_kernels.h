@@ -74,7 +74,7 @@ __device__ __forceinline__ double wave_max(double v)
     return v;
 }
 
-// XCD-aware block order for the stencil sweeps.  The hardware hands the
+// XCD-aware block order for the stencil kernels.  The hardware hands the
 // workgroups of a launch to the 8 XCDs round-robin (workgroup n to XCD
 // n mod 8, in order), and each XCD has its own L2.  Renumbering so that the
 // workgroups of XCD x take a contiguous range of logical blocks (x fastest,
@@ -91,6 +91,25 @@ __device__ __forceinline__ void xcd_block(int &bx, int &by, int &bz)
     const int r = logical / gridDim.x;
     by = r % gridDim.y;
     bz = r / gridDim.y;
+}
+
+// XCD y-band order for kernels whose blocks each cover one plane of one or
+// more full-size arrays (a slab of planes per XCD would not fit its 4 MB
+// L2).  Within each z-layer of the launch the blocks are renumbered so that
+// XCD x takes a contiguous band of them (x fastest, then y); all XCDs
+// advance through z together, so an XCD's +-y, +-x and +-z neighbours stay
+// in its own L2 while its working set is a band of a few planes.  The
+// z-layer keeps its index.  A bijection for any grid; the band lands on one
+// XCD in every layer when gridDim.x*gridDim.y is a multiple of 8.
+__device__ __forceinline__ void xcd_band(int &bx, int &by, int &bz)
+{
+    const int nb = gridDim.x * gridDim.y;
+    const int m = blockIdx.x + gridDim.x * blockIdx.y;
+    const int x = m & 7, idx = m >> 3, per = nb >> 3, rem = nb & 7;
+    const int logical = x * per + (x < rem ? x : rem) + idx;
+    bx = logical % gridDim.x;
+    by = logical / gridDim.x;
+    bz = blockIdx.z;
 }
 
 // Boundary-condition codes, one per (face, component) in the order
@@ -128,8 +147,10 @@ __host__ __device__ inline long slot_index(long t, int b)
 {
     return (t * SLOT_BUCKETS + b) * SLOT_STRIDE;
 }
-// cells per thread along k in the red-black sweep kernels
+// cells per thread along k in the red-black sweep kernels: FP32 and the
+// mixed solver's kernels, and the FP64 sweep
 constexpr int RB_KPT = 4;
+constexpr int RB_KPT64 = 8;
 // FP32 copies of the metrics used by the FP32 sweeps (offset by +1 like KGrid)
 struct FMetric {
     const float *iDx, *iDy, *iDz, *rx, *ry, *rz;
@@ -211,8 +232,6 @@ void launch_rb_sweep32_spec(const KGrid &g, const SplitGrid &s, const FMetric &f
                             double *slots, hipStream_t st);
 void launch_slots_clear(double *slots, int t0, int n, hipStream_t st);
 void launch_slots_max(double *slots, int t0, int n, double *m, hipStream_t st);
-void launch_rb_absorb(const KGrid &g, const SplitGrid &s, double *p0, double *p1, float *e0,
-                      float *e1, const int *dirty, hipStream_t st);
 // one launch per sweep (red then black on LDS tiles), reading copy "in" of
 // the colour planes and writing copy "out"
 void launch_rb_sweep_fused(const KGrid &g, const SplitGrid &s, const double *rx,
@@ -232,6 +251,11 @@ void launch_rb_residual(const KGrid &g, const SplitGrid &s, const double *rx, co
                         const double *rz, const double *p0, const double *p1, const double *b0,
                         const double *b1, float *r0, float *r1, const int *dirty,
                         hipStream_t st);
+// absorb followed by the residual, in two colour launches (same bits)
+void launch_rb_correct(const KGrid &g, const SplitGrid &s, const double *rx, const double *ry,
+                       const double *rz, double *p0, double *p1, float *e0, float *e1,
+                       const double *b0, const double *b1, float *r0, float *r1,
+                       const int *dirty, hipStream_t st);
 
 // projection, direct forcing, centring, update
 void launch_project(const KGrid &g, const double *us, const double *vs, const double *ws,
