@@ -180,8 +180,8 @@ def airfoil_leg(hip, calls, cpu):
     from ntustcfdlab_amd.airfoil import naca4_polygon
     from ntustcfdlab_amd.grid import Grid, tanh_nodes
     nx, ny, nz = 512, 384, 128
-    g = Grid(nx, ny, nz, ly=8.0, lz=8.0, Y=tanh_nodes(ny, 8.0, 0.5, 2.5),
-             Z=tanh_nodes(nz, 8.0, 0.5, 2.5))
+    g = Grid(nx, ny, nz, ly=8.0, lz=8.0, Y=tanh_nodes(ny, 8.0, 4.0, 5.0),
+             Z=tanh_nodes(nz, 8.0, 4.0, 5.0))
     az0, ay0 = naca4_polygon(poly=5000)
     s = hip.ProjectionSolver(g)
     try:
@@ -218,46 +218,54 @@ def airfoil_leg(hip, calls, cpu):
     return out
 
 
-def airfoil_les_leg(hip, hiptime, steps, warmup):
-    """The large airfoil configuration, forward: NACA 0012 (5000 vertices) at
-    8 degrees, Re = 5000, QUICK, Smagorinsky LES Cs = 0.1, dt = 5e-4, on
-    512 x 384 x 128 cells (span x, normal y, chord z; y and z tanh-clustered
-    at the airfoil, domain 1 x 8 x 8), ETA cast on the device, SOR omega 1.5,
-    zeta 1e-10, itmax 3000.  Uniform inflow w = 1 through the back face,
+def airfoil_case(hip):
+    """The large airfoil configuration: NACA 0012 (5000 vertices) on 512 x 384
+    x 128 cells (span x, normal y, chord z; y and z clustered at the airfoil
+    with Roberts' stretching, finest spacing 0.0086 / 0.026 there, domain 1 x
+    8 x 8), Re = 5000, QUICK, Smagorinsky LES Cs = 0.1, dt = 1e-4, SOR omega
+    1.5, zeta 1e-10, itmax 3000.  Uniform inflow w = 1 through the back face,
     Neumann outflow at the front, Neumann tangential components on the other
-    faces (this leg's own choice: the reference ships only the cavity's
-    boundary conditions).  Both solvers from the same state; C_D, C_L of the
-    last step from the force integral."""
+    faces (this configuration's own choice: the reference ships only the
+    cavity's boundary conditions).  Returns the grid, the boundary codes,
+    the polygon and a configure(s, sor) function."""
     from ntustcfdlab_amd.airfoil import naca4_polygon
     from ntustcfdlab_amd.grid import Grid, bc_codes, tanh_nodes
     nx, ny, nz = 512, 384, 128
-    g = Grid(nx, ny, nz, lx=1.0, ly=8.0, lz=8.0, Y=tanh_nodes(ny, 8.0, 0.5, 2.5),
-             Z=tanh_nodes(nz, 8.0, 0.5, 2.5))
+    g = Grid(nx, ny, nz, lx=1.0, ly=8.0, lz=8.0, Y=tanh_nodes(ny, 8.0, 4.0, 5.0),
+             Z=tanh_nodes(nz, 8.0, 4.0, 5.0))
     bc = bc_codes(BackWall_w="Dirichlet", FrontWall_u="Neumann", FrontWall_v="Neumann",
                   FrontWall_w="Neumann", NorthWall_u="Neumann", NorthWall_w="Neumann",
                   SouthWall_u="Neumann", SouthWall_w="Neumann", WestWall_v="Neumann",
                   WestWall_w="Neumann", EastWall_v="Neumann", EastWall_w="Neumann")
-    s = hip.ProjectionSolver(g, bc)
+    az0, ay0 = naca4_polygon(poly=5000)
 
-    def configure_sor(sor):
-        s.configure(dt=5e-4, nu=1.0 / 5000.0, Cs=0.1, LES=1, scheme=hip.QUICK_CENTRE, omega=1.5,
-                    zeta=1e-10, itmax=3000.0, sor=sor, lx=1.0, force_every=1)
-    s.configure_sor = configure_sor
-    configure_sor(hip.SOR_REDBLACK)
+    def configure(s, sor, force_every=1):
+        s.configure(dt=1e-4, nu=1.0 / 5000.0, Cs=0.1, LES=1, scheme=hip.QUICK_CENTRE, omega=1.5,
+                    zeta=1e-10, itmax=3000.0, sor=sor, lx=1.0, force_every=force_every)
+    return g, bc, az0, ay0, configure
+
+
+def airfoil_les_leg(hip, hiptime, steps, warmup):
+    """The large airfoil configuration (airfoil_case) at 8 degrees, forward:
+    ETA cast on the device, both solvers from the same inflow state; C_D,
+    C_L of the last step from the force integral."""
+    g, bc, az0, ay0, configure = airfoil_case(hip)
+    s = hip.ProjectionSolver(g, bc)
+    configure(s, hip.SOR_REDBLACK)
     u, v, w, p = g.halo(), g.halo(), g.halo(), g.halo()
     w[...] = 1.0
     s.boundary_conditions(u, v, w, p)
     eta = g.halo()
-    az0, ay0 = naca4_polygon(poly=5000)
     s.ray_casting(az0, ay0, 8.0, 4.0, 4.0, ETA=eta)
     state = (u, v, w, p, eta)
-    out = {"workload": "NACA 0012 at 8 deg, Re 5000, LES Cs 0.1, 512x384x128 tanh-clustered, "
-                       "inflow w = 1 along z", "steps": steps, "warmup": warmup, "solvers": {}}
+    out = {"workload": "NACA 0012 at 8 deg, Re 5000, LES Cs 0.1, dt 1e-4, 512x384x128 clustered "
+                       "at the airfoil, inflow w = 1 along z", "steps": steps, "warmup": warmup,
+           "solvers": {}}
     finals = {}
     try:
         for name, sor in (("fp64", hip.SOR_REDBLACK), ("mixed", hip.SOR_MIXED)):
             print(f"# airfoil LES {name}", file=sys.stderr, flush=True)
-            s.configure_sor(sor)
+            configure(s, sor)
             s.upload(*state)
             if warmup:
                 s.step(warmup)
@@ -285,6 +293,150 @@ def airfoil_les_leg(hip, hiptime, steps, warmup):
                                          out["solvers"]["fp64"]["value"], 3)
     out["solid_cells"] = float(eta[2:-2, 2:-2, 2:-2].sum())
     return out
+
+
+def actuator_strip(g, aoa, seed):
+    """The actuator control of the large configuration: a seeded strip 0.05
+    chord wide at x/c = 0.1, next to the surface (normal offset within 0.1
+    chord of the chord line), uniform along the span like the extruded
+    airfoil, pushing along the chord.  Chord frame of the cast airfoil
+    (quarter chord at (z, y) = (4, 4), rotated by -AOA): s = (z-4) cos a -
+    (y-4) sin a + 0.25 from the leading edge.  Returns (fu, fv, fw) halo
+    arrays with seeded U(0, 1) amplitudes.  Uniform along x keeps the flow
+    x-invariant, so u stays exactly 0: an x-dependent field would make it
+    non-zero with either sign, and the QUICK upwind choice at a zero face
+    velocity (discretisation_QUICK_centre.f90:18-102, 0 takes the else
+    branch) puts a kink in the map there that a central difference
+    straddles."""
+    _, ys, zs = g.centres()
+    a = np.deg2rad(aoa)
+    Z, Y = np.meshgrid(zs, ys, indexing="xy")          # (ny, nz)
+    sc = (Z - 4.0) * np.cos(a) - (Y - 4.0) * np.sin(a) + 0.25
+    nr = (Z - 4.0) * np.sin(a) + (Y - 4.0) * np.cos(a)
+    mask = (sc >= 0.075) & (sc <= 0.125) & (np.abs(nr) <= 0.1)
+    amp = np.random.default_rng(seed).uniform(0.0, 1.0, (g.ny, g.nz)) * mask
+    amp = np.broadcast_to(amp[None, :, :], (g.nx, g.ny, g.nz))
+    fu, fv, fw = g.halo(), g.halo(), g.halo()
+    fv[2:-2, 2:-2, 2:-2] = -np.sin(a) * amp
+    fw[2:-2, 2:-2, 2:-2] = np.cos(a) * amp
+    return fu, fv, fw, int(mask.sum()) * g.nx
+
+
+def airfoil_adjoint_leg(hip, hiptime, steps, every, navg):
+    """The adjoint of the large configuration (airfoil_case, FP32 sweeps +
+    FP64 refine): J = mean C_D over the last navg of `steps` steps from the
+    state two steps after the inflow, checkpoints every `every` steps.
+    Outputs: the cost of the checkpointed reverse pass against a plain
+    forward of the same steps; dJ/df for the actuator strip against the
+    central difference of the forward along it (the FP32-sweep one, step for
+    a 1e-4 relative change of J, and the FP64 one, step for 1e-6); dJ/dAOA by
+    the chain rule through the cast ETA (ETA at AOA +- 0.01 degrees: the
+    surface moves a small fraction of a cell) against the central difference
+    of the FP64 forward's J at those angles.  C_D is piecewise smooth in
+    AOA: the force integral skips samples that are exactly zero
+    (virtualForceIntegrator.f90:41-75), so a row that starts or stops
+    touching the body makes it jump; the chain rule gives the derivative
+    between such jumps."""
+    g, bc, az0, ay0, configure = airfoil_case(hip)
+    aoa, h_aoa = 8.0, 0.01
+    s = hip.ProjectionSolver(g, bc)
+    try:
+        configure(s, hip.SOR_MIXED)
+        u, v, w, p = g.halo(), g.halo(), g.halo(), g.halo()
+        w[...] = 1.0
+        s.boundary_conditions(u, v, w, p)
+
+        def cast(angle):
+            e = g.halo()
+            s.ray_casting(az0, ay0, angle, 4.0, 4.0, ETA=e)
+            return e
+
+        # the horizon starts from the state two steps after the inflow: on the
+        # uniform inflow itself b = 0, the first solve stops after one sweep
+        # (GS.f90:11 with max|dp| = 0), and any perturbation makes it run
+        # more -- a jump in the map that a central difference would straddle
+        s.upload(u, v, w, p, cast(aoa))
+        s.step(2)
+        state = [g.halo() for _ in range(4)]
+        s.download(*state)
+
+        zero = [g.halo() for _ in range(3)]
+        fu, fv, fw, ncells = actuator_strip(g, aoa, 2025)
+        d = [fu, fv, fw]
+
+        counts = []
+
+        def plain(eta, f):
+            s.set_body_force(*f)
+            s.upload(*[x.copy(order="F") for x in state], eta)
+            e0, e1 = hiptime.Event(), hiptime.Event()
+            hiptime.device_synchronize()
+            e0.record(s.stream())
+            tot = 0.0
+            ks = []
+            for k in range(steps):
+                ik1, pm, cd, cl, rc = s.step(1)
+                ks.append(int(ik1[0]))
+                if k >= steps - navg:
+                    tot += float(cd[0])
+            e1.record(s.stream())
+            counts.append(ks)
+            return tot * (1.0 / navg), e0.elapsed_ms(e1)
+
+        eta0 = cast(aoa)
+        J0, t_plain = plain(eta0, zero)
+        s.adjoint_objective(0.0, 1.0, 0.0, eta_gradient=True)
+        s.set_body_force(*zero)
+        s.upload(*[x.copy(order="F") for x in state], eta0)
+        J, ik, ms = s.adjoint_run(steps, every, navg)
+        gf = s.adjoint_gradient()
+        ge = s.adjoint_eta_gradient()
+        ad = float(sum(np.vdot(x, y) for x, y in zip(gf, d)))
+        # central difference of the forward the adjoint differentiates (the
+        # FP32-sweep one; step for a 1e-4 relative change of J, well above
+        # its FP32 rounding)
+        hm = 1e-4 * abs(J) / abs(ad) if ad != 0.0 else 1.0
+        jp, _ = plain(eta0, [hm * x for x in d])
+        jm, _ = plain(eta0, [-hm * x for x in d])
+        fd_m = (jp - jm) / (2 * hm)
+        # and of the FP64 forward (the reference's arithmetic)
+        configure(s, hip.SOR_REDBLACK)
+        c0 = len(counts)
+        J64, t64 = plain(eta0, zero)
+        hf = 1e-6 * abs(J64) / abs(ad) if ad != 0.0 else 1.0
+        jp, _ = plain(eta0, [hf * x for x in d])
+        jm, _ = plain(eta0, [-hf * x for x in d])
+        fd_f = (jp - jm) / (2 * hf)
+        ik64 = counts[c0]
+        ep, em = cast(aoa + h_aoa), cast(aoa - h_aoa)
+        chain = float(np.vdot(ge, (ep - em) / (2 * h_aoa)))
+        jap, _ = plain(ep, zero)
+        jam, _ = plain(em, zero)
+        fd_a = (jap - jam) / (2 * h_aoa)
+    finally:
+        s.close()
+    return {
+        "workload": (f"NACA 0012 at {aoa} deg, 512x384x128 clustered, Re 5000, LES Cs 0.1, dt "
+                     f"1e-4, FP32 sweeps + FP64 refine; J = mean C_D over the last {navg} of "
+                     f"{steps} steps from the state 2 steps after the inflow; checkpoint every "
+                     f"{every}"),
+        "J": J, "J_plain_forward": J0, "J_fp64_forward": J64,
+        "fp64_forward_ms": round(t64, 1), "sweeps_per_step": float(np.mean(ik)),
+        "forward_ms": round(float(ms[0]), 1), "plain_forward_ms": round(t_plain, 1),
+        "reverse_ms": round(float(ms[1]), 1), "recompute_ms": round(float(ms[2]), 1),
+        "adjoint_kernels_ms": round(float(ms[3]), 1),
+        "adjoint_over_forward": round(float(ms[1]) / t_plain, 3),
+        "fp64_sweeps_per_step": float(np.mean(ik64)),
+        "max_sweep_count_difference_fp64_vs_fp32": int(np.max(np.abs(np.array(ik64) - ik))),
+        "actuator": {"cells": ncells, "adjoint": ad,
+                     "fd_fp32_sweeps": {"step": hm, "fd": fd_m,
+                                        "rel_err": abs(fd_m - ad) / abs(ad) if ad else None},
+                     "fd_fp64": {"step": hf, "fd": fd_f,
+                                 "rel_err": abs(fd_f - ad) / abs(ad) if ad else None}},
+        "aoa": {"step_deg": h_aoa, "eta_cells_changed": int((ep != em).sum()),
+                "chain_rule_per_deg": chain, "fd_per_deg": fd_a,
+                "rel_err": abs(fd_a - chain) / abs(fd_a) if fd_a else None},
+    }
 
 
 def krylov_leg(hip, n):
@@ -443,6 +595,11 @@ def main():
                          "the z-streaming one-launch sweep, or the 3-D-tile one")
     ap.add_argument("--only-adjoint", action="store_true",
                     help="run the adjoint leg alone and print its JSON")
+    ap.add_argument("--airfoil-adjoint-steps", type=int, default=8,
+                    help="steps of the large airfoil adjoint leg (the configuration's "
+                         "horizon is 256; checkpoints every ~sqrt(steps))")
+    ap.add_argument("--only-airfoil-adjoint", action="store_true",
+                    help="run the large airfoil adjoint leg alone and print its JSON")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported: the path runs "
@@ -461,6 +618,12 @@ def main():
     if a.only_adjoint:
         print(json.dumps({"adjoint": adjoint_leg(hip, hiptime, a.adjoint_size,
                                                  a.adjoint_steps, a.fd_directions)}))
+        return
+    n_aa = max(a.airfoil_adjoint_steps, 2)
+    aa_args = (n_aa, max(1, int(round(np.sqrt(n_aa)))), max(1, n_aa // 2))
+    if a.only_airfoil_adjoint:
+        print(json.dumps({"airfoil_adjoint_512x384x128":
+                          airfoil_adjoint_leg(hip, hiptime, *aa_args)}))
         return
 
     head_run = measure(hip, hiptime, a.size, a.steps, a.warmup, a.sweep, a.only, a.calibrate)
@@ -498,6 +661,9 @@ def main():
         print("# airfoil geometry", file=sys.stderr, flush=True)
         out["airfoil_ray_casting"] = airfoil_leg(hip, 10, not a.no_cpu_baseline)
         out["airfoil_les_512x384x128"] = airfoil_les_leg(hip, hiptime, 2, 1)
+        if not a.no_adjoint:
+            print("# airfoil adjoint", file=sys.stderr, flush=True)
+            out["airfoil_adjoint_512x384x128"] = airfoil_adjoint_leg(hip, hiptime, *aa_args)
     if not a.no_adjoint:
         print("# adjoint", file=sys.stderr, flush=True)
         out["adjoint"] = adjoint_leg(hip, hiptime, a.adjoint_size, a.adjoint_steps,

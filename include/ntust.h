@@ -336,8 +336,31 @@ int ntust_adjoint_step(int *ik);
  * ntust_adjoint_run (halo arrays, interior meaningful; NULL skips). */
 int ntust_adjoint_gradient(double *gfu, double *gfv, double *gfw);
 
-/* Forward nsteps from the device state and back.  Objective: J = mean of
- * the kinetic energy after each of the last navg steps.  The forward pass
+/* Objective of ntust_adjoint_run (no counterpart in the reference):
+ *   J = mean over the last navg steps of
+ *       w_ke * KE + w_cd * C_D + w_cl * C_L
+ * with KE the kinetic energy after the step (ntust_kinetic_energy) and
+ * C_D, C_L the step's drag and lift from the force integral
+ * (virtualForceIntegrator.f90:22-81, the drag from FZ, the lift from FY,
+ * as ntust_step reports them; ntust_configure's lx).  The force integral is
+ * linear in the forcing once its skip tests (a line sum skips a sample that
+ * is exactly zero) are taken as they fell in the step, and its adjoint uses
+ * them so.  eta_gradient != 0: the reverse pass also accumulates dJ/dETA
+ * (ntust_adjoint_eta_gradient), the sensitivity to the immersed-boundary
+ * volume fraction through the direct forcing and the force integral; with
+ * the ETA of a geometry at two angles of attack it gives dJ/dAOA by the
+ * chain rule.  Default: (1, 0, 0, 0), the kinetic energy alone.  Kept
+ * until changed or ntust_finalize. */
+int ntust_adjoint_objective(double w_ke, double w_cd, double w_cl, int eta_gradient);
+
+/* dJ/dETA accumulated by the last ntust_adjoint_run (or by the
+ * ntust_adjoint_step calls since ntust_adjoint_set), a halo array with the
+ * interior meaningful.  NTUST_E_STATE unless ntust_adjoint_objective asked
+ * for it. */
+int ntust_adjoint_eta_gradient(double *geta);
+
+/* Forward nsteps from the device state and back.  Objective: J of
+ * ntust_adjoint_objective over the last navg steps.  The forward pass
  * keeps a checkpoint every checkpoint_every steps; the reverse pass
  * recomputes each segment from its checkpoint (the last segment's states
  * are kept from the forward pass) and applies the reverse steps.  Outputs:

@@ -163,25 +163,68 @@ void launch_adj_boundary_conditions(const KGrid &g, const BcTable &bc, double *l
 // projection and direct forcing, transposed
 // ---------------------------------------------------------------------
 
+// Drag and lift of one step (virtualForceIntegrator.f90:22-81) as functions
+// of the projection's u1.  With the step's skip tests frozen (a y or x line
+// sum skips a sample that is exactly zero), C_D is linear in FZ:
+//   C_D = -2/lx sum_ijk Ax(i) Ay(i,j) Az(k) FZ(i,j,k)
+//   Ax(i)   = Dxs(i)/6 (4 mx(i) + mx(i-1) [i>1] + mx(i+1) [i<nx]),   mx(i)   = [FXy(i) != 0]
+//   Ay(i,j) = Dys(j)/6 (4 my(i,j) + my(i,j-1) [j>1] + my(i,j+1) [j<ny]), my(i,j) = [FXz(i,j) != 0]
+//   Az(k)   = Dzs(k)/6 (4 + [k>1] + [k<nz])
+// (the halo planes of FZ and the padding of the line sums are constants),
+// and C_L alike in FY with FYz, FYy.  FZ = (w2 - w1)/dt with
+// w2 = ETA w_solid + (1-ETA) w1 and w_solid = 0 in the resident step, so
+// dFZ/dw1 = ((1-ETA) - 1)/dt and dFZ/dETA = -w1/dt.
+__device__ __forceinline__ double nz_mask(const double *a, long s)
+{
+    return a[s] != 0.0 ? 1.0 : 0.0;
+}
+
+__device__ __forceinline__ double force_weight(const KGrid &g, const double *fz, const double *fy,
+                                               int i, int j, int k, double lx)
+{
+    const long px = g.nx + 2;
+    double ax = 4.0 * nz_mask(fy, i);
+    if (i > 1) ax += nz_mask(fy, i - 1);
+    if (i < g.nx) ax += nz_mask(fy, i + 1);
+    double ay = 4.0 * nz_mask(fz, i + px * j);
+    if (j > 1) ay += nz_mask(fz, i + px * (j - 1));
+    if (j < g.ny) ay += nz_mask(fz, i + px * (j + 1));
+    const double az = 4.0 + (k > 1 ? 1.0 : 0.0) + (k < g.nz ? 1.0 : 0.0);
+    return (-2.0 / lx) * (g.Dxs[i] / 6.0 * ax) * (g.Dys[j] / 6.0 * ay) * (g.Dzs[k] / 6.0 * az);
+}
+
 // u = (1-ETA) u1 + ETA u_solid,  u1 = u* - dt (p(i+1) - p(i)) / Dxs(i):
-// lus = (1-ETA) lu on the interior.  u1(nx), v1(ny), w1(nz) read the p
-// ghost planes nx+1, ny+1, nz+1, which hold the previous step's values
-// during the step; their adjoint goes to the ghost entries of lp (zero
-// here: the transposed boundary conditions cleared them), each written by
-// exactly one thread.
+// lus = (1-ETA) lu on the interior, plus the drag / lift terms of the
+// objective (F.sd = dJ/dC_D, F.sl = dJ/dC_L of this step, with the step's
+// line sums).  u1(nx), v1(ny), w1(nz) read the p ghost planes nx+1, ny+1,
+// nz+1, which hold the previous step's values during the step; their
+// adjoint goes to the ghost entries of lp (zero here: the transposed
+// boundary conditions cleared them), each written by exactly one thread.
+// F.geta (with the step's u1, v1, w1): dJ/dETA += lu du2/dETA + ... =
+// -(lu u1 + lv v1 + lw w1) - (dJ/dFY v1 + dJ/dFZ w1)/dt.
 __global__ void __launch_bounds__(256)
 k_adj_project_a(KGrid g, const double *__restrict__ lu, const double *__restrict__ lv,
                 const double *__restrict__ lw, const double *__restrict__ eta, double dt,
-                double *__restrict__ lus, double *__restrict__ lvs, double *__restrict__ lws,
-                double *__restrict__ lp)
+                AdjForce F, double *__restrict__ lus, double *__restrict__ lvs,
+                double *__restrict__ lws, double *__restrict__ lp)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     const int k = blockIdx.z + 1;
     if (i > g.nx || j > g.ny) return;
     const long o = hix(g, i, j, k);
-    const double s = 1.0 - eta[o];
-    const double a = s * lu[o], b = s * lv[o], c = s * lw[o];
+    const double e = eta[o];
+    const double s = 1.0 - e;
+    double a = s * lu[o], b = s * lv[o], c = s * lw[o];
+    const double wd = F.sd != 0.0 ? F.sd * force_weight(g, F.fxz, F.fxy, i, j, k, F.lx) : 0.0;
+    const double wl = F.sl != 0.0 ? F.sl * force_weight(g, F.fyz, F.fyy, i, j, k, F.lx) : 0.0;
+    const double dfd = ((1.0 - e) - 1.0) / dt;
+    b += wl * dfd;
+    c += wd * dfd;
+    if (F.geta) {
+        const double u1 = F.u1[o], v1 = F.v1[o], w1 = F.w1[o];
+        F.geta[o] += -(lu[o] * u1 + lv[o] * v1 + lw[o] * w1) - (wl * v1 + wd * w1) / dt;
+    }
     lus[o] = a;
     lvs[o] = b;
     lws[o] = c;
@@ -299,6 +342,92 @@ k_adj_sweep(KGrid g, SplitGrid s, const double *__restrict__ rx,
     const double zn = self * z_own[o] + omega * sum / aP;
     z_own[o] = zn;
     if (acc) Z_own[o] += zn;
+}
+
+// The transposed half-sweeps with FP32 arithmetic, for the adjoint of the
+// FP32-sweep solver.  The transposed half-sweep (1-omega) z + omega S(z)/aP
+// is affine in z like the forward one, so within a segment of half-sweeps
+// z = z0 + e with z0 FP64 and fixed, and the FP32 correction e follows the
+// forward FP32 sweep's update with the segment's residual
+//   r = S(z0) - aP z0            (the forward residual with b = 0)
+//   e <- e + omega ((S(e) + r)/aP - e)
+// and Ef sums e after every accumulating half-sweep, so that at the end of
+// the segment Z += n z0 + Ef and z0 += e (k_adj_fold).  The FP32 rounding
+// is then relative to one segment's change of z, as in the forward.
+// Thread layout of the forward FP32 sweep: RB_KPT cells along k, XCD-slab
+// block order, loads first; no convergence maxima (the count is fixed).
+template <int C>
+__global__ void __launch_bounds__(256)
+k_adj_sweep32(KGrid g, SplitGrid s, FMetric fm, float *__restrict__ e_own,
+              const float *__restrict__ e_oth, const float *__restrict__ r_own,
+              float *__restrict__ Ef_own, float omega, int acc)
+{
+    int bx, by, bz;
+    xcd_block(bx, by, bz);
+    const int q = bx * blockDim.x + threadIdx.x;
+    const int jr = by * blockDim.y + threadIdx.y + 1;
+    const int j = min(jr, g.ny);
+    const int k0 = bz * RB_KPT + 1;
+    const int sh0 = (C + 1 + j + k0) & 1;
+    const int ia = min(2 * q + 1 + sh0, g.nx + 1), ib = min(2 * q + 2 - sh0, g.nx + 1);
+    const float axa = fm.iDx[ia], rEa = fm.rx[ia], rWa = fm.rx[ia - 1];
+    const float axb = fm.iDx[ib], rEb = fm.rx[ib], rWb = fm.rx[ib - 1];
+    const float ay = fm.iDy[j], rN = fm.ry[j], rS = fm.ry[j - 1];
+    const unsigned mp = (unsigned)s.mp, pl = (unsigned)s.mp * (unsigned)s.ry;
+    const unsigned q1 = (unsigned)min(q, (g.nx + 1) / 2);
+    float vW[RB_KPT], vE[RB_KPT], vN[RB_KPT], vS[RB_KPT], vF[RB_KPT], vB[RB_KPT], vr[RB_KPT],
+        vo[RB_KPT], vf[RB_KPT];
+    unsigned at[RB_KPT];
+#pragma unroll
+    for (int r = 0; r < RB_KPT; r++) {
+        const int k = min(k0 + r, g.nz);
+        const unsigned row = mp * ((unsigned)j + (unsigned)s.ry * (unsigned)k);
+        const unsigned o = row + q1 + (unsigned)(sh0 ^ (r & 1));
+        at[r] = o;
+        vW[r] = e_oth[row + q1];
+        vE[r] = e_oth[row + q1 + 1];
+        vN[r] = e_oth[o + mp];
+        vS[r] = e_oth[o - mp];
+        vF[r] = e_oth[o + pl];
+        vB[r] = e_oth[o - pl];
+        vr[r] = r_own[o];
+        vo[r] = e_own[o];
+        vf[r] = acc ? Ef_own[o] : 0.0f;
+    }
+#pragma unroll
+    for (int r = 0; r < RB_KPT; r++) {
+        const int k = min(k0 + r, g.nz);
+        const bool odd = r & 1;
+        const float ax = odd ? axb : axa, rE = odd ? rEb : rEa, rW = odd ? rWb : rWa;
+        const float az = fm.iDz[k], rF = fm.rz[k], rB = fm.rz[k - 1];
+        const float ayz = ay * az, axz = ax * az, axy = ax * ay;
+        const float t1 = fmaf(rE, vE[r], rW * vW[r]);
+        const float t2 = fmaf(rN, vN[r], rS * vS[r]);
+        const float t3 = fmaf(rF, vF[r], rB * vB[r]);
+        const float sum = fmaf(ayz, t1, fmaf(axz, t2, axy * t3));
+        const float aP = fmaf(ayz, rE + rW, fmaf(axz, rN + rS, axy * (rF + rB)));
+        const float nv = fmaf(omega, (sum + vr[r]) / aP - vo[r], vo[r]);
+        const int i = 2 * q + 1 + (sh0 ^ (int)odd);
+        if (jr <= g.ny && k0 + r <= g.nz && i <= g.nx) {
+            e_own[at[r]] = nv;
+            if (acc) Ef_own[at[r]] = vf[r] + nv;
+        }
+    }
+}
+
+// end of a segment, one colour plane: Z += n z0 + Ef, z0 += e, e = Ef = 0
+// (every entry; the ghost and padding entries are all zero and stay so)
+__global__ void __launch_bounds__(256)
+k_adj_fold(long n, double nacc, double *__restrict__ z, float *__restrict__ e,
+           float *__restrict__ Ef, double *__restrict__ Z)
+{
+    const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n) return;
+    const double z0 = z[x];
+    Z[x] = Z[x] + (nacc * z0 + (double)Ef[x]);
+    z[x] = z0 + (double)e[x];
+    e[x] = 0.0f;
+    Ef[x] = 0.0f;
 }
 
 // colour planes -> lp (interior: adjoint of the initial guess, the previous
@@ -705,30 +834,64 @@ k_axpy_interior(KGrid g, double a, const double *__restrict__ x, const double *_
 static dim3 cells_grid(const KGrid &g) { return dim3((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz); }
 
 void launch_adj_project(const KGrid &g, const double *lu, const double *lv, const double *lw,
-                        const double *eta, double dt, double *lus, double *lvs, double *lws,
-                        double *lp, double *lu_clear, double *lv_clear, double *lw_clear,
-                        hipStream_t st)
+                        const double *eta, double dt, const AdjForce &F, double *lus,
+                        double *lvs, double *lws, double *lp, double *lu_clear,
+                        double *lv_clear, double *lw_clear, hipStream_t st)
 {
     hipLaunchKernelGGL(k_adj_project_a, cells_grid(g), dim3(64, 4), 0, st, g, lu, lv, lw, eta,
-                       dt, lus, lvs, lws, lp);
+                       dt, F, lus, lvs, lws, lp);
     hipLaunchKernelGGL(k_adj_project_b, cells_grid(g), dim3(64, 4), 0, st, g, lus, lvs, lws, dt,
                        lp, lu_clear, lv_clear, lw_clear);
 }
 
 void launch_adj_pressure(const KGrid &g, const SplitGrid &s, const double *rx, const double *ry,
                          const double *rz, double omega, int sweeps, double *lp, double *lb,
-                         double *z0, double *z1, double *Z0, double *Z1, hipStream_t st)
+                         double *z0, double *z1, double *Z0, double *Z1,
+                         const AdjMixed *mx, hipStream_t st)
 {
     const dim3 blk(64, 4);
     const int nq = (g.nx + 2) / 2;
     const dim3 grs((nq + 63) / 64, (g.ny + 3) / 4, g.nz);
     hipLaunchKernelGGL(k_adj_to_split, cells_grid(g), blk, 0, st, g, s, rx, ry, rz, lp, omega,
                        z0, z1, Z0, Z1);
-    for (int t = sweeps; t >= 1; t--) {
-        hipLaunchKernelGGL(k_adj_sweep<0>, grs, blk, 0, st, g, s, rx, ry, rz, z0, z1, Z0,
-                           t == sweeps ? 1.0 : 1.0 - omega, omega, 1);
-        hipLaunchKernelGGL(k_adj_sweep<1>, grs, blk, 0, st, g, s, rx, ry, rz, z1, z0, Z1,
-                           1.0 - omega, omega, t > 1 ? 1 : 0);
+    if (!mx) {
+        for (int t = sweeps; t >= 1; t--) {
+            hipLaunchKernelGGL(k_adj_sweep<0>, grs, blk, 0, st, g, s, rx, ry, rz, z0, z1, Z0,
+                               t == sweeps ? 1.0 : 1.0 - omega, omega, 1);
+            hipLaunchKernelGGL(k_adj_sweep<1>, grs, blk, 0, st, g, s, rx, ry, rz, z1, z0, Z1,
+                               1.0 - omega, omega, t > 1 ? 1 : 0);
+        }
+    } else if (sweeps > 0) {
+        // the first red half-sweep (self factor 1) in FP64, then the
+        // sequence black, (red, black) x (sweeps-1) -- the last black not
+        // accumulating -- in FP32 segments of mx->seg half-sweeps
+        hipLaunchKernelGGL(k_adj_sweep<0>, grs, blk, 0, st, g, s, rx, ry, rz, z0, z1, Z0, 1.0,
+                           omega, 1);
+        const int nq32 = (g.nx + 1) / 2;
+        const dim3 g32((nq32 + 63) / 64, (g.ny + 3) / 4, (g.nz + RB_KPT - 1) / RB_KPT);
+        const dim3 gf((unsigned)((s.plane + 255) / 256));
+        const long nh = 2L * sweeps - 1;   // half-sweeps after the first
+        for (long h0 = 0; h0 < nh; h0 += mx->seg) {
+            const long h1 = h0 + mx->seg < nh ? h0 + mx->seg : nh;
+            launch_rb_residual(g, s, rx, ry, rz, z0, z1, mx->bzero, mx->bzero, mx->r0, mx->r1,
+                               nullptr, st);
+            double nacc[2] = {0.0, 0.0};
+            for (long h = h0; h < h1; h++) {
+                const int c = (h & 1) ? 0 : 1;            // black, red, black, ...
+                const int acc = h == nh - 1 ? 0 : 1;      // the last black does not
+                if (c == 0)
+                    hipLaunchKernelGGL(k_adj_sweep32<0>, g32, blk, 0, st, g, s, mx->fm, mx->e0,
+                                       mx->e1, mx->r0, mx->f0, (float)omega, acc);
+                else
+                    hipLaunchKernelGGL(k_adj_sweep32<1>, g32, blk, 0, st, g, s, mx->fm, mx->e1,
+                                       mx->e0, mx->r1, mx->f1, (float)omega, acc);
+                nacc[c] += acc;
+            }
+            hipLaunchKernelGGL(k_adj_fold, gf, dim3(256), 0, st, s.plane, nacc[0], z0, mx->e0,
+                               mx->f0, Z0);
+            hipLaunchKernelGGL(k_adj_fold, gf, dim3(256), 0, st, s.plane, nacc[1], z1, mx->e1,
+                               mx->f1, Z1);
+        }
     }
     if (sweeps > 0) {
         hipLaunchKernelGGL(k_adj_from_split, cells_grid(g), blk, 0, st, g, s, rx, ry, rz, z0, z1,

@@ -85,6 +85,10 @@ struct Ctx {
     double *fu = nullptr, *fv = nullptr, *fw = nullptr;
     // exact FP64 checkpoints of (u, v, w, p), slots of 4 halo arrays
     int nckpt = 0;
+    // objective of the adjoint run: weights of the mean kinetic energy,
+    // C_D and C_L over the averaging window; ETA gradient wanted
+    double adj_w[3] = {1.0, 0.0, 0.0};
+    bool adj_eta = false;
 };
 
 Ctx C;
@@ -571,6 +575,9 @@ struct StepBufs {
     double *u, *v, *w, *p, *eta, *us, *vs, *ws, *u2, *v2, *w2, *pre, *uc, *vc, *wc, *vis, *filt;
     double *FY = nullptr, *FZ = nullptr, *fxz = nullptr, *fyz = nullptr, *fxy = nullptr,
            *fyy = nullptr, *coef = nullptr;
+    // when set, the projection also writes its u1, v1, w1 there (the
+    // adjoint's ETA gradient reads them)
+    double *u1 = nullptr, *v1 = nullptr, *w1 = nullptr;
 };
 
 int step_buffers(StepBufs &b, bool force)
@@ -629,8 +636,8 @@ int one_step(const StepBufs &b, bool force, bool centre, int *ik, double *pm, do
     TRY(solve_pressure(b.us, b.vs, b.ws, b.p, b.pre, q.dt, q.omega, q.zeta, q.itmax, q.sor, ik,
                        pm));
     // main.f90:247,255 -- projection, forcing and the update u = u2
-    launch_project(g, b.us, b.vs, b.ws, b.p, b.eta, 0.0, 0.0, 0.0, q.dt, nullptr, nullptr,
-                   nullptr, nullptr, nullptr, nullptr, nullptr, force ? b.FY : nullptr,
+    launch_project(g, b.us, b.vs, b.ws, b.p, b.eta, 0.0, 0.0, 0.0, q.dt, b.u1, b.v1, b.w1,
+                   nullptr, nullptr, nullptr, nullptr, force ? b.FY : nullptr,
                    force ? b.FZ : nullptr, centre ? b.uc : nullptr, centre ? b.vc : nullptr,
                    centre ? b.wc : nullptr, b.u, b.v, b.w, b.u2, b.v2, b.w2, 1, C.nz, C.st);
     // main.f90:251
@@ -696,17 +703,36 @@ int adjoint_ready()
 // it.  (ui, vi, wi) is the velocity at the start of the step, sweeps the
 // step's SOR sweep count.  Adds dt*lambda_u* to the body-force gradient.
 int adjoint_of_step(const AdjBufs &a, const double *eta, const double *filt, const double *ui,
-                    const double *vi, const double *wi, int sweeps)
+                    const double *vi, const double *wi, int sweeps, const AdjForce &F)
 {
     const Params &q = C.prm;
     const KGrid &g = C.g;
     const double *rx = C.recip[0] + 1, *ry = C.recip[1] + 1, *rz = C.recip[2] + 1;
     launch_adj_boundary_conditions(g, C.bc, a.lu, a.lv, a.lw, a.lp, C.st);
-    launch_adj_project(g, a.lu, a.lv, a.lw, eta, q.dt, a.lus, a.lvs, a.lws, a.lp, a.lu, a.lv,
+    launch_adj_project(g, a.lu, a.lv, a.lw, eta, q.dt, F, a.lus, a.lvs, a.lws, a.lp, a.lu, a.lv,
                        a.lw, C.st);
+    // the FP32-sweep solver's adjoint runs its transposed sweeps in FP32
+    // too (FP64 folds every mix_segment sweeps); the FP64 solver's in FP64
+    AdjMixed mx{};
+    const bool mixed = q.sor == NTUST_SOR_MIXED;
+    if (mixed) {
+        const size_t nf = (size_t)C.sg.plane / 2 + 1;   // float planes in double buffers
+        const char *fn[6] = {"adj_e0", "adj_e1", "adj_r0", "adj_r1", "adj_f0", "adj_f1"};
+        float **fp[6] = {&mx.e0, &mx.e1, &mx.r0, &mx.r1, &mx.f0, &mx.f1};
+        for (int k = 0; k < 6; k++) {
+            double *d = buf(fn[k], nf, true);
+            if (!d) return fail(NTUST_E_ALLOC, "device allocation of %s failed", fn[k]);
+            *fp[k] = (float *)d;
+        }
+        NEED(bz, "adj_bzero", (size_t)C.sg.plane, true);
+        mx.bzero = bz;
+        mx.fm = FMetric{C.fmetric[0] + 1, C.fmetric[1] + 1, C.fmetric[2] + 1,
+                        C.fmetric[3] + 1, C.fmetric[4] + 1, C.fmetric[5] + 1};
+        mx.seg = 2 * C.mix_segment;
+    }
     if (sweeps > 0)
         launch_adj_pressure(g, C.sg, rx, ry, rz, q.omega, sweeps, a.lp, a.lb, a.z0, a.z1, a.Z0,
-                            a.Z1, C.st);
+                            a.Z1, mixed ? &mx : nullptr, C.st);
     else
         HIPCHK(hipMemsetAsync(a.lb, 0, C.nh * sizeof(double), C.st));
     launch_adj_divergence(g, a.lb, q.dt, a.lus, a.lvs, a.lws, a.lu, a.lv, a.lw, C.st);
@@ -1436,6 +1462,10 @@ int ntust_adjoint_set(const double *lu, const double *lv, const double *lw, cons
     }
     double *gr[3] = {a.gu, a.gv, a.gw};
     for (int q = 0; q < 3; q++) HIPCHK(hipMemsetAsync(gr[q], 0, C.nh * sizeof(double), C.st));
+    if (C.adj_eta) {
+        NEED(ge, "adj_geta", C.nh, true);
+        HIPCHK(hipMemsetAsync(ge, 0, C.nh * sizeof(double), C.st));
+    }
     return sync();
 }
 
@@ -1459,6 +1489,27 @@ int ntust_adjoint_gradient(double *gfu, double *gfv, double *gfw)
     if (gfu) TRY(d2h(gfu, a.gu, C.nh));
     if (gfv) TRY(d2h(gfv, a.gv, C.nh));
     if (gfw) TRY(d2h(gfw, a.gw, C.nh));
+    return sync();
+}
+int ntust_adjoint_objective(double w_ke, double w_cd, double w_cl, int eta_gradient)
+{
+    NEED_INIT();
+    if (!std::isfinite(w_ke) || !std::isfinite(w_cd) || !std::isfinite(w_cl))
+        return fail(NTUST_E_ARG, "objective weights must be finite");
+    C.adj_w[0] = w_ke;
+    C.adj_w[1] = w_cd;
+    C.adj_w[2] = w_cl;
+    C.adj_eta = eta_gradient != 0;
+    return 0;
+}
+int ntust_adjoint_eta_gradient(double *geta)
+{
+    NEED_INIT();
+    if (!C.adj_eta)
+        return fail(NTUST_E_STATE, "no ETA gradient: ntust_adjoint_objective(..., eta_gradient=1) "
+                                   "was not set before the adjoint");
+    NEED(ge, "adj_geta", C.nh, true);
+    if (geta) TRY(d2h(geta, ge, C.nh));
     return sync();
 }
 
@@ -1488,10 +1539,22 @@ int ntust_adjoint_step(int *ik)
     NEED(ip, "adj_in_p", C.nh, false);
     TRY(copy3(iu, iv, iw, b.u, b.v, b.w));
     HIPCHK(hipMemcpyAsync(ip, b.p, C.nh * sizeof(double), hipMemcpyDeviceToDevice, C.st));
+    AdjForce F;
+    if (C.adj_eta) {
+        NEED(g1, "adj_u1", 3 * C.nh, false);
+        NEED(ge, "adj_geta", C.nh, true);
+        b.u1 = g1;
+        b.v1 = g1 + C.nh;
+        b.w1 = g1 + 2 * C.nh;
+        F.u1 = b.u1;
+        F.v1 = b.v1;
+        F.w1 = b.w1;
+        F.geta = ge;
+    }
     int n = 0;
     double pm = 0.0, cd = NAN, cl = NAN;
     TRY(one_step(b, false, false, &n, &pm, &cd, &cl));
-    TRY(adjoint_of_step(a, b.eta, b.filt, iu, iv, iw, n));
+    TRY(adjoint_of_step(a, b.eta, b.filt, iu, iv, iw, n, F));
     // the state is left as it was
     TRY(copy3(b.u, b.v, b.w, iu, iv, iw));
     HIPCHK(hipMemcpyAsync(b.p, ip, C.nh * sizeof(double), hipMemcpyDeviceToDevice, C.st));
@@ -1511,10 +1574,29 @@ int ntust_adjoint_run(int nsteps, int checkpoint_every, int navg, double *J, int
     if (navg < 0 || navg > nsteps) return fail(NTUST_E_ARG, "navg outside 0..nsteps");
     const int N = nsteps, K = checkpoint_every < N ? checkpoint_every : N;
     const int nseg = (N + K - 1) / K;
+    const double wke = C.adj_w[0], wcd = C.adj_w[1], wcl = C.adj_w[2];
+    const double wJ = navg > 0 ? 1.0 / navg : 0.0;
+    const bool force = wcd != 0.0 || wcl != 0.0, want_eta = C.adj_eta;
     StepBufs b;
-    TRY(step_buffers(b, false));
+    TRY(step_buffers(b, force));
     AdjBufs a;
     TRY(adj_buffers(a));
+    // per step of a segment: the force integral's line sums FXz, FYz, FXy,
+    // FYy (their zero borders are never written) and the projection's u1,
+    // v1, w1, as the reverse step of a drag / lift objective and of the ETA
+    // gradient read them
+    const size_t pz = (size_t)(C.nx + 2) * (C.ny + 2), fsz = 2 * pz + 2 * (size_t)(C.nx + 2);
+    double *fseg = nullptr, *useg = nullptr, *geta = nullptr;
+    if (force) {
+        NEED(x, "adj_fseg", (size_t)K * fsz, true);
+        fseg = x;
+    }
+    if (want_eta) {
+        NEED(x, "adj_useg", (size_t)K * 3 * C.nh, false);
+        NEED(y, "adj_geta", C.nh, true);
+        useg = x;
+        geta = y;
+    }
     NEED(ck, "ckpt", (size_t)nseg * 4 * C.nh, false);
     if (nseg > C.nckpt) C.nckpt = nseg;
     // velocity at the start of every step of one segment, and after its last
@@ -1523,9 +1605,46 @@ int ntust_adjoint_run(int nsteps, int checkpoint_every, int navg, double *J, int
     const size_t nh = C.nh;
     auto slot = [&](int n) { return ck + (size_t)n * 4 * nh; };
     auto segv = [&](int j) { return seg + (size_t)j * 3 * nh; };
+    // direct the step's line sums and u1 to slot j of the segment (j < 0:
+    // line sums to slot 0, u1 not kept)
+    auto bind = [&](int j) {
+        if (force) {
+            double *f = fseg + (size_t)(j < 0 ? 0 : j) * fsz;
+            b.fxz = f;
+            b.fyz = f + pz;
+            b.fxy = f + 2 * pz;
+            b.fyy = f + 2 * pz + (C.nx + 2);
+        }
+        if (want_eta) {
+            double *u = j < 0 ? nullptr : useg + (size_t)j * 3 * nh;
+            b.u1 = u;
+            b.v1 = u ? u + nh : nullptr;
+            b.w1 = u ? u + 2 * nh : nullptr;
+        }
+    };
+    auto adj_force = [&](int n, int j) {
+        AdjForce F;
+        if (force && n >= N - navg) {
+            const double *f = fseg + (size_t)j * fsz;
+            F.sd = wJ * wcd;
+            F.sl = wJ * wcl;
+            F.fxz = f;
+            F.fyz = f + pz;
+            F.fxy = f + 2 * pz;
+            F.fyy = f + 2 * pz + (C.nx + 2);
+            F.lx = C.prm.lx;
+        }
+        if (want_eta) {
+            const double *u = useg + (size_t)j * 3 * nh;
+            F.u1 = u;
+            F.v1 = u + nh;
+            F.w1 = u + 2 * nh;
+            F.geta = geta;
+        }
+        return F;
+    };
     std::vector<int> ikf((size_t)N), ikr((size_t)N);
     const int last0 = (nseg - 1) * K;
-    const double wJ = navg > 0 ? 1.0 / navg : 0.0;
     double Jsum = 0.0;
     double t_fwd = 0.0, t_rec = 0.0, t_adj = 0.0;
     auto now = [] { return std::chrono::steady_clock::now(); };
@@ -1548,12 +1667,17 @@ int ntust_adjoint_run(int nsteps, int checkpoint_every, int navg, double *J, int
             double *d = segv(n - last0);
             TRY(copy3(d, d + nh, d + 2 * nh, b.u, b.v, b.w));
         }
+        bind(n >= last0 ? n - last0 : -1);
         double pm = 0.0, cd = NAN, cl = NAN;
-        TRY(one_step(b, false, n == N - 1, &ikf[n], &pm, &cd, &cl));
-        if (n >= N - navg) {
-            double ke = 0.0;
-            TRY(kinetic_energy(b.u, b.v, b.w, &ke));
-            Jsum += ke;
+        const bool win = n >= N - navg;
+        TRY(one_step(b, force && win, n == N - 1, &ikf[n], &pm, &cd, &cl));
+        if (win) {
+            if (wke != 0.0) {
+                double ke = 0.0;
+                TRY(kinetic_energy(b.u, b.v, b.w, &ke));
+                Jsum += wke * ke;
+            }
+            if (force) Jsum += wcd * cd + wcl * cl;
         }
     }
     {
@@ -1570,6 +1694,7 @@ int ntust_adjoint_run(int nsteps, int checkpoint_every, int navg, double *J, int
     // reverse pass
     double *clear[7] = {a.lu, a.lv, a.lw, a.lp, a.gu, a.gv, a.gw};
     for (double *x : clear) HIPCHK(hipMemsetAsync(x, 0, nh * sizeof(double), C.st));
+    if (geta) HIPCHK(hipMemsetAsync(geta, 0, nh * sizeof(double), C.st));
     for (int sg = nseg - 1; sg >= 0; sg--) {
         const int s0 = sg * K, s1 = s0 + K < N ? s0 + K : N;
         if (sg != nseg - 1) {
@@ -1582,8 +1707,9 @@ int ntust_adjoint_run(int nsteps, int checkpoint_every, int navg, double *J, int
             for (int n = s0; n < s1; n++) {
                 double *e = segv(n - s0);
                 TRY(copy3(e, e + nh, e + 2 * nh, b.u, b.v, b.w));
+                bind(n - s0);
                 double pm = 0.0, cd = NAN, cl = NAN;
-                TRY(one_step(b, false, false, &ikr[n], &pm, &cd, &cl));
+                TRY(one_step(b, force && n >= N - navg, false, &ikr[n], &pm, &cd, &cl));
                 if (ikr[n] != ikf[n])
                     return fail(NTUST_E_STATE,
                                 "step %d recomputed from its checkpoint ran %d SOR sweeps, "
@@ -1596,12 +1722,13 @@ int ntust_adjoint_run(int nsteps, int checkpoint_every, int navg, double *J, int
         }
         auto a0 = now();
         for (int n = s1 - 1; n >= s0; n--) {
-            if (n >= N - navg) {
+            if (n >= N - navg && wke != 0.0) {
                 const double *e = segv(n - s0 + 1);
-                launch_ke_gradient(C.g, e, e + nh, e + 2 * nh, wJ, a.lu, a.lv, a.lw, C.st);
+                launch_ke_gradient(C.g, e, e + nh, e + 2 * nh, wJ * wke, a.lu, a.lv, a.lw, C.st);
             }
             const double *e = segv(n - s0);
-            TRY(adjoint_of_step(a, b.eta, b.filt, e, e + nh, e + 2 * nh, ikf[n]));
+            TRY(adjoint_of_step(a, b.eta, b.filt, e, e + nh, e + 2 * nh, ikf[n],
+                                adj_force(n, n - s0)));
         }
         TRY(sync());
         t_adj += msec(a0, now());

@@ -284,13 +284,36 @@ void launch_copy_shell(const KGrid &g, const double *u, const double *v, const d
 // discrete adjoint of the resident step (transposed stages, reverse order)
 void launch_adj_boundary_conditions(const KGrid &g, const BcTable &bc, double *lu, double *lv,
                                     double *lw, double *lp, hipStream_t st);
+// objective terms of the transposed projection of one step: dJ/dC_D and
+// dJ/dC_L of the step with its force-integral line sums (FXz, FXy, FYz,
+// FYy; unused when the scale is 0), and the ETA gradient accumulator with
+// the step's u1, v1, w1 (geta null: none)
+struct AdjForce {
+    double sd = 0.0, sl = 0.0;
+    const double *fxz = nullptr, *fxy = nullptr, *fyz = nullptr, *fyy = nullptr;
+    double lx = 1.0;
+    const double *u1 = nullptr, *v1 = nullptr, *w1 = nullptr;
+    double *geta = nullptr;
+};
 void launch_adj_project(const KGrid &g, const double *lu, const double *lv, const double *lw,
-                        const double *eta, double dt, double *lus, double *lvs, double *lws,
-                        double *lp, double *lu_clear, double *lv_clear, double *lw_clear,
-                        hipStream_t st);
+                        const double *eta, double dt, const AdjForce &F, double *lus,
+                        double *lvs, double *lws, double *lp, double *lu_clear,
+                        double *lv_clear, double *lw_clear, hipStream_t st);
+// FP32 transposed sweeps for the adjoint of the FP32-sweep solver: the
+// correction e, its residual r and running sum Ef per colour (FP32 colour
+// planes, ghost and padding entries zero), a zero FP64 plane for b = 0, and
+// the segment length in half-sweeps between FP64 folds
+struct AdjMixed {
+    FMetric fm;
+    float *e0, *e1, *r0, *r1, *f0, *f1;
+    const double *bzero;
+    long seg;
+};
+// mx null: the FP64 transposed sweeps
 void launch_adj_pressure(const KGrid &g, const SplitGrid &s, const double *rx, const double *ry,
                          const double *rz, double omega, int sweeps, double *lp, double *lb,
-                         double *z0, double *z1, double *Z0, double *Z1, hipStream_t st);
+                         double *z0, double *z1, double *Z0, double *Z1, const AdjMixed *mx,
+                         hipStream_t st);
 void launch_adj_divergence(const KGrid &g, const double *lb, double dt, double *lus,
                            double *lvs, double *lws, double *lu, double *lv, double *lw,
                            hipStream_t st);

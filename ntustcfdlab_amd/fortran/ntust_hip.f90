@@ -332,6 +332,19 @@ module ntust_hip
          integer(c_int), intent(out) :: ik(*)
          real(c_double), intent(out) :: ms(4)
       end function
+
+      integer(c_int) function ntust_adjoint_objective(w_ke, w_cd, w_cl, eta_gradient) &
+            bind(C, name="ntust_adjoint_objective")
+         import :: c_int, c_double
+         real(c_double), value :: w_ke, w_cd, w_cl
+         integer(c_int), value :: eta_gradient
+      end function
+
+      integer(c_int) function ntust_adjoint_eta_gradient(geta) &
+            bind(C, name="ntust_adjoint_eta_gradient")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: geta
+      end function
    end interface
 
 contains

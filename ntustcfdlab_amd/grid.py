@@ -56,19 +56,22 @@ def uniform_nodes(n: int, length: float) -> np.ndarray:
 
 
 def tanh_nodes(n: int, length: float, centre: float, beta: float) -> np.ndarray:
-    """Two-sided tanh stretching clustered around `centre` (0<centre<length):
-    x(s) = c + A*tanh(beta*(s - s0))/tanh(beta) type mapping of a uniform
-    parameter s in [0,1], with the end nodes at 0 and length and uniform
-    ghost spacing copied from the end cells.  The reference has only uniform
-    and piecewise-uniform grids (gridder.f90:1-223); this generator serves
-    the stretched airfoil grids."""
+    """Nodes clustered around the interior point `centre` (0 < centre <
+    length, an absolute coordinate): Roberts' interior-clustering transform
+    of a uniform parameter s in [0, 1],
+        x(s) = c (1 + sinh(beta (s - A)) / sinh(beta A)),
+        A = ln[(1 + (e^beta - 1) c/L) / (1 + (e^-beta - 1) c/L)] / (2 beta),
+    which puts x(0) = 0, x(1) = L and the finest spacing at x = c (the
+    spacing grows like cosh(beta (s - A)) away from it; beta > 0 sets the
+    strength).  Uniform ghost spacing copied from the end cells.  The
+    reference has only uniform and piecewise-uniform grids
+    (gridder.f90:1-223); this generator serves the stretched airfoil grids."""
+    if not (0.0 < centre < length) or not beta > 0.0:
+        raise ValueError("tanh_nodes needs 0 < centre < length and beta > 0")
     s = np.linspace(0.0, 1.0, n + 1)
-    c = centre / length
-    # map s -> xi in [0,1] with clustering at xi = c
-    a = np.arctanh(np.tanh(beta) * (0.0 - c) / max(c, 1 - c)) / beta
-    b = np.arctanh(np.tanh(beta) * (1.0 - c) / max(c, 1 - c)) / beta
-    t = a + (b - a) * s
-    xi = c + max(c, 1 - c) * np.tanh(beta * t) / np.tanh(beta)
+    r = centre / length
+    A = np.log((1.0 + (np.exp(beta) - 1.0) * r) / (1.0 + (np.exp(-beta) - 1.0) * r)) / (2 * beta)
+    xi = r * (1.0 + np.sinh(beta * (s - A)) / np.sinh(beta * A))
     xi[0], xi[-1] = 0.0, 1.0
     X = np.zeros(n + 5)
     X[2:n + 3] = xi * length           # nodes 1..n+1

@@ -84,6 +84,8 @@ def load_library() -> ct.CDLL:
         "ntust_adjoint_step": [_I],
         "ntust_adjoint_gradient": [_D] * 3,
         "ntust_adjoint_run": [i, i, i, _D, _I, _D],
+        "ntust_adjoint_objective": [d, d, d, i],
+        "ntust_adjoint_eta_gradient": [_D],
         "ntust_set_sweep_variant": [i],
         "ntust_poisson_coeffs": [_D, _I],
         "ntust_poisson_rhs": [_D, _D, _D, d, _D],
@@ -412,6 +414,18 @@ class ProjectionSolver:
     def adjoint_gradient(self):
         out = [self.grid.halo() for _ in range(3)]
         _check(self.lib.ntust_adjoint_gradient(*[_p(a) for a in out]))
+        return out
+
+    def adjoint_objective(self, w_ke: float = 1.0, w_cd: float = 0.0, w_cl: float = 0.0,
+                          eta_gradient: bool = False):
+        """J = mean over the averaging window of w_ke*KE + w_cd*C_D + w_cl*C_L;
+        eta_gradient also accumulates dJ/dETA (adjoint_eta_gradient)."""
+        _check(self.lib.ntust_adjoint_objective(float(w_ke), float(w_cd), float(w_cl),
+                                                1 if eta_gradient else 0))
+
+    def adjoint_eta_gradient(self):
+        out = self.grid.halo()
+        _check(self.lib.ntust_adjoint_eta_gradient(_p(out)))
         return out
 
     def adjoint_run(self, nsteps: int, checkpoint_every: int, navg: int):

@@ -23,8 +23,8 @@ from oracle import Oracle, dynamic_aoa
 
 def airfoil_grid(n=96, ny=None):
     ny = ny or n
-    return Grid(4, ny, n, ly=8.0, lz=8.0, Y=tanh_nodes(ny, 8.0, 0.5, 2.5),
-                Z=tanh_nodes(n, 8.0, 0.5, 2.5))
+    return Grid(4, ny, n, ly=8.0, lz=8.0, Y=tanh_nodes(ny, 8.0, 4.0, 5.0),
+                Z=tanh_nodes(n, 8.0, 4.0, 5.0))
 
 
 def centres(N, n):

@@ -22,8 +22,8 @@ def same(a, b):
 
 
 def airfoil_grid(nx, ny, nz, beta=2.5):
-    return Grid(nx, ny, nz, ly=8.0, lz=8.0, Y=tanh_nodes(ny, 8.0, 0.5, beta),
-                Z=tanh_nodes(nz, 8.0, 0.5, beta))
+    return Grid(nx, ny, nz, ly=8.0, lz=8.0, Y=tanh_nodes(ny, 8.0, 4.0, beta),
+                Z=tanh_nodes(nz, 8.0, 4.0, beta))
 
 
 CASES = {
