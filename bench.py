@@ -295,6 +295,66 @@ def airfoil_les_leg(hip, hiptime, steps, warmup):
     return out
 
 
+def airfoil_c3_leg(hip, hiptime, steps, warmup):
+    """The immersed-boundary configuration at moderate size: NACA 0012 (5000
+    vertices) at 5 degrees, Re 1000, QUICK, LES off, dt 2e-4; 384 cells
+    streamwise (z, 12.5 long) x 192 normal (y, 3.95) x 64 span (x, 3.95),
+    y and z clustered at the airfoil; uniform inflow w = 1 through the back
+    face, Neumann outflow, slip side walls (no-slip normal + Neumann
+    tangential components).  The reference places the airfoil at (z, y) =
+    (4, 4) (variables_module.f90:144-146), outside a 3.95-high domain; here
+    it sits at (4, 1.975).  Both solvers from the inflow state after
+    `warmup` steps; the per-step C_D and C_L series and their differences."""
+    from ntustcfdlab_amd.airfoil import naca4_polygon
+    from ntustcfdlab_amd.grid import Grid, bc_codes, tanh_nodes
+    nx, ny, nz = 64, 192, 384
+    zc, yc = 4.0, 1.975
+    g = Grid(nx, ny, nz, lx=3.95, ly=3.95, lz=12.5, Y=tanh_nodes(ny, 3.95, yc, 4.0),
+             Z=tanh_nodes(nz, 12.5, zc, 4.0))
+    bc = bc_codes(BackWall_w="Dirichlet", FrontWall_u="Neumann", FrontWall_v="Neumann",
+                  FrontWall_w="Neumann", NorthWall_u="Neumann", NorthWall_w="Neumann",
+                  SouthWall_u="Neumann", SouthWall_w="Neumann", WestWall_v="Neumann",
+                  WestWall_w="Neumann", EastWall_v="Neumann", EastWall_w="Neumann")
+    az0, ay0 = naca4_polygon(poly=5000)
+    s = hip.ProjectionSolver(g, bc)
+    out = {"workload": "NACA 0012 at 5 deg, Re 1000, QUICK, dt 2e-4, 64x192x384 (span x normal x "
+                       "streamwise) clustered at the airfoil, inflow w = 1, slip side walls",
+           "steps": steps, "warmup": warmup, "solvers": {}}
+    series = {}
+    try:
+        u, v, w, p = g.halo(), g.halo(), g.halo(), g.halo()
+        w[...] = 1.0
+        s.boundary_conditions(u, v, w, p)
+        eta = g.halo()
+        s.ray_casting(az0, ay0, 5.0, zc, yc, ETA=eta)
+        for name, sor in (("fp64", hip.SOR_REDBLACK), ("mixed", hip.SOR_MIXED)):
+            print(f"# airfoil C3 {name}", file=sys.stderr, flush=True)
+            s.configure(dt=2e-4, nu=1.0 / 1000.0, Cs=0.1, LES=0, scheme=hip.QUICK_CENTRE,
+                        omega=1.5, zeta=1e-10, itmax=3000.0, sor=sor, lx=3.95, force_every=1)
+            s.upload(u.copy(order="F"), v.copy(order="F"), w.copy(order="F"), p.copy(order="F"),
+                     eta)
+            if warmup:
+                s.step(warmup)
+            e0, e1 = hiptime.Event(), hiptime.Event()
+            hiptime.device_synchronize()
+            e0.record(s.stream())
+            ik, pm, cd, cl, rc = s.step(steps)
+            e1.record(s.stream())
+            ms = e0.elapsed_ms(e1)
+            series[name] = (cd.copy(), cl.copy())
+            out["solvers"][name] = {"value": round(g.cells * steps / (ms / 1e3) / 1e6, 3),
+                                    "ms_per_step": round(ms / steps, 3),
+                                    "sweeps_per_step": round(float(np.mean(ik)), 1),
+                                    "cD": [float(x) for x in cd], "cL": [float(x) for x in cl]}
+    finally:
+        s.close()
+    (d64, l64), (dm, lm) = series["fp64"], series["mixed"]
+    out["mixed_vs_fp64_max_rel_diff_cD"] = float(np.max(np.abs(dm - d64)) / np.max(np.abs(d64)))
+    out["mixed_vs_fp64_max_rel_diff_cL"] = float(np.max(np.abs(lm - l64)) / np.max(np.abs(l64)))
+    out["solid_cells"] = float(eta[2:-2, 2:-2, 2:-2].sum())
+    return out
+
+
 def actuator_strip(g, aoa, seed):
     """The actuator control of the large configuration: a seeded strip 0.05
     chord wide at x/c = 0.1, next to the surface (normal offset within 0.1
@@ -598,6 +658,8 @@ def main():
     ap.add_argument("--airfoil-adjoint-steps", type=int, default=8,
                     help="steps of the large airfoil adjoint leg (the configuration's "
                          "horizon is 256; checkpoints every ~sqrt(steps))")
+    ap.add_argument("--only-airfoil-c3", action="store_true",
+                    help="run the moderate immersed-boundary airfoil leg alone")
     ap.add_argument("--only-airfoil-adjoint", action="store_true",
                     help="run the large airfoil adjoint leg alone and print its JSON")
     a = ap.parse_args()
@@ -621,6 +683,9 @@ def main():
         return
     n_aa = max(a.airfoil_adjoint_steps, 2)
     aa_args = (n_aa, max(1, int(round(np.sqrt(n_aa)))), max(1, n_aa // 2))
+    if a.only_airfoil_c3:
+        print(json.dumps({"airfoil_c3_64x192x384": airfoil_c3_leg(hip, hiptime, 6, 2)}))
+        return
     if a.only_airfoil_adjoint:
         print(json.dumps({"airfoil_adjoint_512x384x128":
                           airfoil_adjoint_leg(hip, hiptime, *aa_args)}))
@@ -661,6 +726,7 @@ def main():
         print("# airfoil geometry", file=sys.stderr, flush=True)
         out["airfoil_ray_casting"] = airfoil_leg(hip, 10, not a.no_cpu_baseline)
         out["airfoil_les_512x384x128"] = airfoil_les_leg(hip, hiptime, 2, 1)
+        out["airfoil_c3_64x192x384"] = airfoil_c3_leg(hip, hiptime, 6, 2)
         if not a.no_adjoint:
             print("# airfoil adjoint", file=sys.stderr, flush=True)
             out["airfoil_adjoint_512x384x128"] = airfoil_adjoint_leg(hip, hiptime, *aa_args)

@@ -245,10 +245,12 @@ def test_body_force_step_matches_oracle(hip_lib):
         assert np.array_equal(a, r.f[n]), n
 
 
-def test_mixed_forward_adjoint_matches_fp64(hip_lib):
+@pytest.mark.parametrize("n", [16, 13])
+def test_mixed_forward_adjoint_matches_fp64(n, hip_lib):
     """With a fixed sweep count the FP32-sweep forward runs the same linear
-    iteration, so its adjoint gradient agrees with the FP64 one."""
-    g, state = cavity_setup(16, seed=5)
+    iteration, so its adjoint gradient (FP32 transposed sweeps with FP64
+    folds) agrees with the FP64 one; n = 13 gives odd extents (13, 11, 9)."""
+    g, state = cavity_setup(n, seed=5)
     eta = g.halo()
     out = {}
     for sor in (SOR_REDBLACK, SOR_MIXED):

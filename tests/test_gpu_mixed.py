@@ -54,6 +54,30 @@ def test_mixed_solve_tracks_fp64_iterates(itmax, hip_lib):
     assert abs(out[SOR_MIXED][1] - out[SOR_REDBLACK][1]) <= 1e-3 * out[SOR_REDBLACK][1]
 
 
+@pytest.mark.parametrize("shape", [(33, 9, 17), (7, 5, 3), (20, 31, 13)])
+@pytest.mark.parametrize("itmax", [65.0, 130.0])
+def test_mixed_tracks_fp64_on_odd_shapes(shape, itmax, hip_lib):
+    """Odd extents exercise the colour-split layout's row ends and the
+    interior tests of the correction kernel (absorb + residual); the FP32
+    solve must still track the FP64 one to one segment's FP32 rounding."""
+    g = stretched_grid(*shape)
+    s = ProjectionSolver(g)
+    rng = np.random.default_rng(sum(shape) + int(itmax))
+    us, vs, ws = (rand_halo(g, rng, 1e-2) for _ in range(3))
+    p0 = rand_halo(g, rng, 1e-3)
+    out = {}
+    try:
+        for mode in (SOR_REDBLACK, SOR_MIXED):
+            q = p0.copy(order="F")
+            pre = g.interior()
+            ik, pm, _ = s.gauss_seidel(us, vs, ws, q, pre, 1e-3, 1.5, 1e-30, itmax, mode)
+            out[mode] = (ik, pm, q)
+    finally:
+        s.close()
+    assert out[SOR_REDBLACK][0] == out[SOR_MIXED][0] == int(itmax)
+    assert rel(out[SOR_MIXED][2], out[SOR_REDBLACK][2]) < 64 * 2.0 ** -24
+
+
 def test_mixed_stops_where_fp64_stops(hip_lib):
     """zeta reached well before itmax: same sweep count (the change per sweep
     is far from zeta's resolution limit in FP32 here)."""
