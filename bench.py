@@ -100,8 +100,8 @@ def adjoint_workload(n=96):
 
 
 def adjoint_leg(hip, hiptime, n, steps, fd_dirs):
-    """Forward + checkpointed reverse pass; cost ratio against a plain
-    forward of the same steps; central finite differences (step 1e-6) of
+    """Forward + checkpointed reverse pass; cost ratio against the faster of a
+    plain forward of the same steps and the adjoint run's own forward pass; central finite differences (step 1e-6) of
     the plain forward along seeded random directions of the body force."""
     from ntustcfdlab_amd.grid import Grid, bc_codes, gaussian_blob
     a = adjoint_workload(n)
@@ -164,9 +164,91 @@ def adjoint_leg(hip, hiptime, n, steps, fd_dirs):
         "forward_ms": round(float(ms[0]), 1), "plain_forward_ms": round(t_plain, 1),
         "reverse_ms": round(float(ms[1]), 1), "recompute_ms": round(float(ms[2]), 1),
         "adjoint_kernels_ms": round(float(ms[3]), 1),
-        "adjoint_over_forward": round(float(ms[1]) / t_plain, 3),
+        "adjoint_over_forward": round(float(ms[1]) / min(t_plain, float(ms[0])), 3),
         "fd_step": h, "fd": fd,
         "fd_max_rel_err": max((x["rel_err"] for x in fd), default=None),
+    }
+
+
+def adjoint_cavity_leg(hip, hiptime, n, steps, every, navg, fd_dirs):
+    """The adjoint at the size the cost target is stated for: the LES cavity
+    Re 1000 of the headline workload on n^3 (seeded noise state), FP32
+    sweeps + FP64 refine forward; control a body force (Gaussian blob,
+    amplitude 0.1, width 0.1, seeded centre); J = mean kinetic energy over
+    the last navg of `steps` steps; checkpoints every `every` steps.  Cost
+    of the checkpointed reverse pass against the faster of a plain forward
+    of the same solver and the adjoint run's own forward pass; central differences of the FP64 forward along seeded random
+    directions with step h = 2e-4.  In the seeded noise state some faces'
+    advecting velocity is near zero, and a perturbation flips QUICK's upwind
+    choice there (discretisation_QUICK_centre.f90:18-102) on a fraction of
+    faces that grows with h: the central difference's error is then O(h)
+    (2.6e-4 at h = 0.026).  At 2e-4 J still changes by about 1e-8 of
+    itself, far above its rounding."""
+    from ntustcfdlab_amd.grid import Grid, bc_codes, gaussian_blob
+    w = workload(n)
+    g = Grid(n, n, n)
+    s = hip.ProjectionSolver(g, bc_codes())
+
+    def configure(sor):
+        s.configure(dt=w["dt"], nu=1.0 / w["Re"], Cs=w["Cs"], LES=w["LES"], scheme=w["scheme"],
+                    omega=w["omega"], zeta=w["zeta"], itmax=w["itmax"], sor=sor, lx=1.0,
+                    force_every=0)
+    try:
+        configure(hip.SOR_MIXED)
+        u, v, ww, p, eta = initial_state(g, w, s)
+        state = [u, v, ww, p]
+        rng = np.random.default_rng(96)
+        f = gaussian_blob(g, 0.1, 0.1, rng.uniform(0.3, 0.7, 3))
+
+        def plain(fu):
+            s.set_body_force(*fu)
+            s.upload(*[x.copy(order="F") for x in state], eta)
+            e0, e1 = hiptime.Event(), hiptime.Event()
+            hiptime.device_synchronize()
+            e0.record(s.stream())
+            tot = 0.0
+            for k in range(steps):
+                s.step(1)
+                if k >= steps - navg:
+                    tot += s.kinetic_energy()
+            e1.record(s.stream())
+            return tot / navg, e0.elapsed_ms(e1)
+
+        J0, t_plain = plain(f)
+        s.adjoint_objective(1.0, 0.0, 0.0)
+        s.set_body_force(*f)
+        s.upload(*[x.copy(order="F") for x in state], eta)
+        J, ik, ms = s.adjoint_run(steps, every, navg)
+        gf = s.adjoint_gradient()
+        configure(hip.SOR_REDBLACK)
+        J64, t64 = plain(f)
+        frng = np.random.default_rng(97)
+        fd = []
+        for _ in range(fd_dirs):
+            d = [g.halo() for _ in range(3)]
+            for x in d:
+                x[2:-2, 2:-2, 2:-2] = frng.standard_normal((n, n, n))
+            adv = float(sum(np.vdot(x, y) for x, y in zip(gf, d)))
+            h = 2e-4
+            jp, _ = plain([x + h * y for x, y in zip(f, d)])
+            jm, _ = plain([x - h * y for x, y in zip(f, d)])
+            fdv = (jp - jm) / (2 * h)
+            fd.append({"step": h, "fd": fdv, "adjoint": adv, "rel_err": abs(fdv - adv) / abs(adv)})
+    finally:
+        s.close()
+    return {
+        "workload": (f"LES cavity Re=1000 {n}^3, QUICK, Cs=0.1, dt=5e-4, FP32 sweeps + FP64 "
+                     f"refine (zeta=1e-10, itmax=3000); body force Gaussian amp 0.1 width 0.1; "
+                     f"J = mean KE over the last {navg} of {steps} steps; checkpoint every "
+                     f"{every}; finite differences of the FP64 forward"),
+        "J": J, "J_plain_forward": J0, "J_fp64_forward": J64,
+        "sweeps_per_step": float(np.mean(ik)),
+        "forward_ms": round(float(ms[0]), 1), "plain_forward_ms": round(t_plain, 1),
+        "fp64_forward_ms": round(t64, 1),
+        "reverse_ms": round(float(ms[1]), 1), "recompute_ms": round(float(ms[2]), 1),
+        "adjoint_kernels_ms": round(float(ms[3]), 1),
+        "adjoint_over_forward": round(float(ms[1]) / min(t_plain, float(ms[0])), 3),
+        "fd": fd, "fd_max_rel_err": max((x["rel_err"] for x in fd), default=None),
     }
 
 
@@ -386,8 +468,8 @@ def airfoil_adjoint_leg(hip, hiptime, steps, every, navg):
     """The adjoint of the large configuration (airfoil_case, FP32 sweeps +
     FP64 refine): J = mean C_D over the last navg of `steps` steps from the
     state two steps after the inflow, checkpoints every `every` steps.
-    Outputs: the cost of the checkpointed reverse pass against a plain
-    forward of the same steps; dJ/df for the actuator strip against the
+    Outputs: the cost of the checkpointed reverse pass against the faster of
+    a plain forward of the same steps and the adjoint run's own forward; dJ/df for the actuator strip against the
     central difference of the forward along it (the FP32-sweep one, step for
     a 1e-4 relative change of J, and the FP64 one, step for 1e-6); dJ/dAOA by
     the chain rule through the cast ETA (ETA at AOA +- 0.01 degrees: the
@@ -485,7 +567,7 @@ def airfoil_adjoint_leg(hip, hiptime, steps, every, navg):
         "forward_ms": round(float(ms[0]), 1), "plain_forward_ms": round(t_plain, 1),
         "reverse_ms": round(float(ms[1]), 1), "recompute_ms": round(float(ms[2]), 1),
         "adjoint_kernels_ms": round(float(ms[3]), 1),
-        "adjoint_over_forward": round(float(ms[1]) / t_plain, 3),
+        "adjoint_over_forward": round(float(ms[1]) / min(t_plain, float(ms[0])), 3),
         "fp64_sweeps_per_step": float(np.mean(ik64)),
         "max_sweep_count_difference_fp64_vs_fp32": int(np.max(np.abs(np.array(ik64) - ik))),
         "actuator": {"cells": ncells, "adjoint": ad,
@@ -658,6 +740,8 @@ def main():
     ap.add_argument("--airfoil-adjoint-steps", type=int, default=8,
                     help="steps of the large airfoil adjoint leg (the configuration's "
                          "horizon is 256; checkpoints every ~sqrt(steps))")
+    ap.add_argument("--only-adjoint-256", action="store_true",
+                    help="run the 256^3 cavity adjoint leg alone")
     ap.add_argument("--only-airfoil-c3", action="store_true",
                     help="run the moderate immersed-boundary airfoil leg alone")
     ap.add_argument("--only-airfoil-adjoint", action="store_true",
@@ -683,6 +767,10 @@ def main():
         return
     n_aa = max(a.airfoil_adjoint_steps, 2)
     aa_args = (n_aa, max(1, int(round(np.sqrt(n_aa)))), max(1, n_aa // 2))
+    if a.only_adjoint_256:
+        print(json.dumps({"adjoint_les_cavity_256":
+                          adjoint_cavity_leg(hip, hiptime, 256, 4, 2, 2, 1)}))
+        return
     if a.only_airfoil_c3:
         print(json.dumps({"airfoil_c3_64x192x384": airfoil_c3_leg(hip, hiptime, 6, 2)}))
         return
@@ -734,6 +822,9 @@ def main():
         print("# adjoint", file=sys.stderr, flush=True)
         out["adjoint"] = adjoint_leg(hip, hiptime, a.adjoint_size, a.adjoint_steps,
                                      a.fd_directions)
+        if not a.no_256 and not a.only:
+            print("# adjoint 256^3", file=sys.stderr, flush=True)
+            out["adjoint_les_cavity_256"] = adjoint_cavity_leg(hip, hiptime, 256, 4, 2, 2, 1)
     if not a.no_cpu_baseline:
         print("# cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(g, w, state, sweeps)
