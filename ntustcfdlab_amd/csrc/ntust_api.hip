@@ -871,14 +871,11 @@ int ntust_finalize(void)
     return 0;
 }
 
-int ntust_init(int nx, int ny, int nz, const double *iDx, const double *Dxs,
-               const double *iDy, const double *Dys, const double *iDz, const double *Dzs)
+// the device part of ntust_init; on failure the caller releases what it made
+static int init_device(int nx, int ny, int nz, const double *iDx, const double *Dxs,
+                       const double *iDy, const double *Dys, const double *iDz,
+                       const double *Dzs)
 {
-    if (nx < 2 || ny < 2 || nz < 2)
-        return fail(NTUST_E_ARG, "grid %dx%dx%d: every extent must be >= 2", nx, ny, nz);
-    if (!iDx || !Dxs || !iDy || !Dys || !iDz || !Dzs)
-        return fail(NTUST_E_ARG, "metric array is NULL");
-    if (C.ready) ntust_finalize();
     C.nx = nx; C.ny = ny; C.nz = nz;
     C.nh = (long)(nx + 4) * (ny + 4) * (nz + 4);
     C.ni = (long)nx * ny * nz;
@@ -923,6 +920,25 @@ int ntust_init(int nx, int ny, int nz, const double *iDx, const double *Dxs,
     C.bc.c[11] = BC_DIRICHLET;
     C.ready = true;
     return 0;
+}
+
+int ntust_init(int nx, int ny, int nz, const double *iDx, const double *Dxs,
+               const double *iDy, const double *Dys, const double *iDz, const double *Dzs)
+{
+    if (nx < 2 || ny < 2 || nz < 2)
+        return fail(NTUST_E_ARG, "grid %dx%dx%d: every extent must be >= 2", nx, ny, nz);
+    if (!iDx || !Dxs || !iDy || !Dys || !iDz || !Dzs)
+        return fail(NTUST_E_ARG, "metric array is NULL");
+    if (C.ready) ntust_finalize();
+    const int rc = init_device(nx, ny, nz, iDx, Dxs, iDy, Dys, iDz, Dzs);
+    if (rc < 0) {
+        // release whatever was created, keeping the original message
+        char msg[sizeof g_err];
+        memcpy(msg, g_err, sizeof msg);
+        ntust_finalize();
+        memcpy(g_err, msg, sizeof msg);
+    }
+    return rc;
 }
 
 int ntust_set_bc(const int *codes)

@@ -48,3 +48,28 @@ def test_calls_before_init_report_not_init():
     rc = lib.ntust_step(1, None, None, None, None)
     assert rc == -1
     assert b"ntust_init" in lib.ntust_last_error()
+
+
+def test_init_rejects_bad_arguments_before_touching_the_device():
+    """Argument errors are reported as NTUST_E_ARG with a message, before any
+    HIP call (so they behave the same with and without a GPU)."""
+    import numpy as np
+    from ntustcfdlab_amd import hip
+    lib = hip.load_library()
+    m = np.ones(16)
+    p = m.ctypes.data_as(hip.ct.POINTER(hip.ct.c_double))
+    for n in ((1, 4, 4), (4, 0, 4), (4, 4, -3)):
+        assert lib.ntust_init(*n, p, p, p, p, p, p) == -2
+        assert b"must be >= 2" in lib.ntust_last_error()
+    assert lib.ntust_init(4, 4, 4, p, p, None, p, p, p) == -2
+    assert b"NULL" in lib.ntust_last_error()
+
+
+def test_new_entry_points_report_not_init():
+    from ntustcfdlab_amd import hip
+    lib = hip.load_library()
+    lib.ntust_finalize()
+    assert lib.ntust_adjoint_objective(1.0, 0.0, 0.0, 0) == -1
+    assert lib.ntust_adjoint_eta_gradient(None) == -1
+    assert lib.ntust_adjoint_run(2, 1, 1, None, None, None) == -1
+    assert b"ntust_init" in lib.ntust_last_error()
