@@ -766,7 +766,8 @@ def main():
                                                  a.adjoint_steps, a.fd_directions)}))
         return
     n_aa = max(a.airfoil_adjoint_steps, 2)
-    aa_args = (n_aa, max(1, int(round(np.sqrt(n_aa)))), max(1, n_aa // 2))
+    # the configuration's objective: mean C_D over the whole horizon
+    aa_args = (n_aa, max(1, int(round(np.sqrt(n_aa)))), n_aa)
     if a.only_adjoint_256:
         print(json.dumps({"adjoint_les_cavity_256":
                           adjoint_cavity_leg(hip, hiptime, 256, 4, 2, 2, 1)}))
