@@ -464,7 +464,7 @@ def actuator_strip(g, aoa, seed):
     return fu, fv, fw, int(mask.sum()) * g.nx
 
 
-def airfoil_adjoint_leg(hip, hiptime, steps, every, navg):
+def airfoil_adjoint_leg(hip, hiptime, steps, every, navg, fd=True):
     """The adjoint of the large configuration (airfoil_case, FP32 sweeps +
     FP64 refine): J = mean C_D over the last navg of `steps` steps from the
     state two steps after the inflow, checkpoints every `every` steps.
@@ -534,6 +534,19 @@ def airfoil_adjoint_leg(hip, hiptime, steps, every, navg):
         gf = s.adjoint_gradient()
         ge = s.adjoint_eta_gradient()
         ad = float(sum(np.vdot(x, y) for x, y in zip(gf, d)))
+        if not fd:
+            return {"workload": (f"NACA 0012 at {aoa} deg, 512x384x128 clustered, Re 5000, "
+                                 f"LES Cs 0.1, dt 1e-4, FP32 sweeps + FP64 refine; J = mean C_D "
+                                 f"over {navg} of {steps} steps; checkpoint every {every}; no "
+                                 f"finite differences"),
+                    "J": J, "J_plain_forward": J0, "sweeps_per_step": float(np.mean(ik)),
+                    "forward_ms": round(float(ms[0]), 1), "plain_forward_ms": round(t_plain, 1),
+                    "reverse_ms": round(float(ms[1]), 1), "recompute_ms": round(float(ms[2]), 1),
+                    "adjoint_kernels_ms": round(float(ms[3]), 1),
+                    "adjoint_over_forward": round(float(ms[1]) / min(t_plain, float(ms[0])), 3),
+                    "actuator_gradient": ad,
+                    "dJ_dAOA_chain_rule_per_deg": float(np.vdot(
+                        ge, (cast(aoa + h_aoa) - cast(aoa - h_aoa)) / (2 * h_aoa)))}
         # central difference of the forward the adjoint differentiates (the
         # FP32-sweep one; step for a 1e-4 relative change of J, well above
         # its FP32 rounding)
@@ -744,6 +757,9 @@ def main():
                     help="run the 256^3 cavity adjoint leg alone")
     ap.add_argument("--only-airfoil-c3", action="store_true",
                     help="run the moderate immersed-boundary airfoil leg alone")
+    ap.add_argument("--no-airfoil-adjoint-fd", action="store_true",
+                    help="skip the finite differences of the large airfoil adjoint leg "
+                         "(for long horizons)")
     ap.add_argument("--only-airfoil-adjoint", action="store_true",
                     help="run the large airfoil adjoint leg alone and print its JSON")
     a = ap.parse_args()
@@ -777,7 +793,8 @@ def main():
         return
     if a.only_airfoil_adjoint:
         print(json.dumps({"airfoil_adjoint_512x384x128":
-                          airfoil_adjoint_leg(hip, hiptime, *aa_args)}))
+                          airfoil_adjoint_leg(hip, hiptime, *aa_args,
+                                              fd=not a.no_airfoil_adjoint_fd)}))
         return
 
     head_run = measure(hip, hiptime, a.size, a.steps, a.warmup, a.sweep, a.only, a.calibrate)
