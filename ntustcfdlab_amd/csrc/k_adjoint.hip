@@ -281,6 +281,13 @@ k_adj_project_b(KGrid g, const double *__restrict__ lus, const double *__restric
 // its own colour read: lambda_b = -Z, and a ghost adjacent to the boundary
 // cell P collects c_g Z_P.  No convergence test: the sweep count is the
 // forward's.
+//
+// Only the red half-sweeps accumulate.  The black sum follows from the red
+// one: the black update is z1 <- (1-omega) z1 + omega S(z0)/aP with S
+// linear, so summed over its k updates
+//   sum_{m<k} z1_m = S(sum_{m<=k} z0_m)/aP + (z1_0 - z1_k)/omega,
+// with z0_m the red values they read (k_adj_black_sum).  This saves the
+// read and write of a black sum plane per black half-sweep.
 
 __device__ __forceinline__ double cell_aP(const KGrid &g, const double *rx, const double *ry,
                                           const double *rz, int i, int j, int k)
@@ -344,6 +351,35 @@ k_adj_sweep(KGrid g, SplitGrid s, const double *__restrict__ rx,
     if (acc) Z_own[o] += zn;
 }
 
+// after the FP64 transposed sweeps (the black ones not accumulating): the
+// black sum from the red one, Z1 = S(Z0)/aP + (z1_0 - z1)/omega, with z1_0
+// the black cells' initial z, which k_adj_to_split stores in Z1
+__global__ void __launch_bounds__(256)
+k_adj_black_sum(KGrid g, SplitGrid s, const double *__restrict__ rx,
+                const double *__restrict__ ry, const double *__restrict__ rz,
+                const double *__restrict__ Z0, const double *__restrict__ z1,
+                double *__restrict__ Z1, double omega)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    const int i = 2 * q + ((1 + j + k) & 1);
+    if (i < 1 || i > g.nx || j > g.ny) return;
+    const long row = (long)s.mp * ((long)j + (long)s.ry * k);
+    const long o = row + q;
+    const long pl = (long)s.mp * s.ry;
+    const double ZW = Z0[row + ((i - 1) >> 1)], ZE = Z0[row + ((i + 1) >> 1)];
+    const double ZS = Z0[o - s.mp], ZN = Z0[o + s.mp];
+    const double ZB = Z0[o - pl], ZF = Z0[o + pl];
+    const double ax = g.iDx[i], ay = g.iDy[j], az = g.iDz[k];
+    const double sum = ay * az * (rx[i] * ZE + rx[i - 1] * ZW) +
+                       ax * az * (ry[j] * ZN + ry[j - 1] * ZS) +
+                       ax * ay * (rz[k] * ZF + rz[k - 1] * ZB);
+    const double aP = ay * az * (rx[i] + rx[i - 1]) + ax * az * (ry[j] + ry[j - 1]) +
+                      ax * ay * (rz[k] + rz[k - 1]);
+    Z1[o] = sum / aP + (Z1[o] - z1[o]) / omega;
+}
+
 // The transposed half-sweeps with FP32 arithmetic, for the adjoint of the
 // FP32-sweep solver.  The transposed half-sweep (1-omega) z + omega S(z)/aP
 // is affine in z like the forward one, so within a segment of half-sweeps
@@ -351,16 +387,18 @@ k_adj_sweep(KGrid g, SplitGrid s, const double *__restrict__ rx,
 // forward FP32 sweep's update with the segment's residual
 //   r = S(z0) - aP z0            (the forward residual with b = 0)
 //   e <- e + omega ((S(e) + r)/aP - e)
-// and Ef sums e after every accumulating half-sweep, so that at the end of
-// the segment Z += n z0 + Ef and z0 += e (k_adj_fold).  The FP32 rounding
-// is then relative to one segment's change of z, as in the forward.
-// Thread layout of the forward FP32 sweep: RB_KPT cells along k, XCD-slab
-// block order, loads first; no convergence maxima (the count is fixed).
-template <int C>
+// and (ACC, the red half-sweeps) Ef sums e after every half-sweep, so that
+// at the end of the segment Z += n z0 + Ef and z0 += e (k_adj_fold); the
+// black sum is derived there from Ef (k_adj_fold_black).  The FP32
+// rounding is then relative to one segment's change of z, as in the
+// forward.  Thread layout of the forward FP32 sweep: RB_KPT cells along k,
+// XCD-slab block order, loads first; no convergence maxima (the count is
+// fixed).
+template <int C, bool ACC>
 __global__ void __launch_bounds__(256)
 k_adj_sweep32(KGrid g, SplitGrid s, FMetric fm, float *__restrict__ e_own,
               const float *__restrict__ e_oth, const float *__restrict__ r_own,
-              float *__restrict__ Ef_own, float omega, int acc)
+              float *__restrict__ Ef_own, float omega)
 {
     int bx, by, bz;
     xcd_block(bx, by, bz);
@@ -392,7 +430,7 @@ k_adj_sweep32(KGrid g, SplitGrid s, FMetric fm, float *__restrict__ e_own,
         vB[r] = e_oth[o - pl];
         vr[r] = r_own[o];
         vo[r] = e_own[o];
-        vf[r] = acc ? Ef_own[o] : 0.0f;
+        if (ACC) vf[r] = Ef_own[o];
     }
 #pragma unroll
     for (int r = 0; r < RB_KPT; r++) {
@@ -410,7 +448,7 @@ k_adj_sweep32(KGrid g, SplitGrid s, FMetric fm, float *__restrict__ e_own,
         const int i = 2 * q + 1 + (sh0 ^ (int)odd);
         if (jr <= g.ny && k0 + r <= g.nz && i <= g.nx) {
             e_own[at[r]] = nv;
-            if (acc) Ef_own[at[r]] = vf[r] + nv;
+            if (ACC) Ef_own[at[r]] = vf[r] + nv;
         }
     }
 }
@@ -428,6 +466,50 @@ k_adj_fold(long n, double nacc, double *__restrict__ z, float *__restrict__ e,
     z[x] = z0 + (double)e[x];
     e[x] = 0.0f;
     Ef[x] = 0.0f;
+}
+
+// end of a segment, black plane (before the red fold clears e0 and Ef0).
+// The black half-sweeps do not accumulate: summing the black update
+//   e1 <- (1-omega) e1 + omega (S(e0) + r1)/aP
+// over the segment's nb black updates (e0 = e1 = 0 at its start) gives
+//   sum e1 = (S(E) + nb r1)/aP - ((1-omega)/omega) e1_end,
+// E the sum of the red corrections they read: Ef0, less the last red
+// correction when the segment ends with a red update (ends_red).  tail is
+// (1-omega)/omega, plus 1 when the last black update does not count.
+// Then Z1 += nacc z1 + sum, z1 += e1, e1 = 0 on the interior black cells
+// (the ghost and padding entries are zero and stay so).  Arithmetic in
+// FP64 on the FP32 metric the sweeps use.
+__global__ void __launch_bounds__(256)
+k_adj_fold_black(KGrid g, SplitGrid s, FMetric fm, double nb, double nacc, double tail,
+                 int ends_red, const float *__restrict__ Ef0, const float *__restrict__ e0,
+                 const float *__restrict__ r1, float *__restrict__ e1,
+                 double *__restrict__ z1, double *__restrict__ Z1)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    const int i = 2 * q + ((1 + j + k) & 1);
+    if (i < 1 || i > g.nx || j > g.ny) return;
+    const long row = (long)s.mp * ((long)j + (long)s.ry * k);
+    const long o = row + q;
+    const long pl = (long)s.mp * s.ry;
+    const long at[6] = {row + ((i - 1) >> 1), row + ((i + 1) >> 1), o - s.mp, o + s.mp,
+                        o - pl, o + pl};
+    double E[6];
+#pragma unroll
+    for (int n = 0; n < 6; n++)
+        E[n] = (double)Ef0[at[n]] - (ends_red ? (double)e0[at[n]] : 0.0);
+    const double ax = fm.iDx[i], ay = fm.iDy[j], az = fm.iDz[k];
+    const double rW = fm.rx[i - 1], rE = fm.rx[i], rS = fm.ry[j - 1], rN = fm.ry[j];
+    const double rB = fm.rz[k - 1], rF = fm.rz[k];
+    const double sum = ay * az * (rE * E[1] + rW * E[0]) + ax * az * (rN * E[3] + rS * E[2]) +
+                       ax * ay * (rF * E[5] + rB * E[4]);
+    const double aP = ay * az * (rE + rW) + ax * az * (rN + rS) + ax * ay * (rF + rB);
+    const double e = (double)e1[o];
+    const double z = z1[o];
+    Z1[o] = Z1[o] + (nacc * z + ((sum + nb * (double)r1[o]) / aP - tail * e));
+    z1[o] = z + e;
+    e1[o] = 0.0f;
 }
 
 // colour planes -> lp (interior: adjoint of the initial guess, the previous
@@ -859,12 +941,17 @@ void launch_adj_pressure(const KGrid &g, const SplitGrid &s, const double *rx, c
             hipLaunchKernelGGL(k_adj_sweep<0>, grs, blk, 0, st, g, s, rx, ry, rz, z0, z1, Z0,
                                t == sweeps ? 1.0 : 1.0 - omega, omega, 1);
             hipLaunchKernelGGL(k_adj_sweep<1>, grs, blk, 0, st, g, s, rx, ry, rz, z1, z0, Z1,
-                               1.0 - omega, omega, t > 1 ? 1 : 0);
+                               1.0 - omega, omega, 0);
         }
+        if (sweeps > 0)
+            hipLaunchKernelGGL(k_adj_black_sum, grs, blk, 0, st, g, s, rx, ry, rz, Z0, z1, Z1,
+                               omega);
     } else if (sweeps > 0) {
         // the first red half-sweep (self factor 1) in FP64, then the
         // sequence black, (red, black) x (sweeps-1) -- the last black not
-        // accumulating -- in FP32 segments of mx->seg half-sweeps
+        // accumulating -- in FP32 segments of mx->seg half-sweeps (even:
+        // every segment starts with a black update, and only the last can
+        // end with one)
         hipLaunchKernelGGL(k_adj_sweep<0>, grs, blk, 0, st, g, s, rx, ry, rz, z0, z1, Z0, 1.0,
                            omega, 1);
         const int nq32 = (g.nx + 1) / 2;
@@ -875,22 +962,22 @@ void launch_adj_pressure(const KGrid &g, const SplitGrid &s, const double *rx, c
             const long h1 = h0 + mx->seg < nh ? h0 + mx->seg : nh;
             launch_rb_residual(g, s, rx, ry, rz, z0, z1, mx->bzero, mx->bzero, mx->r0, mx->r1,
                                nullptr, st);
-            double nacc[2] = {0.0, 0.0};
             for (long h = h0; h < h1; h++) {
-                const int c = (h & 1) ? 0 : 1;            // black, red, black, ...
-                const int acc = h == nh - 1 ? 0 : 1;      // the last black does not
-                if (c == 0)
-                    hipLaunchKernelGGL(k_adj_sweep32<0>, g32, blk, 0, st, g, s, mx->fm, mx->e0,
-                                       mx->e1, mx->r0, mx->f0, (float)omega, acc);
+                if (h & 1)
+                    hipLaunchKernelGGL((k_adj_sweep32<0, true>), g32, blk, 0, st, g, s, mx->fm,
+                                       mx->e0, mx->e1, mx->r0, mx->f0, (float)omega);
                 else
-                    hipLaunchKernelGGL(k_adj_sweep32<1>, g32, blk, 0, st, g, s, mx->fm, mx->e1,
-                                       mx->e0, mx->r1, mx->f1, (float)omega, acc);
-                nacc[c] += acc;
+                    hipLaunchKernelGGL((k_adj_sweep32<1, false>), g32, blk, 0, st, g, s, mx->fm,
+                                       mx->e1, mx->e0, mx->r1, nullptr, (float)omega);
             }
-            hipLaunchKernelGGL(k_adj_fold, gf, dim3(256), 0, st, s.plane, nacc[0], z0, mx->e0,
+            const long nb = (h1 - h0 + 1) / 2, nr = (h1 - h0) / 2;
+            const bool last = h1 == nh;   // ends with the black update that does not count
+            hipLaunchKernelGGL(k_adj_fold_black, grs, blk, 0, st, g, s, mx->fm, (double)nb,
+                               (double)(last ? nb - 1 : nb),
+                               (1.0 - omega) / omega + (last ? 1.0 : 0.0), last ? 0 : 1, mx->f0,
+                               mx->e0, mx->r1, mx->e1, z1, Z1);
+            hipLaunchKernelGGL(k_adj_fold, gf, dim3(256), 0, st, s.plane, (double)nr, z0, mx->e0,
                                mx->f0, Z0);
-            hipLaunchKernelGGL(k_adj_fold, gf, dim3(256), 0, st, s.plane, nacc[1], z1, mx->e1,
-                               mx->f1, Z1);
         }
     }
     if (sweeps > 0) {

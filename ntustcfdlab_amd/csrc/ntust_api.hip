@@ -717,9 +717,9 @@ int adjoint_of_step(const AdjBufs &a, const double *eta, const double *filt, con
     const bool mixed = q.sor == NTUST_SOR_MIXED;
     if (mixed) {
         const size_t nf = (size_t)C.sg.plane / 2 + 1;   // float planes in double buffers
-        const char *fn[6] = {"adj_e0", "adj_e1", "adj_r0", "adj_r1", "adj_f0", "adj_f1"};
-        float **fp[6] = {&mx.e0, &mx.e1, &mx.r0, &mx.r1, &mx.f0, &mx.f1};
-        for (int k = 0; k < 6; k++) {
+        const char *fn[5] = {"adj_e0", "adj_e1", "adj_r0", "adj_r1", "adj_f0"};
+        float **fp[5] = {&mx.e0, &mx.e1, &mx.r0, &mx.r1, &mx.f0};
+        for (int k = 0; k < 5; k++) {
             double *d = buf(fn[k], nf, true);
             if (!d) return fail(NTUST_E_ALLOC, "device allocation of %s failed", fn[k]);
             *fp[k] = (float *)d;

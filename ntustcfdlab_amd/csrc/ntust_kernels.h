@@ -305,7 +305,7 @@ void launch_adj_project(const KGrid &g, const double *lu, const double *lv, cons
 // the segment length in half-sweeps between FP64 folds
 struct AdjMixed {
     FMetric fm;
-    float *e0, *e1, *r0, *r1, *f0, *f1;
+    float *e0, *e1, *r0, *r1, *f0;   // f0: the red running sum
     const double *bzero;
     long seg;
 };
