@@ -386,8 +386,9 @@ int ntust_prof_read(double *ms, long long *launches);
 
 /* The timed kernel: returns 0 two-pass FP64, 1 fused FP64, 2 two-pass FP32,
  * 3 fused FP32 sweep, 4 tile FP64 sweep, 5 tile FP32 sweep; *bytes_per_cell
- * gets its algorithmic bytes per grid
- * cell and launch (read p and b, write p). */
+ * gets its algorithmic bytes per grid cell and launch, averaged over a
+ * sweep (FP64: read p and b, write p; FP32: read and write e, read r on the
+ * colour-0 cells, the colour-1 sweeps being shifted). */
 int ntust_prof_kernel(double *bytes_per_cell);
 
 /* Selects the red-black sweep kernel (NTUST_SWEEP_*); not a reference

@@ -389,7 +389,10 @@ k_adj_black_sum(KGrid g, SplitGrid s, const double *__restrict__ rx,
 //   e <- e + omega ((S(e) + r)/aP - e)
 // and (ACC, the red half-sweeps) Ef sums e after every half-sweep, so that
 // at the end of the segment Z += n z0 + Ef and z0 += e (k_adj_fold); the
-// black sum is derived there from Ef (k_adj_fold_black).  The FP32
+// black sum is derived there from Ef (k_adj_fold_black).  The black
+// correction runs shifted as in the forward, e1' = e1 - s1 with s1 = r1/aP
+// in the r1 plane and the red residual r0 + S(s1) (launch_rb_residual with
+// the shift), so the black half-sweeps read no residual.  The FP32
 // rounding is then relative to one segment's change of z, as in the
 // forward.  Thread layout of the forward FP32 sweep: RB_KPT cells along k,
 // XCD-slab block order, loads first; no convergence maxima (the count is
@@ -428,7 +431,7 @@ k_adj_sweep32(KGrid g, SplitGrid s, FMetric fm, float *__restrict__ e_own,
         vS[r] = e_oth[o - mp];
         vF[r] = e_oth[o + pl];
         vB[r] = e_oth[o - pl];
-        vr[r] = r_own[o];
+        vr[r] = C == 0 ? r_own[o] : 0.0f;   // colour 1: shifted, no right-hand side
         vo[r] = e_own[o];
         if (ACC) vf[r] = Ef_own[o];
     }
@@ -469,20 +472,21 @@ k_adj_fold(long n, double nacc, double *__restrict__ z, float *__restrict__ e,
 }
 
 // end of a segment, black plane (before the red fold clears e0 and Ef0).
-// The black half-sweeps do not accumulate: summing the black update
-//   e1 <- (1-omega) e1 + omega (S(e0) + r1)/aP
-// over the segment's nb black updates (e0 = e1 = 0 at its start) gives
-//   sum e1 = (S(E) + nb r1)/aP - ((1-omega)/omega) e1_end,
+// The black half-sweeps do not accumulate: summing the black update, in
+// the shifted correction e1' = e1 - s1 (s1 in the r1 plane),
+//   e1' <- (1-omega) e1' + omega S(e0)/aP,   e1' = -s1 at the start,
+// over the segment's nb black updates (e0 = 0 at its start) gives
+//   sum e1 = S(E)/aP + nb s1 - ((1-omega)/omega) e1_end,
 // E the sum of the red corrections they read: Ef0, less the last red
 // correction when the segment ends with a red update (ends_red).  tail is
 // (1-omega)/omega, plus 1 when the last black update does not count.
-// Then Z1 += nacc z1 + sum, z1 += e1, e1 = 0 on the interior black cells
-// (the ghost and padding entries are zero and stay so).  Arithmetic in
-// FP64 on the FP32 metric the sweeps use.
+// Then Z1 += nacc z1 + sum, z1 += e1_end = e1' + s1, e1' = 0 on the
+// interior black cells (the ghost and padding entries are zero and stay
+// so).  Arithmetic in FP64 on the FP32 metric the sweeps use.
 __global__ void __launch_bounds__(256)
 k_adj_fold_black(KGrid g, SplitGrid s, FMetric fm, double nb, double nacc, double tail,
                  int ends_red, const float *__restrict__ Ef0, const float *__restrict__ e0,
-                 const float *__restrict__ r1, float *__restrict__ e1,
+                 const float *__restrict__ s1, float *__restrict__ e1,
                  double *__restrict__ z1, double *__restrict__ Z1)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -505,9 +509,10 @@ k_adj_fold_black(KGrid g, SplitGrid s, FMetric fm, double nb, double nacc, doubl
     const double sum = ay * az * (rE * E[1] + rW * E[0]) + ax * az * (rN * E[3] + rS * E[2]) +
                        ax * ay * (rF * E[5] + rB * E[4]);
     const double aP = ay * az * (rE + rW) + ax * az * (rN + rS) + ax * ay * (rF + rB);
-    const double e = (double)e1[o];
+    const double sh = (double)s1[o];
+    const double e = (double)e1[o] + sh;
     const double z = z1[o];
-    Z1[o] = Z1[o] + (nacc * z + ((sum + nb * (double)r1[o]) / aP - tail * e));
+    Z1[o] = Z1[o] + (nacc * z + ((sum / aP + nb * sh) - tail * e));
     z1[o] = z + e;
     e1[o] = 0.0f;
 }
@@ -961,7 +966,7 @@ void launch_adj_pressure(const KGrid &g, const SplitGrid &s, const double *rx, c
         for (long h0 = 0; h0 < nh; h0 += mx->seg) {
             const long h1 = h0 + mx->seg < nh ? h0 + mx->seg : nh;
             launch_rb_residual(g, s, rx, ry, rz, z0, z1, mx->bzero, mx->bzero, mx->r0, mx->r1,
-                               nullptr, st);
+                               mx->e1, nullptr, nullptr, st);
             for (long h = h0; h < h1; h++) {
                 if (h & 1)
                     hipLaunchKernelGGL((k_adj_sweep32<0, true>), g32, blk, 0, st, g, s, mx->fm,

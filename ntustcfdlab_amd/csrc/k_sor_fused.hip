@@ -62,7 +62,8 @@ __device__ __forceinline__ T point_update(T pW, T pE, T pS, T pN, T pB, T pF, T 
 }
 
 // MIXED = false: p and b, d = (sum - b)/aP - p   (k_rb_sweep)
-// MIXED = true:  e and r, d = (sum + r)/aP - e   (k_rb_sweep32)
+// MIXED = true:  e and r, d = (sum + r)/aP - e   (k_rb_sweep32), colour 1
+//                shifted as there: r = 0, the r1 plane is not read
 //
 // Thread t < (TX/2+1)(TY+2) owns the cell pair (x0 + 2p, x0 + 2p + 1) of row
 // j0 - 1 + jr (p = t mod (TX/2+1), jr = t div (TX/2+1)) on every plane: the
@@ -245,7 +246,7 @@ k_rb_sweep_stream(KGrid g, SplitGrid s, SweepMetric<T> m, const T *__restrict__ 
                 out1[gb + pl * (unsigned)kb] = nv;
             }
         }
-        cB = rhs1[gb + kcl(r + 1)];
+        cB = MIXED ? (T)0 : rhs1[gb + kcl(r + 1)];
         sb = sb + 1 == NSLOT ? 0 : sb + 1;
     };
     // both steps of a pair always run (a step past k1+1 stores nothing: its
@@ -320,12 +321,13 @@ void launch_rb_sweep32_fused(const KGrid &g, const SplitGrid &s, const FMetric &
                        dirty);
 }
 
-// p0 += e on the interior and e = 0, for the copy of e the last sweep wrote
-// (copy ik&1); skipped when no sweep of the segment ran
+// p += e on the interior and e = 0, for the copy of e the last sweep wrote
+// (copy ik&1); the shifted colour-1 correction is e1' + s1 (s1 in the r1
+// plane); skipped when no sweep of the segment ran
 __global__ void __launch_bounds__(256)
 k_rb_absorb_pp(KGrid g, SplitGrid s, double *__restrict__ p0, double *__restrict__ p1,
-               float *ea0, float *ea1, float *eb0, float *eb1, const int *dirty,
-               const SorControl *ctl)
+               float *ea0, float *ea1, float *eb0, float *eb1, const float *__restrict__ s1,
+               const int *dirty, const SorControl *ctl)
 {
     if (!load_flag(dirty)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
@@ -336,7 +338,7 @@ k_rb_absorb_pp(KGrid g, SplitGrid s, double *__restrict__ p0, double *__restrict
     float *e0 = second ? eb0 : ea0, *e1 = second ? eb1 : ea1;
     const long o = six(s, i, j, k);
     if ((i + j + k) & 1) {
-        p1[o] = p1[o] + (double)e1[o];
+        p1[o] = p1[o] + ((double)e1[o] + (double)s1[o]);
         e1[o] = 0.0f;
     } else {
         p0[o] = p0[o] + (double)e0[o];
@@ -345,12 +347,12 @@ k_rb_absorb_pp(KGrid g, SplitGrid s, double *__restrict__ p0, double *__restrict
 }
 
 void launch_rb_absorb_pp(const KGrid &g, const SplitGrid &s, double *p0, double *p1,
-                         float *ea0, float *ea1, float *eb0, float *eb1, const int *dirty,
-                         const SorControl *ctl, hipStream_t st)
+                         float *ea0, float *ea1, float *eb0, float *eb1, const float *s1,
+                         const int *dirty, const SorControl *ctl, hipStream_t st)
 {
     dim3 blk(64, 4), grd((g.nx + 63) / 64, (g.ny + 3) / 4, g.nz);
-    hipLaunchKernelGGL(k_rb_absorb_pp, grd, blk, 0, st, g, s, p0, p1, ea0, ea1, eb0, eb1, dirty,
-                       ctl);
+    hipLaunchKernelGGL(k_rb_absorb_pp, grd, blk, 0, st, g, s, p0, p1, ea0, ea1, eb0, eb1, s1,
+                       dirty, ctl);
 }
 
 }  // namespace ntust

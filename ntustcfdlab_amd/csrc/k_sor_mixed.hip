@@ -13,10 +13,19 @@
 // to the change max|eNew - e| = max|pNew - p| of each sweep, with the same
 // per-sweep gating as the FP64 kernels.
 //
-// Traffic: an FP32 half-sweep streams one colour of e and r and reads the
-// other colour of e: 8 B per cell of the grid (16 per sweep, FP64: 32); the
-// correction of a segment (absorb and residual, k_rb_correct) reads p, e, b
-// and writes p, e, r: 24 + 22 B/cell over its two colour launches.
+// Colour 1 runs shifted: with s1 = r1/aP1 its correction is carried as
+// e1' = e1 - s1, which turns its update into the homogeneous one,
+//   e1' <- (1-omega) e1' + omega S(e0)/aP1,
+// and moves the constant into the colour-0 residual, r0' = r0 + S(s1)
+// (k_rb_correct; the segment starts from e1' = -s1, e1 = 0).
+// The colour-1 half-sweeps then read no residual.  The change of a sweep,
+// e1'New - e1' = e1New - e1, is the same.
+//
+// Traffic: an FP32 half-sweep streams one colour of e (and of r for colour
+// 0) and reads the other colour of e: 8 + 6 B per cell of the grid per
+// sweep (FP64: 32); the correction of a segment (absorb and residual,
+// k_rb_correct) reads p, e, r1, b and writes p, e, r: 26 + 24 B/cell over
+// its two colour launches.
 #include "ntust_kernels.h"
 
 namespace ntust {
@@ -71,7 +80,7 @@ k_rb_sweep32(KGrid g, SplitGrid s, FMetric fm, float *__restrict__ e_own,
         vS[r] = e_oth[o - mp];
         vF[r] = e_oth[o + pl];
         vB[r] = e_oth[o - pl];
-        vr[r] = r_own[o];
+        vr[r] = C == 0 ? r_own[o] : 0.0f;   // colour 1: shifted, no right-hand side
         vo[r] = e_own[o];
     }
     float pcm = 0.0f, nv[RB_KPT];
@@ -133,29 +142,41 @@ k_rb_sweep32(KGrid g, SplitGrid s, FMetric fm, float *__restrict__ e_own,
 }
 
 // The FP64 correction of a segment in colour-split form, two launches
-// (colour 0, then colour 1): each cell absorbs its correction, p = p + e,
-// e = 0, and gets its residual
+// (colour 1, then colour 0): each cell absorbs its correction, p = p + e,
+// and gets its residual
 //   r = (sum_nb a_nb p_nb - b) - aP p
 // from the absorbed values of its neighbours, which are all of the other
-// colour.  During the colour-0 launch the colour-1 cells are not yet
+// colour.  During the colour-1 launch the colour-0 cells are not yet
 // absorbed, so their value is formed on the fly as p + e (the same
-// operation the absorb performs, so the same bits); during the colour-1
-// launch the colour-0 cells are already absorbed and read as they are.
+// operation the absorb performs, so the same bits); during the colour-0
+// launch the colour-1 cells are already absorbed and read as they are.
 // The result is bit for bit that of an absorb over the grid followed by the
-// residual.  ABSORB = false: the residual alone (p and e untouched; e may be
-// null).  Thread layout of the sweeps (RB_KPT cells along k, other
-// colour's W/E rows shared for the +-k neighbours, XCD-slab block order).
-// Traffic per cell of the grid: 24 B (colour 0) + 22 B (colour 1) against
-// 24 + 28 for the two whole-grid kernels.
+// residual.  ABSORB = false: the residual alone (p untouched).
+//
+// Colour 1 is shifted: its correction is e1' + s1 with s1 the scaled
+// residual the r1 plane holds; its launch leaves s1 = r1/aP1 in that plane
+// and e1' = -s1 (also in e_copy when not null).  Colour 0 adds S(s1) to its
+// residual, read as -e1' from the e1 plane, and leaves e0 = 0.
+//
+// Thread layout of the sweeps (RB_KPT cells along k, other colour's W/E
+// rows shared for the +-k neighbours, XCD-slab block order).  Traffic per
+// cell of the grid: 26 B (colour 1) + 24 B (colour 0).
 template <int C, bool ABSORB>
 __global__ void __launch_bounds__(256)
 k_rb_correct(KGrid g, SplitGrid s, const double *__restrict__ rx, const double *__restrict__ ry,
              const double *__restrict__ rz, double *__restrict__ p_own,
              const double *__restrict__ p_oth, float *__restrict__ e_own,
              const float *__restrict__ e_oth, const double *__restrict__ b_own,
-             float *__restrict__ r_own, const int *dirty)
+             float *__restrict__ r_own, float *__restrict__ e_copy, const int *dirty)
 {
     constexpr int KPT = RB_KPT;
+    // FLY: the other colour is formed as p + e; SSUM: the residual gains
+    // S(s1) = -S(e1'); LOADE: either reads the other colour's e; SOWN: the
+    // own cell is shifted
+    constexpr bool FLY = ABSORB && C == 1;
+    constexpr bool SSUM = C == 0;
+    constexpr bool LOADE = FLY || SSUM;
+    constexpr bool SOWN = C == 1;
     if (dirty && !load_flag(dirty)) return;
     int bx, by, bz;
     xcd_block(bx, by, bz);
@@ -170,7 +191,7 @@ k_rb_correct(KGrid g, SplitGrid s, const double *__restrict__ rx, const double *
     const double ay = g.iDy[j], rN = ry[j], rS = ry[j - 1];
     const unsigned mp = (unsigned)s.mp;
     const unsigned q1 = (unsigned)min(q, (g.nx + 1) / 2);
-    // Other-colour values: during the colour-0 launch they are formed as
+    // Other-colour values: during the first launch they are formed as
     // p + e, but only for interior cells -- the absorb touches interior
     // cells only, and a ghost entry (whose e is zero) is read as it is,
     // since p + 0 would turn a -0 into +0.  Colour c of pair x in row
@@ -178,11 +199,11 @@ k_rb_correct(KGrid g, SplitGrid s, const double *__restrict__ rx, const double *
     // first and unconditionally; the interior selection follows, so the
     // e loads are not serialised behind per-load branches.
     auto interior = [&](int x, int jj, int kk) -> bool {
-        const int i = 2 * x + (((jj + kk) & 1) ^ 1);
+        const int i = 2 * x + (((jj + kk) & 1) ^ (C ^ 1));
         return jj >= 1 && jj <= g.ny && kk >= 1 && kk <= g.nz && i >= 1 && i <= g.nx;
     };
     double vW[KPT + 2], vE[KPT + 2], vN[KPT], vS[KPT], vb[KPT], vp[KPT];
-    float eW[KPT + 2], eE[KPT + 2], eN[KPT], eS[KPT], ve[KPT];
+    float eW[KPT + 2], eE[KPT + 2], eN[KPT], eS[KPT], ve[KPT], vs[KPT];
     unsigned at[KPT];
 #pragma unroll
     for (int r = -1; r <= KPT; r++) {
@@ -190,7 +211,7 @@ k_rb_correct(KGrid g, SplitGrid s, const double *__restrict__ rx, const double *
         const unsigned row = mp * ((unsigned)j + (unsigned)s.ry * (unsigned)k);
         vW[r + 1] = p_oth[row + q1];
         vE[r + 1] = p_oth[row + q1 + 1];
-        if (ABSORB && C == 0) {
+        if (LOADE) {
             eW[r + 1] = e_oth[row + q1];
             eE[r + 1] = e_oth[row + q1 + 1];
         }
@@ -203,15 +224,16 @@ k_rb_correct(KGrid g, SplitGrid s, const double *__restrict__ rx, const double *
         at[r] = o;
         vN[r] = p_oth[o + mp];
         vS[r] = p_oth[o - mp];
-        if (ABSORB && C == 0) {
+        if (LOADE) {
             eN[r] = e_oth[o + mp];
             eS[r] = e_oth[o - mp];
         }
         vb[r] = b_own[o];
         vp[r] = p_own[o];
         ve[r] = ABSORB ? e_own[o] : 0.0f;
+        vs[r] = ABSORB && SOWN ? r_own[o] : 0.0f;
     }
-    if (ABSORB && C == 0) {
+    if (FLY) {
 #pragma unroll
         for (int r = -1; r <= KPT; r++) {
             const int k = min(k0 + r, g.nz + 1);
@@ -237,20 +259,35 @@ k_rb_correct(KGrid g, SplitGrid s, const double *__restrict__ rx, const double *
         const double az = g.iDz[k], rF = rz[k], rB = rz[k - 1];
         const double ayz = ay * az, axz = ax * az, axy = ax * ay;
         const double pF = sh ? vE[r + 2] : vW[r + 2], pB = sh ? vE[r] : vW[r];
-        const double own = ABSORB ? vp[r] + (double)ve[r] : vp[r];
+        const double own = !ABSORB ? vp[r]
+                           : SOWN  ? vp[r] + ((double)ve[r] + (double)vs[r])
+                                   : vp[r] + (double)ve[r];
         const double t1 = fma(rE, vE[r + 1], rW * vW[r + 1]);
         const double t2 = fma(rN, vN[r], rS * vS[r]);
         const double t3 = fma(rF, pF, rB * pB);
         const double sum = fma(ayz, t1, fma(axz, t2, axy * t3));
         const double aP = fma(ayz, rE + rW, fma(axz, rN + rS, axy * (rF + rB)));
-        const double res = (sum - vb[r]) - aP * own;
+        double res = (sum - vb[r]) - aP * own;
+        if (SSUM) {
+            // + S(s1), s1 = -e1' (zero on the ghosts)
+            const float fF = sh ? eE[r + 2] : eW[r + 2], fB = sh ? eE[r] : eW[r];
+            const double u1 = fma(rE, (double)eE[r + 1], rW * (double)eW[r + 1]);
+            const double u2 = fma(rN, (double)eN[r], rS * (double)eS[r]);
+            const double u3 = fma(rF, (double)fF, rB * (double)fB);
+            res -= fma(ayz, u1, fma(axz, u2, axy * u3));
+        }
         const int i = 2 * q + 1 + (int)sh;
         if (jr <= g.ny && k0 + r <= g.nz && i <= g.nx) {
-            if (ABSORB) {
-                p_own[at[r]] = own;
-                e_own[at[r]] = 0.0f;
+            if (ABSORB) p_own[at[r]] = own;
+            if (SOWN) {
+                const float sv = (float)(res / aP);
+                r_own[at[r]] = sv;
+                e_own[at[r]] = -sv;
+                if (e_copy) e_copy[at[r]] = -sv;
+            } else {
+                if (ABSORB) e_own[at[r]] = 0.0f;
+                r_own[at[r]] = (float)res;
             }
-            r_own[at[r]] = (float)res;
         }
     }
 }
@@ -282,20 +319,21 @@ void launch_rb_sweep32_spec(const KGrid &g, const SplitGrid &s, const FMetric &f
 
 void launch_rb_residual(const KGrid &g, const SplitGrid &s, const double *rx, const double *ry,
                         const double *rz, const double *p0, const double *p1, const double *b0,
-                        const double *b1, float *r0, float *r1, const int *dirty,
-                        hipStream_t st)
+                        const double *b1, float *r0, float *r1, float *e1_shift,
+                        float *e1_copy, const int *dirty, hipStream_t st)
 {
     const int nq = (g.nx + 1) / 2;
     dim3 blk(64, 4), grd((nq + 63) / 64, (g.ny + 3) / 4, (g.nz + RB_KPT - 1) / RB_KPT);
     // p is only read when ABSORB is false
     double *q0 = const_cast<double *>(p0), *q1 = const_cast<double *>(p1);
-    hipLaunchKernelGGL((k_rb_correct<0, false>), grd, blk, 0, st, g, s, rx, ry, rz, q0, p1,
-                       nullptr, nullptr, b0, r0, dirty);
     hipLaunchKernelGGL((k_rb_correct<1, false>), grd, blk, 0, st, g, s, rx, ry, rz, q1, p0,
-                       nullptr, nullptr, b1, r1, dirty);
+                       e1_shift, nullptr, b1, r1, e1_copy, dirty);
+    hipLaunchKernelGGL((k_rb_correct<0, false>), grd, blk, 0, st, g, s, rx, ry, rz, q0, p1,
+                       nullptr, e1_shift, b0, r0, nullptr, dirty);
 }
 
-// absorb + residual of a segment (k_rb_correct, both colours)
+// absorb + residual of a segment of the shifted sweeps (k_rb_correct,
+// colour 1 then colour 0)
 void launch_rb_correct(const KGrid &g, const SplitGrid &s, const double *rx, const double *ry,
                        const double *rz, double *p0, double *p1, float *e0, float *e1,
                        const double *b0, const double *b1, float *r0, float *r1,
@@ -303,10 +341,10 @@ void launch_rb_correct(const KGrid &g, const SplitGrid &s, const double *rx, con
 {
     const int nq = (g.nx + 1) / 2;
     dim3 blk(64, 4), grd((nq + 63) / 64, (g.ny + 3) / 4, (g.nz + RB_KPT - 1) / RB_KPT);
-    hipLaunchKernelGGL((k_rb_correct<0, true>), grd, blk, 0, st, g, s, rx, ry, rz, p0, p1, e0, e1,
-                       b0, r0, dirty);
-    hipLaunchKernelGGL((k_rb_correct<1, true>), grd, blk, 0, st, g, s, rx, ry, rz, p1, p0, e1, e0,
-                       b1, r1, dirty);
+    hipLaunchKernelGGL((k_rb_correct<1, true>), grd, blk, 0, st, g, s, rx, ry, rz, p1, p0,
+                       e1, e0, b1, r1, nullptr, dirty);
+    hipLaunchKernelGGL((k_rb_correct<0, true>), grd, blk, 0, st, g, s, rx, ry, rz, p0, p1,
+                       e0, e1, b0, r0, nullptr, dirty);
 }
 
 }  // namespace ntust

@@ -120,7 +120,7 @@ k_rb_sweep_tile(KGrid g, SplitGrid s, TileMetric<T> m, const T *__restrict__ in0
         vrr[kk] = rhs0[o];
     }
 #pragma unroll
-    for (int kk = 0; kk < TZ; kk++) vbr[kk] = rhs1[oo + kof(k0 + kk)];
+    for (int kk = 0; kk < TZ; kk++) vbr[kk] = MIXED ? (T)0 : rhs1[oo + kof(k0 + kk)];
     // the pair's x metrics (cells a = 2q and a+1), the row's y metrics
     const int q = q0 + rp;
     const int a = 2 * q, ac = min(max(a, 0), g.nx + 1);

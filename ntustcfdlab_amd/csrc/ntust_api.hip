@@ -327,7 +327,8 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
         HIPCHK(hipMemsetAsync(dirty, 0, nseg * sizeof(int), C.st));
         HIPCHK(hipMemsetAsync(E0, 0, np * sizeof(float), C.st));
         HIPCHK(hipMemsetAsync(E1, 0, np * sizeof(float), C.st));
-        launch_rb_residual(g, C.sg, rx, ry, rz, P0, P1, B0, B1, R0, R1, nullptr, C.st);
+        // colour 1 runs shifted (-s1 in both copies of e for the fused sweeps)
+        launch_rb_residual(g, C.sg, rx, ry, rz, P0, P1, B0, B1, R0, R1, E1, F1, nullptr, C.st);
         FMetric fm{C.fmetric[0] + 1, C.fmetric[1] + 1, C.fmetric[2] + 1,
                    C.fmetric[3] + 1, C.fmetric[4] + 1, C.fmetric[5] + 1};
         long c = 0;
@@ -358,7 +359,8 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
                 launch_rb_gather(g, C.sg, p, P0, P1, C.st);
                 HIPCHK(hipMemsetAsync(E0, 0, np * sizeof(float), C.st));
                 HIPCHK(hipMemsetAsync(E1, 0, np * sizeof(float), C.st));
-                launch_rb_residual(g, C.sg, rx, ry, rz, P0, P1, B0, B1, R0, R1, nullptr, C.st);
+                launch_rb_residual(g, C.sg, rx, ry, rz, P0, P1, B0, B1, R0, R1, E1, nullptr,
+                                   nullptr, C.st);
                 run(q + 1, false);
                 HIPCHK(hipGetLastError());
                 TRY(sync());
@@ -433,9 +435,10 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
                 }
                 if (!timed) HIPCHK(hipEventRecord(C.ev1, C.st));
                 if (fused) {
-                    launch_rb_absorb_pp(g, C.sg, P0, P1, E0, E1, F0, F1, dirty + c, C.ctl, C.st);
-                    launch_rb_residual(g, C.sg, rx, ry, rz, P0, P1, B0, B1, R0, R1, dirty + c,
-                                       C.st);
+                    launch_rb_absorb_pp(g, C.sg, P0, P1, E0, E1, F0, F1, R1, dirty + c, C.ctl,
+                                        C.st);
+                    launch_rb_residual(g, C.sg, rx, ry, rz, P0, P1, B0, B1, R0, R1, E1, F1,
+                                       dirty + c, C.st);
                 } else {
                     launch_rb_correct(g, C.sg, rx, ry, rz, P0, P1, E0, E1, B0, B1, R0, R1,
                                       dirty + c, C.st);
@@ -1786,12 +1789,14 @@ int ntust_set_sweep_variant(int variant)
 
 int ntust_prof_kernel(double *bytes_per_cell)
 {
-    // algorithmic bytes per grid cell of one timed launch: read p and b,
-    // write p per sweep (24 B FP64, 12 B FP32); a two-pass launch is half
+    // algorithmic bytes per grid cell of one timed launch: per sweep, read
+    // and write p and read b (24 B FP64); FP32: read and write e, read r on
+    // the colour-0 cells only (colour 1 runs shifted, with no right-hand
+    // side): 8 + 2 = 10 B.  A two-pass launch is half a sweep.
     const bool mixed = C.prm.sor == NTUST_SOR_MIXED;
     const bool tile = C.sweep_variant == NTUST_SWEEP_TILE;
     const bool fused = C.sweep_variant == NTUST_SWEEP_FUSED || tile;
-    if (bytes_per_cell) *bytes_per_cell = (mixed ? 12.0 : 24.0) / (fused ? 1.0 : 2.0);
+    if (bytes_per_cell) *bytes_per_cell = (mixed ? 10.0 : 24.0) / (fused ? 1.0 : 2.0);
     if (tile) return mixed ? 5 : 4;
     return mixed ? (fused ? 3 : 2) : (fused ? 1 : 0);
 }

@@ -245,13 +245,17 @@ void launch_rb_sweep32_fused(const KGrid &g, const SplitGrid &s, const FMetric &
                              double itmax, int t, double *slots, SorControl *ctl, int *dirty,
                              hipStream_t st);
 void launch_rb_absorb_pp(const KGrid &g, const SplitGrid &s, double *p0, double *p1,
-                         float *ea0, float *ea1, float *eb0, float *eb1, const int *dirty,
-                         const SorControl *ctl, hipStream_t st);
+                         float *ea0, float *ea1, float *eb0, float *eb1, const float *s1,
+                         const int *dirty, const SorControl *ctl, hipStream_t st);
+// the residual for the shifted colour-1 sweeps: r1 holds s1 = r1/aP1,
+// e1_shift (and e1_copy, the second copy of the fused sweeps, when not
+// null) = -s1, r0 gains S(s1)
 void launch_rb_residual(const KGrid &g, const SplitGrid &s, const double *rx, const double *ry,
                         const double *rz, const double *p0, const double *p1, const double *b0,
-                        const double *b1, float *r0, float *r1, const int *dirty,
-                        hipStream_t st);
-// absorb followed by the residual, in two colour launches (same bits)
+                        const double *b1, float *r0, float *r1, float *e1_shift,
+                        float *e1_copy, const int *dirty, hipStream_t st);
+// absorb followed by the residual, in two colour launches (same bits),
+// for the shifted colour-1 sweeps
 void launch_rb_correct(const KGrid &g, const SplitGrid &s, const double *rx, const double *ry,
                        const double *rz, double *p0, double *p1, float *e0, float *e1,
                        const double *b0, const double *b1, float *r0, float *r1,
