@@ -464,12 +464,21 @@ def actuator_strip(g, aoa, seed):
     return fu, fv, fw, int(mask.sum()) * g.nx
 
 
+# device memory left to the step's and the adjoint's working arrays (about
+# 40 halo arrays, 8.5 GB, at 512x384x128) when the checkpoint plan decides
+# whether every step's state fits
+ADJ_RESERVE_BYTES = 32e9
+
+
 def airfoil_adjoint_leg(hip, hiptime, steps, every, navg, fd=True):
     """The adjoint of the large configuration (airfoil_case, FP32 sweeps +
     FP64 refine): J = mean C_D over the last navg of `steps` steps from the
-    state two steps after the inflow, checkpoints every `every` steps.
-    Outputs: the cost of the checkpointed reverse pass against the faster of
-    a plain forward of the same steps and the adjoint run's own forward; dJ/df for the actuator strip against the
+    state two steps after the inflow, checkpoints every `every` steps (None:
+    the library's plan -- every step's state kept from the forward pass when
+    it fits in HBM, so the reverse pass recomputes nothing -- and, for
+    comparison, the reverse pass with ceil(sqrt(steps))-spaced checkpoints).
+    Outputs: the cost of the reverse pass against the faster of a plain
+    forward of the same steps and the adjoint run's own forward; dJ/df for the actuator strip against the
     central difference of the forward along it (the FP32-sweep one, step for
     a 1e-4 relative change of J, and the FP64 one, step for 1e-6); dJ/dAOA by
     the chain rule through the cast ETA (ETA at AOA +- 0.01 degrees: the
@@ -527,23 +536,48 @@ def airfoil_adjoint_leg(hip, hiptime, steps, every, navg, fd=True):
 
         eta0 = cast(aoa)
         J0, t_plain = plain(eta0, zero)
+        print(f"#   airfoil adjoint: plain forward {t_plain:.0f} ms", file=sys.stderr, flush=True)
         s.adjoint_objective(0.0, 1.0, 0.0, eta_gradient=True)
-        s.set_body_force(*zero)
-        s.upload(*[x.copy(order="F") for x in state], eta0)
-        J, ik, ms = s.adjoint_run(steps, every, navg)
+        sq = max(1, int(np.ceil(np.sqrt(steps))))
+        planned = every is None
+        store = None
+        if planned:
+            every, store = s.adjoint_checkpoint_plan(steps, ADJ_RESERVE_BYTES)
+
+        def adjoint(k):
+            s.set_body_force(*zero)
+            s.upload(*[x.copy(order="F") for x in state], eta0)
+            J, ik, ms = s.adjoint_run(steps, k, navg)
+            print(f"#   airfoil adjoint: checkpoints every {k}, reverse {ms[1]:.0f} ms",
+                  file=sys.stderr, flush=True)
+            return J, ik, ms
+
+        J, ik, ms = adjoint(every)
         gf = s.adjoint_gradient()
         ge = s.adjoint_eta_gradient()
         ad = float(sum(np.vdot(x, y) for x, y in zip(gf, d)))
+        cost = {"checkpoint_every": every, "planned": planned,
+                "store_GB": round(store / 1e9, 1) if store else None,
+                "forward_ms": round(float(ms[0]), 1), "plain_forward_ms": round(t_plain, 1),
+                "reverse_ms": round(float(ms[1]), 1), "recompute_ms": round(float(ms[2]), 1),
+                "adjoint_kernels_ms": round(float(ms[3]), 1),
+                "adjoint_over_forward": round(float(ms[1]) / min(t_plain, float(ms[0])), 3)}
+        if planned and every != sq:
+            # the same reverse pass with sqrt-spaced checkpoints (what the
+            # plan falls back to when the states do not fit)
+            J2, ik2, ms2 = adjoint(sq)
+            assert J2 == J and np.array_equal(ik2, ik)
+            cost["sqrt_checkpoints"] = {
+                "checkpoint_every": sq, "reverse_ms": round(float(ms2[1]), 1),
+                "recompute_ms": round(float(ms2[2]), 1),
+                "adjoint_over_forward": round(float(ms2[1]) / min(t_plain, float(ms2[0])), 3)}
         if not fd:
             return {"workload": (f"NACA 0012 at {aoa} deg, 512x384x128 clustered, Re 5000, "
                                  f"LES Cs 0.1, dt 1e-4, FP32 sweeps + FP64 refine; J = mean C_D "
                                  f"over {navg} of {steps} steps; checkpoint every {every}; no "
                                  f"finite differences"),
                     "J": J, "J_plain_forward": J0, "sweeps_per_step": float(np.mean(ik)),
-                    "forward_ms": round(float(ms[0]), 1), "plain_forward_ms": round(t_plain, 1),
-                    "reverse_ms": round(float(ms[1]), 1), "recompute_ms": round(float(ms[2]), 1),
-                    "adjoint_kernels_ms": round(float(ms[3]), 1),
-                    "adjoint_over_forward": round(float(ms[1]) / min(t_plain, float(ms[0])), 3),
+                    **cost,
                     "actuator_gradient": ad,
                     "dJ_dAOA_chain_rule_per_deg": float(np.vdot(
                         ge, (cast(aoa + h_aoa) - cast(aoa - h_aoa)) / (2 * h_aoa)))}
@@ -577,10 +611,7 @@ def airfoil_adjoint_leg(hip, hiptime, steps, every, navg, fd=True):
                      f"{every}"),
         "J": J, "J_plain_forward": J0, "J_fp64_forward": J64,
         "fp64_forward_ms": round(t64, 1), "sweeps_per_step": float(np.mean(ik)),
-        "forward_ms": round(float(ms[0]), 1), "plain_forward_ms": round(t_plain, 1),
-        "reverse_ms": round(float(ms[1]), 1), "recompute_ms": round(float(ms[2]), 1),
-        "adjoint_kernels_ms": round(float(ms[3]), 1),
-        "adjoint_over_forward": round(float(ms[1]) / min(t_plain, float(ms[0])), 3),
+        **cost,
         "fp64_sweeps_per_step": float(np.mean(ik64)),
         "max_sweep_count_difference_fp64_vs_fp32": int(np.max(np.abs(np.array(ik64) - ik))),
         "actuator": {"cells": ncells, "adjoint": ad,
@@ -752,7 +783,11 @@ def main():
                     help="run the adjoint leg alone and print its JSON")
     ap.add_argument("--airfoil-adjoint-steps", type=int, default=8,
                     help="steps of the large airfoil adjoint leg (the configuration's "
-                         "horizon is 256; checkpoints every ~sqrt(steps))")
+                         "horizon is 256)")
+    ap.add_argument("--airfoil-adjoint-every", type=int, default=None,
+                    help="checkpoint spacing of the airfoil adjoint (default: the library's "
+                         "plan, every step kept when the states fit in HBM, and the "
+                         "sqrt-spaced reverse pass for comparison)")
     ap.add_argument("--only-adjoint-256", action="store_true",
                     help="run the 256^3 cavity adjoint leg alone")
     ap.add_argument("--only-airfoil-c3", action="store_true",
@@ -783,7 +818,7 @@ def main():
         return
     n_aa = max(a.airfoil_adjoint_steps, 2)
     # the configuration's objective: mean C_D over the whole horizon
-    aa_args = (n_aa, max(1, int(round(np.sqrt(n_aa)))), n_aa)
+    aa_args = (n_aa, a.airfoil_adjoint_every, n_aa)
     if a.only_adjoint_256:
         print(json.dumps({"adjoint_les_cavity_256":
                           adjoint_cavity_leg(hip, hiptime, 256, 4, 2, 2, 1)}))

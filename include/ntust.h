@@ -372,6 +372,17 @@ int ntust_adjoint_eta_gradient(double *geta);
 int ntust_adjoint_run(int nsteps, int checkpoint_every, int navg, double *J, int *ik,
                       double *ms);
 
+/* Checkpoint spacing for ntust_adjoint_run (no counterpart in the
+ * reference), for the current grid, objective and ETA-gradient setting:
+ * *every = nsteps -- every step's state kept from the forward pass, no
+ * recomputation -- when the run's stores for that spacing fit in the
+ * device's free memory (plus what those stores already hold) with
+ * reserve_bytes left over; else ceil(sqrt(nsteps)).  *bytes (may be NULL)
+ * gets the stores' size for the chosen spacing.  Allocates nothing; the
+ * run's other buffers (the step's and the adjoint's working arrays) are the
+ * caller's to cover with reserve_bytes. */
+int ntust_adjoint_checkpoint_plan(int nsteps, double reserve_bytes, int *every, double *bytes);
+
 /* ---- measurement -------------------------------------------------------- */
 
 /* The library's HIP stream (hipStream_t), for event timing by the caller. */

@@ -1583,6 +1583,47 @@ int ntust_adjoint_step(int *ik)
     return sync();
 }
 
+// doubles of the per-run stores of ntust_adjoint_run for nsteps N and
+// checkpoint spacing K: the checkpoints, the velocities of one segment and
+// after it, the final state, and per step of a segment the line sums (drag
+// or lift objective) and u1, v1, w1 with dJ/dETA (ETA gradient)
+static double adjoint_store_doubles(int N, int K, bool force, bool want_eta)
+{
+    const double nh = (double)C.nh;
+    const double pz = (double)(C.nx + 2) * (C.ny + 2), fsz = 2 * pz + 2 * (double)(C.nx + 2);
+    const double nseg = (double)((N + K - 1) / K);
+    double d = nseg * 4 * nh + (K + 1.0) * 3 * nh + 4 * nh;
+    if (force) d += K * fsz;
+    if (want_eta) d += K * 3 * nh + nh;
+    return d;
+}
+
+int ntust_adjoint_checkpoint_plan(int nsteps, double reserve_bytes, int *every, double *bytes)
+{
+    NEED_INIT();
+    if (nsteps < 1) return fail(NTUST_E_ARG, "nsteps must be >= 1");
+    if (!every) return fail(NTUST_E_ARG, "NULL every");
+    if (!(reserve_bytes >= 0.0) || !std::isfinite(reserve_bytes))
+        return fail(NTUST_E_ARG, "reserve_bytes must be finite and >= 0");
+    const bool force = C.adj_w[1] != 0.0 || C.adj_w[2] != 0.0, want_eta = C.adj_eta;
+    size_t fr = 0, tot = 0;
+    HIPCHK(hipMemGetInfo(&fr, &tot));
+    // the run grows these stores in place (free, then allocate): what they
+    // hold now is available to it
+    double held = 0.0;
+    for (const auto &b : C.bufs)
+        if (b.name == "ckpt" || b.name == "adj_seg" || b.name == "adj_final" ||
+            b.name == "adj_fseg" || b.name == "adj_useg" || b.name == "adj_geta")
+            held += 8.0 * (double)b.n;
+    const double avail = (double)fr + held - reserve_bytes;
+    int K = nsteps;
+    if (8.0 * adjoint_store_doubles(nsteps, nsteps, force, want_eta) > avail)
+        K = (int)ceil(sqrt((double)nsteps));
+    *every = K;
+    if (bytes) *bytes = 8.0 * adjoint_store_doubles(nsteps, K, force, want_eta);
+    return 0;
+}
+
 int ntust_adjoint_run(int nsteps, int checkpoint_every, int navg, double *J, int *ik,
                       double *ms)
 {
@@ -1603,7 +1644,7 @@ int ntust_adjoint_run(int nsteps, int checkpoint_every, int navg, double *J, int
     // per step of a segment: the force integral's line sums FXz, FYz, FXy,
     // FYy (their zero borders are never written) and the projection's u1,
     // v1, w1, as the reverse step of a drag / lift objective and of the ETA
-    // gradient read them
+    // gradient read them (sizes as adjoint_store_doubles counts them)
     const size_t pz = (size_t)(C.nx + 2) * (C.ny + 2), fsz = 2 * pz + 2 * (size_t)(C.nx + 2);
     double *fseg = nullptr, *useg = nullptr, *geta = nullptr;
     if (force) {

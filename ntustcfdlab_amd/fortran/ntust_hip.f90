@@ -345,6 +345,15 @@ module ntust_hip
          import :: c_int, c_ptr
          type(c_ptr), value :: geta
       end function
+
+      integer(c_int) function ntust_adjoint_checkpoint_plan(nsteps, reserve_bytes, every, &
+            bytes) bind(C, name="ntust_adjoint_checkpoint_plan")
+         import :: c_int, c_double, c_ptr
+         integer(c_int), value :: nsteps
+         real(c_double), value :: reserve_bytes
+         integer(c_int), intent(out) :: every
+         type(c_ptr), value :: bytes
+      end function
    end interface
 
 contains

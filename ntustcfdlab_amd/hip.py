@@ -86,6 +86,7 @@ def load_library() -> ct.CDLL:
         "ntust_adjoint_run": [i, i, i, _D, _I, _D],
         "ntust_adjoint_objective": [d, d, d, i],
         "ntust_adjoint_eta_gradient": [_D],
+        "ntust_adjoint_checkpoint_plan": [i, d, _I, _D],
         "ntust_set_sweep_variant": [i],
         "ntust_poisson_coeffs": [_D, _I],
         "ntust_poisson_rhs": [_D, _D, _D, d, _D],
@@ -427,6 +428,16 @@ class ProjectionSolver:
         out = self.grid.halo()
         _check(self.lib.ntust_adjoint_eta_gradient(_p(out)))
         return out
+
+    def adjoint_checkpoint_plan(self, nsteps: int, reserve_bytes: float):
+        """(checkpoint spacing, bytes of the run's stores): nsteps when every
+        step's state fits in free device memory with reserve_bytes left,
+        else ceil(sqrt(nsteps))."""
+        k = ct.c_int()
+        b = ct.c_double()
+        _check(self.lib.ntust_adjoint_checkpoint_plan(int(nsteps), float(reserve_bytes),
+                                                      ct.byref(k), ct.byref(b)))
+        return k.value, b.value
 
     def adjoint_run(self, nsteps: int, checkpoint_every: int, navg: int):
         """Returns (J, per-step sweep counts, ms[forward, reverse, recompute, adjoint])."""

@@ -73,3 +73,6 @@ def test_new_entry_points_report_not_init():
     assert lib.ntust_adjoint_eta_gradient(None) == -1
     assert lib.ntust_adjoint_run(2, 1, 1, None, None, None) == -1
     assert b"ntust_init" in lib.ntust_last_error()
+    k = hip.ct.c_int()
+    assert lib.ntust_adjoint_checkpoint_plan(4, 0.0, hip.ct.byref(k), None) == -1
+    assert b"ntust_init" in lib.ntust_last_error()

@@ -277,6 +277,46 @@ def test_mixed_forward_adjoint_matches_fp64(n, itmax, hip_lib):
     assert max(np.abs(x - y).max() for x, y in zip(a[2], b[2])) <= 1e-5 * scale
 
 
+def test_checkpoint_plan(hip_lib):
+    """The spacing plan: every step kept when it fits (one segment, no
+    recomputation), ceil(sqrt(n)) when the reserve leaves no room; the
+    store sizes it reports; argument errors; and the planned spacing giving
+    the gradient of the square-root one."""
+    from ntustcfdlab_amd.hip import NtustError
+    g, state = cavity_setup(12, seed=9)
+    nh = (g.nx + 4) * (g.ny + 4) * (g.nz + 4)
+    fsz = 2 * (g.nx + 2) * (g.ny + 2) + 2 * (g.nx + 2)
+    f = gaussian_blob(g, 0.1, 0.1, (0.5, 0.5, 0.5))
+    s = ProjectionSolver(g, bc_codes())
+    try:
+        s.configure(dt=2e-3, nu=1.0 / 400.0, Cs=0.1, LES=1, scheme=QUICK_CENTRE, omega=1.5,
+                    zeta=0.0, itmax=20.0, sor=SOR_REDBLACK, force_every=0)
+        assert s.adjoint_checkpoint_plan(9, 1e9) == (9, 8.0 * (4 * nh + 10 * 3 * nh + 4 * nh))
+        assert s.adjoint_checkpoint_plan(9, 1e18) == (3, 8.0 * (3 * 4 * nh + 4 * 3 * nh + 4 * nh))
+        assert s.adjoint_checkpoint_plan(1, 1e18) == (1, 8.0 * (4 * nh + 2 * 3 * nh + 4 * nh))
+        s.adjoint_objective(0.0, 1.0, 0.0, eta_gradient=True)
+        k, nb = s.adjoint_checkpoint_plan(9, 1e9)
+        assert k == 9 and nb == 8.0 * (4 * nh + 10 * 3 * nh + 4 * nh + 9 * fsz + 9 * 3 * nh + nh)
+        s.adjoint_objective(1.0, 0.0, 0.0)
+        for bad in ((0, 0.0), (4, -1.0), (4, float("nan")), (4, float("inf"))):
+            with pytest.raises(NtustError):
+                s.adjoint_checkpoint_plan(*bad)
+        out = []
+        for every in (s.adjoint_checkpoint_plan(9, 1e9)[0], s.adjoint_checkpoint_plan(9, 1e18)[0]):
+            st = [a.copy(order="F") for a in state]
+            s.boundary_conditions(*st)
+            s.set_body_force(*f)
+            s.upload(*st, g.halo())
+            J, ik, _ = s.adjoint_run(9, every, 3)
+            out.append((J, ik, s.adjoint_gradient()))
+    finally:
+        s.close()
+    (J1, ik1, g1), (J2, ik2, g2) = out
+    assert J1 == J2 and np.array_equal(ik1, ik2)
+    scale = max(np.abs(x).max() for x in g2)
+    assert max(np.abs(x - y).max() for x, y in zip(g1, g2)) <= 1e-10 * scale
+
+
 def test_adjoint_rejects_lexicographic(hip_lib):
     from ntustcfdlab_amd.hip import SOR_LEXICOGRAPHIC, NtustError
     g = Grid(8, 8, 8)
