@@ -170,14 +170,22 @@ def adjoint_leg(hip, hiptime, n, steps, fd_dirs):
     }
 
 
+# device memory left to the step's and the adjoint's working arrays (about
+# 40 halo arrays, 8.5 GB, at 512x384x128) when the checkpoint plan decides
+# whether every step's state fits
+ADJ_RESERVE_BYTES = 32e9
+
+
 def adjoint_cavity_leg(hip, hiptime, n, steps, every, navg, fd_dirs):
     """The adjoint at the size the cost target is stated for: the LES cavity
     Re 1000 of the headline workload on n^3 (seeded noise state), FP32
     sweeps + FP64 refine forward; control a body force (Gaussian blob,
     amplitude 0.1, width 0.1, seeded centre); J = mean kinetic energy over
-    the last navg of `steps` steps; checkpoints every `every` steps.  Cost
-    of the checkpointed reverse pass against the faster of a plain forward
-    of the same solver and the adjoint run's own forward pass; central differences of the FP64 forward along seeded random
+    the last navg of `steps` steps; checkpoints by the library's plan (every
+    step's state kept when it fits in HBM), and, for comparison, every
+    `every` steps.  Cost of the reverse pass against the faster of a plain
+    forward of the same solver and the adjoint run's own forward pass;
+    central differences of the FP64 forward along seeded random
     directions with step h = 2e-4.  In the seeded noise state some faces'
     advecting velocity is near zero, and a perturbation flips QUICK's upwind
     choice there (discretisation_QUICK_centre.f90:18-102) on a fraction of
@@ -216,10 +224,23 @@ def adjoint_cavity_leg(hip, hiptime, n, steps, every, navg, fd_dirs):
 
         J0, t_plain = plain(f)
         s.adjoint_objective(1.0, 0.0, 0.0)
-        s.set_body_force(*f)
-        s.upload(*[x.copy(order="F") for x in state], eta)
-        J, ik, ms = s.adjoint_run(steps, every, navg)
+
+        def adjoint(k):
+            s.set_body_force(*f)
+            s.upload(*[x.copy(order="F") for x in state], eta)
+            return s.adjoint_run(steps, k, navg)
+
+        planned, store = s.adjoint_checkpoint_plan(steps, ADJ_RESERVE_BYTES)
+        J, ik, ms = adjoint(planned)
         gf = s.adjoint_gradient()
+        checkpointed = None
+        if every != planned:
+            J2, ik2, ms2 = adjoint(every)
+            assert J2 == J and np.array_equal(ik2, ik)
+            checkpointed = {
+                "checkpoint_every": every, "reverse_ms": round(float(ms2[1]), 1),
+                "recompute_ms": round(float(ms2[2]), 1),
+                "adjoint_over_forward": round(float(ms2[1]) / min(t_plain, float(ms2[0])), 3)}
         configure(hip.SOR_REDBLACK)
         J64, t64 = plain(f)
         frng = np.random.default_rng(97)
@@ -239,15 +260,18 @@ def adjoint_cavity_leg(hip, hiptime, n, steps, every, navg, fd_dirs):
     return {
         "workload": (f"LES cavity Re=1000 {n}^3, QUICK, Cs=0.1, dt=5e-4, FP32 sweeps + FP64 "
                      f"refine (zeta=1e-10, itmax=3000); body force Gaussian amp 0.1 width 0.1; "
-                     f"J = mean KE over the last {navg} of {steps} steps; checkpoint every "
-                     f"{every}; finite differences of the FP64 forward"),
+                     f"J = mean KE over the last {navg} of {steps} steps; checkpoints by the "
+                     f"plan ({planned}) and every {every}; finite differences of the FP64 "
+                     f"forward"),
         "J": J, "J_plain_forward": J0, "J_fp64_forward": J64,
         "sweeps_per_step": float(np.mean(ik)),
+        "checkpoint_every": planned, "store_GB": round(store / 1e9, 1),
         "forward_ms": round(float(ms[0]), 1), "plain_forward_ms": round(t_plain, 1),
         "fp64_forward_ms": round(t64, 1),
         "reverse_ms": round(float(ms[1]), 1), "recompute_ms": round(float(ms[2]), 1),
         "adjoint_kernels_ms": round(float(ms[3]), 1),
         "adjoint_over_forward": round(float(ms[1]) / min(t_plain, float(ms[0])), 3),
+        "checkpointed": checkpointed,
         "fd": fd, "fd_max_rel_err": max((x["rel_err"] for x in fd), default=None),
     }
 
@@ -462,12 +486,6 @@ def actuator_strip(g, aoa, seed):
     fv[2:-2, 2:-2, 2:-2] = -np.sin(a) * amp
     fw[2:-2, 2:-2, 2:-2] = np.cos(a) * amp
     return fu, fv, fw, int(mask.sum()) * g.nx
-
-
-# device memory left to the step's and the adjoint's working arrays (about
-# 40 halo arrays, 8.5 GB, at 512x384x128) when the checkpoint plan decides
-# whether every step's state fits
-ADJ_RESERVE_BYTES = 32e9
 
 
 def airfoil_adjoint_leg(hip, hiptime, steps, every, navg, fd=True):
