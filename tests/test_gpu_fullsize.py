@@ -6,7 +6,10 @@ depend on the size (the oracle is too slow there):
     with a small fraction of the divergence the noise state had;
   * the airfoil cast on 512x384x128: every span slice of ETA integrates to
     the area of the cast polygon (the sheared rotation scales the table's
-    area by cos^2 AOA), and the slices are identical (extrusion along x).
+    area by cos^2 AOA), and the slices are identical (extrusion along x);
+  * the airfoil LES on that grid (stretched in y and z, inflow along z), 3
+    steps from the inflow: FP64 and FP32-sweep solvers run the same sweep
+    counts and agree on the velocity and on C_D, C_L to 1e-6 relative.
 """
 import numpy as np
 import pytest
@@ -84,3 +87,39 @@ def test_airfoil_cast_full_size(hip_lib):
     expect = polygon_area(az0, ay0) * np.cos(np.radians(aoa)) ** 2
     assert abs(area - expect) <= 1e-2 * expect, (area, expect)
     assert ((e > 0.0) & (e < 1.0)).sum() > 0
+
+
+def test_airfoil_les_steps_full_size(hip_lib):
+    nx, ny, nz = 512, 384, 128
+    g = Grid(nx, ny, nz, lx=1.0, ly=8.0, lz=8.0, Y=tanh_nodes(ny, 8.0, 4.0, 5.0),
+             Z=tanh_nodes(nz, 8.0, 4.0, 5.0))
+    bc = bc_codes(BackWall_w="Dirichlet", FrontWall_u="Neumann", FrontWall_v="Neumann",
+                  FrontWall_w="Neumann", NorthWall_u="Neumann", NorthWall_w="Neumann",
+                  SouthWall_u="Neumann", SouthWall_w="Neumann", WestWall_v="Neumann",
+                  WestWall_w="Neumann", EastWall_v="Neumann", EastWall_w="Neumann")
+    az0, ay0 = naca4_polygon(poly=5000)
+    out = {}
+    for sor in (SOR_REDBLACK, SOR_MIXED):
+        s = ProjectionSolver(g, bc)
+        try:
+            s.configure(dt=1e-4, nu=1.0 / 5000.0, Cs=0.1, LES=1, scheme=QUICK_CENTRE, omega=1.5,
+                        zeta=1e-10, itmax=3000.0, sor=sor, lx=1.0, force_every=1)
+            u, v, w, p = g.halo(), g.halo(), g.halo(), g.halo()
+            w[...] = 1.0
+            s.boundary_conditions(u, v, w, p)
+            eta = g.halo()
+            s.ray_casting(az0, ay0, 8.0, 4.0, 4.0, ETA=eta)
+            s.upload(u, v, w, p, eta)
+            ik, pm, cd, cl, rc = s.step(3)
+            fin = [g.halo() for _ in range(4)]
+            s.download(*fin)
+            out[sor] = (np.array(ik), np.array(cd), np.array(cl), fin)
+        finally:
+            s.close()
+    a, b = out[SOR_REDBLACK], out[SOR_MIXED]
+    assert np.array_equal(a[0], b[0]) and a[0][-1] > 1
+    for x, y in ((a[1], b[1]), (a[2], b[2])):
+        assert np.abs(x - y).max() <= 1e-6 * np.abs(x).max(), (x, y)
+    vel = max(np.abs(x).max() for x in a[3][:3])
+    err = max(np.abs(x - y).max() for x, y in zip(b[3][:3], a[3][:3])) / vel
+    assert err < 1e-6, err
