@@ -101,8 +101,11 @@ def adjoint_workload(n=96):
 
 def adjoint_leg(hip, hiptime, n, steps, fd_dirs):
     """Forward + checkpointed reverse pass; cost ratio against the faster of a
-    plain forward of the same steps and the adjoint run's own forward pass; central finite differences (step 1e-6) of
-    the plain forward along seeded random directions of the body force."""
+    plain forward of the same steps and the adjoint run's own forward pass
+    (and, under "planned", the same for the library's checkpoint plan, which
+    keeps every step's state when it fits in HBM); central finite
+    differences (step 1e-6) of the plain forward along seeded random
+    directions of the body force."""
     from ntustcfdlab_amd.grid import Grid, bc_codes, gaussian_blob
     a = adjoint_workload(n)
     a["steps"] = steps
@@ -141,6 +144,18 @@ def adjoint_leg(hip, hiptime, n, steps, fd_dirs):
         s.upload(*[x.copy(order="F") for x in rest], eta)
         J, ik, ms = s.adjoint_run(a["steps"], a["every"], a["navg"])
         gf = s.adjoint_gradient()
+        planned = None
+        k, store = s.adjoint_checkpoint_plan(a["steps"], ADJ_RESERVE_BYTES)
+        if k != a["every"]:
+            s.set_body_force(*f)
+            s.upload(*[x.copy(order="F") for x in rest], eta)
+            J2, ik2, ms2 = s.adjoint_run(a["steps"], k, a["navg"])
+            assert J2 == J and np.array_equal(ik2, ik)
+            planned = {"checkpoint_every": k, "store_GB": round(store / 1e9, 1),
+                       "reverse_ms": round(float(ms2[1]), 1),
+                       "recompute_ms": round(float(ms2[2]), 1),
+                       "adjoint_over_forward":
+                           round(float(ms2[1]) / min(t_plain, float(ms2[0])), 3)}
         frng = np.random.default_rng(a["seed"] + 1)
         fd = []
         h = 1e-6
@@ -165,6 +180,7 @@ def adjoint_leg(hip, hiptime, n, steps, fd_dirs):
         "reverse_ms": round(float(ms[1]), 1), "recompute_ms": round(float(ms[2]), 1),
         "adjoint_kernels_ms": round(float(ms[3]), 1),
         "adjoint_over_forward": round(float(ms[1]) / min(t_plain, float(ms[0])), 3),
+        "planned": planned,
         "fd_step": h, "fd": fd,
         "fd_max_rel_err": max((x["rel_err"] for x in fd), default=None),
     }
