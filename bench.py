@@ -200,7 +200,8 @@ def adjoint_cavity_leg(hip, hiptime, n, steps, every, navg, fd_dirs):
     the last navg of `steps` steps; checkpoints by the library's plan (every
     step's state kept when it fits in HBM), and, for comparison, every
     `every` steps.  Cost of the reverse pass against the faster of a plain
-    forward of the same solver and the adjoint run's own forward pass;
+    forward of the same solver and the adjoint run's own forward pass (also
+    for the FP64 solver's adjoint, FP64 transposed sweeps, planned spacing);
     central differences of the FP64 forward along seeded random
     directions with step h = 2e-4.  In the seeded noise state some faces'
     advecting velocity is near zero, and a perturbation flips QUICK's upwind
@@ -259,6 +260,14 @@ def adjoint_cavity_leg(hip, hiptime, n, steps, every, navg, fd_dirs):
                 "adjoint_over_forward": round(float(ms2[1]) / min(t_plain, float(ms2[0])), 3)}
         configure(hip.SOR_REDBLACK)
         J64, t64 = plain(f)
+        # the FP64 solver's adjoint (FP64 transposed sweeps), planned spacing
+        k64, _ = s.adjoint_checkpoint_plan(steps, ADJ_RESERVE_BYTES)
+        J64a, ik64, ms64 = adjoint(k64)
+        fp64_adjoint = {
+            "checkpoint_every": k64, "J": J64a, "sweeps_per_step": float(np.mean(ik64)),
+            "forward_ms": round(float(ms64[0]), 1), "plain_forward_ms": round(t64, 1),
+            "reverse_ms": round(float(ms64[1]), 1), "recompute_ms": round(float(ms64[2]), 1),
+            "adjoint_over_forward": round(float(ms64[1]) / min(t64, float(ms64[0])), 3)}
         frng = np.random.default_rng(97)
         fd = []
         for _ in range(fd_dirs):
@@ -288,6 +297,7 @@ def adjoint_cavity_leg(hip, hiptime, n, steps, every, navg, fd_dirs):
         "adjoint_kernels_ms": round(float(ms[3]), 1),
         "adjoint_over_forward": round(float(ms[1]) / min(t_plain, float(ms[0])), 3),
         "checkpointed": checkpointed,
+        "fp64_solver_adjoint": fp64_adjoint,
         "fd": fd, "fd_max_rel_err": max((x["rel_err"] for x in fd), default=None),
     }
 

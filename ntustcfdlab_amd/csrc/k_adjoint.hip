@@ -321,34 +321,71 @@ k_adj_to_split(KGrid g, SplitGrid s, const double *__restrict__ rx,
     }
 }
 
-// one transposed half-sweep: colour C cells gather the other colour's z
-template <int C>
+// one transposed half-sweep: colour C cells gather the other colour's z,
+//   z <- self z + omega S(z_other)/aP,   and (ACC) Z += the new z,
+// in the layout of the forward FP64 sweep k_rb_sweep: RB_KPT64 cells along
+// k per thread, the other colour's W/E entries of rows k0-1 .. k0+KPT
+// loaded once and shared as the +-k neighbours, XCD-slab block order,
+// loads first.  The arithmetic is the plain one-cell form's, term for term.
+template <int C, bool ACC>
 __global__ void __launch_bounds__(256)
 k_adj_sweep(KGrid g, SplitGrid s, const double *__restrict__ rx,
             const double *__restrict__ ry, const double *__restrict__ rz,
             double *__restrict__ z_own, const double *__restrict__ z_oth,
-            double *__restrict__ Z_own, double self, double omega, int acc)
+            double *__restrict__ Z_own, double self, double omega)
 {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
-    const int i = 2 * q + ((C + j + k) & 1);
-    if (i < 1 || i > g.nx || j > g.ny) return;
-    const long row = (long)s.mp * ((long)j + (long)s.ry * k);
-    const long o = row + q;
-    const long pl = (long)s.mp * s.ry;
-    const double zW = z_oth[row + ((i - 1) >> 1)], zE = z_oth[row + ((i + 1) >> 1)];
-    const double zS = z_oth[o - s.mp], zN = z_oth[o + s.mp];
-    const double zB = z_oth[o - pl], zF = z_oth[o + pl];
-    const double ax = g.iDx[i], ay = g.iDy[j], az = g.iDz[k];
-    const double sum = ay * az * (rx[i] * zE + rx[i - 1] * zW) +
-                       ax * az * (ry[j] * zN + ry[j - 1] * zS) +
-                       ax * ay * (rz[k] * zF + rz[k - 1] * zB);
-    const double aP = ay * az * (rx[i] + rx[i - 1]) + ax * az * (ry[j] + ry[j - 1]) +
-                      ax * ay * (rz[k] + rz[k - 1]);
-    const double zn = self * z_own[o] + omega * sum / aP;
-    z_own[o] = zn;
-    if (acc) Z_own[o] += zn;
+    constexpr int KPT = RB_KPT64;
+    int bx, by, bz;
+    xcd_block(bx, by, bz);
+    const int q = bx * blockDim.x + threadIdx.x;
+    const int jr = by * blockDim.y + threadIdx.y + 1;
+    const int j = min(jr, g.ny);
+    const int k0 = bz * KPT + 1;
+    const int sh0 = (C + 1 + j + k0) & 1;
+    const int ia = min(2 * q + 1 + sh0, g.nx + 1), ib = min(2 * q + 2 - sh0, g.nx + 1);
+    const double axa = g.iDx[ia], rEa = rx[ia], rWa = rx[ia - 1];
+    const double axb = g.iDx[ib], rEb = rx[ib], rWb = rx[ib - 1];
+    const double ay = g.iDy[j], rN = ry[j], rS = ry[j - 1];
+    const unsigned mp = (unsigned)s.mp;
+    const unsigned q1 = (unsigned)min(q, (g.nx + 1) / 2);
+    double vW[KPT + 2], vE[KPT + 2], vN[KPT], vS[KPT], vo[KPT], vZ[KPT];
+    unsigned at[KPT];
+#pragma unroll
+    for (int r = -1; r <= KPT; r++) {
+        const int k = min(k0 + r, g.nz + 1);
+        const unsigned row = mp * ((unsigned)j + (unsigned)s.ry * (unsigned)k);
+        vW[r + 1] = z_oth[row + q1];
+        vE[r + 1] = z_oth[row + q1 + 1];
+    }
+#pragma unroll
+    for (int r = 0; r < KPT; r++) {
+        const int k = min(k0 + r, g.nz);
+        const unsigned row = mp * ((unsigned)j + (unsigned)s.ry * (unsigned)k);
+        const unsigned o = row + q1 + (unsigned)(sh0 ^ (r & 1));
+        at[r] = o;
+        vN[r] = z_oth[o + mp];
+        vS[r] = z_oth[o - mp];
+        vo[r] = z_own[o];
+        if (ACC) vZ[r] = Z_own[o];
+    }
+#pragma unroll
+    for (int r = 0; r < KPT; r++) {
+        const int k = min(k0 + r, g.nz);
+        const bool odd = r & 1;
+        const bool sh = sh0 ^ (int)odd;
+        const double ax = odd ? axb : axa, rE = odd ? rEb : rEa, rW = odd ? rWb : rWa;
+        const double az = g.iDz[k], rF = rz[k], rB = rz[k - 1];
+        const double zF = sh ? vE[r + 2] : vW[r + 2], zB = sh ? vE[r] : vW[r];
+        const double sum = ay * az * (rE * vE[r + 1] + rW * vW[r + 1]) +
+                           ax * az * (rN * vN[r] + rS * vS[r]) + ax * ay * (rF * zF + rB * zB);
+        const double aP = ay * az * (rE + rW) + ax * az * (rN + rS) + ax * ay * (rF + rB);
+        const double zn = self * vo[r] + omega * sum / aP;
+        const int i = 2 * q + 1 + (int)sh;
+        if (jr <= g.ny && k0 + r <= g.nz && i <= g.nx) {
+            z_own[at[r]] = zn;
+            if (ACC) Z_own[at[r]] = vZ[r] + zn;
+        }
+    }
 }
 
 // after the FP64 transposed sweeps (the black ones not accumulating): the
@@ -939,14 +976,18 @@ void launch_adj_pressure(const KGrid &g, const SplitGrid &s, const double *rx, c
     const dim3 blk(64, 4);
     const int nq = (g.nx + 2) / 2;
     const dim3 grs((nq + 63) / 64, (g.ny + 3) / 4, g.nz);
+    // the transposed sweeps: the forward FP64 sweep's grid (cell pairs,
+    // RB_KPT64 planes per thread)
+    const int nq64 = (g.nx + 1) / 2;
+    const dim3 g64((nq64 + 63) / 64, (g.ny + 3) / 4, (g.nz + RB_KPT64 - 1) / RB_KPT64);
     hipLaunchKernelGGL(k_adj_to_split, cells_grid(g), blk, 0, st, g, s, rx, ry, rz, lp, omega,
                        z0, z1, Z0, Z1);
     if (!mx) {
         for (int t = sweeps; t >= 1; t--) {
-            hipLaunchKernelGGL(k_adj_sweep<0>, grs, blk, 0, st, g, s, rx, ry, rz, z0, z1, Z0,
-                               t == sweeps ? 1.0 : 1.0 - omega, omega, 1);
-            hipLaunchKernelGGL(k_adj_sweep<1>, grs, blk, 0, st, g, s, rx, ry, rz, z1, z0, Z1,
-                               1.0 - omega, omega, 0);
+            hipLaunchKernelGGL((k_adj_sweep<0, true>), g64, blk, 0, st, g, s, rx, ry, rz, z0, z1,
+                               Z0, t == sweeps ? 1.0 : 1.0 - omega, omega);
+            hipLaunchKernelGGL((k_adj_sweep<1, false>), g64, blk, 0, st, g, s, rx, ry, rz, z1,
+                               z0, nullptr, 1.0 - omega, omega);
         }
         if (sweeps > 0)
             hipLaunchKernelGGL(k_adj_black_sum, grs, blk, 0, st, g, s, rx, ry, rz, Z0, z1, Z1,
@@ -957,8 +998,8 @@ void launch_adj_pressure(const KGrid &g, const SplitGrid &s, const double *rx, c
         // accumulating -- in FP32 segments of mx->seg half-sweeps (even:
         // every segment starts with a black update, and only the last can
         // end with one)
-        hipLaunchKernelGGL(k_adj_sweep<0>, grs, blk, 0, st, g, s, rx, ry, rz, z0, z1, Z0, 1.0,
-                           omega, 1);
+        hipLaunchKernelGGL((k_adj_sweep<0, true>), g64, blk, 0, st, g, s, rx, ry, rz, z0, z1, Z0,
+                           1.0, omega);
         const int nq32 = (g.nx + 1) / 2;
         const dim3 g32((nq32 + 63) / 64, (g.ny + 3) / 4, (g.nz + RB_KPT - 1) / RB_KPT);
         const dim3 gf((unsigned)((s.plane + 255) / 256));
