@@ -690,12 +690,6 @@ __device__ __forceinline__ double face_val(const Face3 &f, const double *phi)
 {
     return f.w0 * phi[f.n0] + f.w1 * phi[f.n1] + f.w2 * phi[f.n2];
 }
-__device__ __forceinline__ void face_adj(const Face3 &f, double *lphi, double l)
-{
-    aadd(&lphi[f.n0], f.w0 * l);
-    aadd(&lphi[f.n1], f.w1 * l);
-    aadd(&lphi[f.n2], f.w2 * l);
-}
 
 // axis A of component C: flux difference (ap f+ - am f-)/C_A and diffusion,
 // transposed for the adjoint lam of phi*(P); upwind==true uses first-order
@@ -703,7 +697,8 @@ __device__ __forceinline__ void face_adj(const Face3 &f, double *lphi, double l)
 template <int C, int A, bool UPWIND>
 __device__ __forceinline__ void adj_axis(const KGrid &g, const double *vel_a, double *lvel_a,
                                          const double *phi, double *lphi, long o, int m,
-                                         double dt, double cnu, double lam, double &lap_sum)
+                                         double dt, double cnu, double lam, double &c0,
+                                         double &lap_sum)
 {
     typedef Ax<C, A> X;
     const long s = X::stride(g);
@@ -730,18 +725,47 @@ __device__ __forceinline__ void adj_axis(const KGrid &g, const double *vel_a, do
     const double Cw = X::Cw(g, m);
     const double wv = -dt * lam / Cw;   // d phi*/d (ap f+ - am f-)
     const double vp = face_val(fp, phi), vm = face_val(fm, phi);
-    face_adj(fp, lphi, wv * ap);
-    face_adj(fm, lphi, -wv * am);
-    aadd(&lvel_a[ap0], 0.5 * wv * vp);
-    aadd(&lvel_a[ap1], 0.5 * wv * vp);
-    aadd(&lvel_a[am0], -0.5 * wv * vm);
-    aadd(&lvel_a[am1], -0.5 * wv * vm);
+    // This axis's contributions to lphi at o + d s (d = -2 .. 2) are summed
+    // in registers and added with one atomic per node; the centre one, c0,
+    // is shared by the three axes and added by the caller.
+    double cm2 = 0.0, cm1 = 0.0, cp1 = 0.0, cp2 = 0.0;
+    auto add = [&](long n, double v) {   // n is one of o + d s
+        if (n == o) c0 += v;
+        else if (n == o + s) cp1 += v;
+        else if (n == o - s) cm1 += v;
+        else if (n == o + 2 * s) cp2 += v;
+        else cm2 += v;
+    };
+    const double lp = wv * ap, lm = -wv * am;
+    add(fp.n0, fp.w0 * lp);
+    add(fp.n1, fp.w1 * lp);
+    add(fp.n2, fp.w2 * lp);
+    add(fm.n0, fm.w0 * lm);
+    add(fm.n1, fm.w1 * lm);
+    add(fm.n2, fm.w2 * lm);
+    if (C == A) {
+        // the advecting velocity is phi itself (lvel_a is lphi):
+        // ap0 = o + s, ap1 = am1 = o, am0 = o - s
+        cp1 += 0.5 * wv * vp;
+        c0 += 0.5 * wv * vp;
+        cm1 += -0.5 * wv * vm;
+        c0 += -0.5 * wv * vm;
+    } else {
+        aadd(&lvel_a[ap0], 0.5 * wv * vp);
+        aadd(&lvel_a[ap1], 0.5 * wv * vp);
+        aadd(&lvel_a[am0], -0.5 * wv * vm);
+        aadd(&lvel_a[am1], -0.5 * wv * vm);
+    }
     // diffusion: cnu*dt*((phi+ - phi)/L(m) - (phi - phi-)/L(m-1))/C
     const double Lp = X::L(g, m), Lm = X::L(g, m - 1);
     const double wd = cnu * dt * lam / Cw;
-    aadd(&lphi[o + s], wd / Lp);
-    aadd(&lphi[o], -wd / Lp - wd / Lm);
-    aadd(&lphi[o - s], wd / Lm);
+    cp1 += wd / Lp;
+    c0 += -wd / Lp - wd / Lm;
+    cm1 += wd / Lm;
+    aadd(&lphi[o + s], cp1);
+    aadd(&lphi[o - s], cm1);
+    aadd(&lphi[o + 2 * s], cp2);
+    aadd(&lphi[o - 2 * s], cm2);
     lap_sum += ((phi[o + s] - phi[o]) / Lp - (phi[o] - phi[o - s]) / Lm) / Cw;
 }
 
@@ -755,9 +779,9 @@ __device__ __forceinline__ void adj_component(const KGrid &g, const double *u, c
     const double *phi = C == 0 ? u : (C == 1 ? v : w);
     double *lphi = C == 0 ? lu : (C == 1 ? lv : lw);
     const long o = hix(g, i, j, k);
-    aadd(&lphi[o], lam);   // phi* = phi + ...
+    double c0 = lam;   // phi* = phi + ...; the axes add their centre terms
     double lap = 0.0;
-    adj_axis<C, 0, UPWIND>(g, u, lu, phi, lphi, o, i, dt, cnu, lam, lap);
+    adj_axis<C, 0, UPWIND>(g, u, lu, phi, lphi, o, i, dt, cnu, lam, c0, lap);
     if (C == 2 && UPWIND) {
         // discretisation_upwind.f90:194: the y-advecting v of the w equation
         // averages v(i,j,k) and v(i+1,j,k+1)
@@ -781,9 +805,10 @@ __device__ __forceinline__ void adj_component(const KGrid &g, const double *u, c
         aadd(&lphi[o - s], wd / Lm);
         lap += ((phi[o + s] - phi[o]) / Lp - (phi[o] - phi[o - s]) / Lm) / Cw;
     } else {
-        adj_axis<C, 1, UPWIND>(g, v, lv, phi, lphi, o, j, dt, cnu, lam, lap);
+        adj_axis<C, 1, UPWIND>(g, v, lv, phi, lphi, o, j, dt, cnu, lam, c0, lap);
     }
-    adj_axis<C, 2, UPWIND>(g, w, lw, phi, lphi, o, k, dt, cnu, lam, lap);
+    adj_axis<C, 2, UPWIND>(g, w, lw, phi, lphi, o, k, dt, cnu, lam, c0, lap);
+    aadd(&lphi[o], c0);
     if (lvisc_cell) *lvisc_cell += les * dt * lap * lam;
 }
 
