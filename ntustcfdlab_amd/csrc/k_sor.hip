@@ -192,17 +192,21 @@ __global__ void k_rb_scatter(KGrid g, SplitGrid s, const double *__restrict__ p0
 // Blocks run in the XCD-slab order.  At 256^3 this layout (8 cells, row
 // reuse) measured 55.6 us per half-sweep against 62.0 us for 4 cells with
 // separate front/back loads in the natural order; at 128^3 7.8 against 8.5.
-// SPEC: the speculative form -- no gate, no control block: the sweep runs
-// and records its maximum; the host decides afterwards whether the
-// reference's loop would have run it (and replays from a checkpoint if not).
-template <int C, bool SPEC>
+// MODE 1, 2: the speculative forms -- no gate, no control block: the sweep
+// runs and records its maximum (1) or only whether it exceeds zeta (2,
+// flags[t] = 1: all the stopping rule asks of a sweep that is not the last
+// of its batch); the host decides afterwards whether the reference's loop
+// would have run it (and replays from a checkpoint if not).  MODE 0: the
+// gated sweep.
+template <int C, int MODE>
 __global__ void __launch_bounds__(256)
 k_rb_sweep(KGrid g, SplitGrid s, const double *__restrict__ rx,
            const double *__restrict__ ry, const double *__restrict__ rz,
            double *__restrict__ pc_own, const double *__restrict__ pc_oth,
            const double *__restrict__ b_own, double omega, double zeta, double itmax, int targ,
-           double *slots, SorControl *ctl)
+           double *slots, SorControl *ctl, int *flags)
 {
+    constexpr bool SPEC = MODE != 0;
     constexpr int KPT = RB_KPT64;
     __shared__ double red[4];
     const int tid = threadIdx.x + threadIdx.y * blockDim.x;
@@ -252,7 +256,7 @@ k_rb_sweep(KGrid g, SplitGrid s, const double *__restrict__ rx,
         vo[r] = pc_own[o];
     }
     double pcm = 0.0, nv[KPT];
-    bool in[KPT];
+    bool in[KPT], above = false;
 #pragma unroll
     for (int r = 0; r < KPT; r++) {
         const int k = min(k0 + r, g.nz);
@@ -272,8 +276,13 @@ k_rb_sweep(KGrid g, SplitGrid s, const double *__restrict__ rx,
         const double d = (sum - vb[r]) / aP - vo[r];
         const int i = 2 * q + 1 + (sh0 ^ (int)odd);
         in[r] = jr <= g.ny && k0 + r <= g.nz && i <= g.nx;
-        const double pc = in[r] ? fabs(d) : 0.0;
-        if (pc > pcm) pcm = pc;   // a NaN change never raises the max (GS.f90:61)
+        if (MODE == 2) {
+            // max > zeta exactly when some change is > zeta (NaN is neither)
+            above = above || (in[r] && fabs(d) > zeta);
+        } else {
+            const double pc = in[r] ? fabs(d) : 0.0;
+            if (pc > pcm) pcm = pc;   // a NaN change never raises the max (GS.f90:61)
+        }
         nv[r] = fma(omega, d, vo[r]);
     }
     if (!SPEC) {
@@ -293,6 +302,11 @@ k_rb_sweep(KGrid g, SplitGrid s, const double *__restrict__ rx,
 #pragma unroll
     for (int r = 0; r < KPT; r++)
         if (in[r]) pc_own[at[r]] = nv[r];
+    if (MODE == 2) {
+        // every lane reaches this point; one plain store per wavefront
+        if (__any(above) && lane == 0) flags[t] = 1;
+        return;
+    }
     pcm = wave_max(pcm);
     if ((tid & 63) == 0) red[tid >> 6] = pcm;
     __syncthreads();
@@ -435,22 +449,30 @@ void launch_rb_sweep(const KGrid &g, const SplitGrid &s, const double *rx, const
 {
     const int nq = (g.nx + 1) / 2;
     dim3 blk(64, 4), grd((nq + 63) / 64, (g.ny + 3) / 4, (g.nz + RB_KPT64 - 1) / RB_KPT64);
-    hipLaunchKernelGGL((k_rb_sweep<0, false>), grd, blk, 0, st, g, s, rx, ry, rz, p0, p1, b0,
-                       omega, zeta, itmax, t, slots, ctl);
-    hipLaunchKernelGGL((k_rb_sweep<1, false>), grd, blk, 0, st, g, s, rx, ry, rz, p1, p0, b1,
-                       omega, zeta, itmax, t, slots, ctl);
+    hipLaunchKernelGGL((k_rb_sweep<0, 0>), grd, blk, 0, st, g, s, rx, ry, rz, p0, p1, b0,
+                       omega, zeta, itmax, t, slots, ctl, nullptr);
+    hipLaunchKernelGGL((k_rb_sweep<1, 0>), grd, blk, 0, st, g, s, rx, ry, rz, p1, p0, b1,
+                       omega, zeta, itmax, t, slots, ctl, nullptr);
 }
 
 void launch_rb_sweep_spec(const KGrid &g, const SplitGrid &s, const double *rx, const double *ry,
                           const double *rz, double *p0, double *p1, const double *b0,
-                          const double *b1, double omega, int t, double *slots, hipStream_t st)
+                          const double *b1, double omega, int t, double *slots, double zeta,
+                          int *flags, hipStream_t st)
 {
     const int nq = (g.nx + 1) / 2;
     dim3 blk(64, 4), grd((nq + 63) / 64, (g.ny + 3) / 4, (g.nz + RB_KPT64 - 1) / RB_KPT64);
-    hipLaunchKernelGGL((k_rb_sweep<0, true>), grd, blk, 0, st, g, s, rx, ry, rz, p0, p1, b0,
-                       omega, 0.0, 0.0, t, slots, nullptr);
-    hipLaunchKernelGGL((k_rb_sweep<1, true>), grd, blk, 0, st, g, s, rx, ry, rz, p1, p0, b1,
-                       omega, 0.0, 0.0, t, slots, nullptr);
+    if (flags) {
+        hipLaunchKernelGGL((k_rb_sweep<0, 2>), grd, blk, 0, st, g, s, rx, ry, rz, p0, p1, b0,
+                           omega, zeta, 0.0, t, slots, nullptr, flags);
+        hipLaunchKernelGGL((k_rb_sweep<1, 2>), grd, blk, 0, st, g, s, rx, ry, rz, p1, p0, b1,
+                           omega, zeta, 0.0, t, slots, nullptr, flags);
+    } else {
+        hipLaunchKernelGGL((k_rb_sweep<0, 1>), grd, blk, 0, st, g, s, rx, ry, rz, p0, p1, b0,
+                           omega, 0.0, 0.0, t, slots, nullptr, nullptr);
+        hipLaunchKernelGGL((k_rb_sweep<1, 1>), grd, blk, 0, st, g, s, rx, ry, rz, p1, p0, b1,
+                           omega, 0.0, 0.0, t, slots, nullptr, nullptr);
+    }
 }
 
 // zero the buckets of sweeps t0 .. t0+n-1
@@ -461,29 +483,10 @@ k_slots_clear(double *slots, int t0, int n)
     if (q < n * SLOT_BUCKETS) slots[slot_index(t0 + q / SLOT_BUCKETS, q % SLOT_BUCKETS)] = 0.0;
 }
 
-// m[s] = the maximum change of sweep t0+s (over its buckets)
-__global__ void __launch_bounds__(256)
-k_slots_max(double *slots, int t0, int n, double *m)
-{
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n) return;
-    double v = 0.0;
-    for (int b = 0; b < SLOT_BUCKETS; b++) {
-        const double x = load_flag(&slots[slot_index(t0 + q, b)]);
-        v = x > v ? x : v;
-    }
-    m[q] = v;
-}
-
 void launch_slots_clear(double *slots, int t0, int n, hipStream_t st)
 {
     hipLaunchKernelGGL(k_slots_clear, dim3((n * SLOT_BUCKETS + 255) / 256), dim3(256), 0, st,
                        slots, t0, n);
-}
-
-void launch_slots_max(double *slots, int t0, int n, double *m, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_slots_max, dim3((n + 255) / 256), dim3(256), 0, st, slots, t0, n, m);
 }
 
 }  // namespace ntust

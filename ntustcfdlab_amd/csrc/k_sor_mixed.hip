@@ -30,15 +30,19 @@
 
 namespace ntust {
 
-// SPEC: the speculative form (no gate, no control block, no dirty flag),
-// as for the FP64 sweep
-template <int C, bool SPEC>
+// MODE 0: the gated sweep.  MODE 1, 2: the speculative forms (no gate, no
+// control block, no dirty flag), as for the FP64 sweep; 1 records the
+// sweep's maximum change in the buckets, 2 only whether it exceeds zeta
+// (flags[t] = 1), which is all the stopping rule asks of a sweep that is
+// not the last of its batch.
+template <int C, int MODE>
 __global__ void __launch_bounds__(256)
 k_rb_sweep32(KGrid g, SplitGrid s, FMetric fm, float *__restrict__ e_own,
              const float *__restrict__ e_oth, const float *__restrict__ r_own, float omega,
              double zeta, double itmax, int targ, double *slots, SorControl *ctl, int *dirty,
-             int first)
+             int first, int *flags)
 {
+    constexpr bool SPEC = MODE != 0;
     __shared__ double red[4];
     const int tid = threadIdx.x + threadIdx.y * blockDim.x;
     const int blin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
@@ -84,7 +88,7 @@ k_rb_sweep32(KGrid g, SplitGrid s, FMetric fm, float *__restrict__ e_own,
         vo[r] = e_own[o];
     }
     float pcm = 0.0f, nv[RB_KPT];
-    bool in[RB_KPT];
+    bool in[RB_KPT], above = false;
 #pragma unroll
     for (int r = 0; r < RB_KPT; r++) {
         const int k = min(k0 + r, g.nz);
@@ -100,8 +104,13 @@ k_rb_sweep32(KGrid g, SplitGrid s, FMetric fm, float *__restrict__ e_own,
         const float d = (sum + vr[r]) / aP - vo[r];
         const int i = 2 * q + 1 + (sh0 ^ (int)odd);
         in[r] = jr <= g.ny && k0 + r <= g.nz && i <= g.nx;
-        const float pc = in[r] ? fabsf(d) : 0.0f;
-        if (pc > pcm) pcm = pc;
+        if (MODE == 2) {
+            // max > zeta exactly when some change is > zeta (NaN is neither)
+            above = above || (in[r] && (double)fabsf(d) > zeta);
+        } else {
+            const float pc = in[r] ? fabsf(d) : 0.0f;
+            if (pc > pcm) pcm = pc;
+        }
         nv[r] = fmaf(omega, d, vo[r]);
     }
     if (!SPEC) {
@@ -122,6 +131,11 @@ k_rb_sweep32(KGrid g, SplitGrid s, FMetric fm, float *__restrict__ e_own,
 #pragma unroll
     for (int r = 0; r < RB_KPT; r++)
         if (in[r]) e_own[at[r]] = nv[r];
+    if (MODE == 2) {
+        // every lane reaches this point; one plain store per wavefront
+        if (__any(above) && lane == 0) flags[t] = 1;
+        return;
+    }
     double pm = wave_max((double)pcm);
     if ((tid & 63) == 0) red[tid >> 6] = pm;
     __syncthreads();
@@ -299,22 +313,29 @@ void launch_rb_sweep32(const KGrid &g, const SplitGrid &s, const FMetric &fm, fl
 {
     const int nq = (g.nx + 1) / 2;
     dim3 blk(64, 4), grd((nq + 63) / 64, (g.ny + 3) / 4, (g.nz + RB_KPT - 1) / RB_KPT);
-    hipLaunchKernelGGL((k_rb_sweep32<0, false>), grd, blk, 0, st, g, s, fm, e0, e1, r0, omega, zeta,
-                       itmax, t, slots, ctl, dirty, first);
-    hipLaunchKernelGGL((k_rb_sweep32<1, false>), grd, blk, 0, st, g, s, fm, e1, e0, r1, omega, zeta,
-                       itmax, t, slots, ctl, dirty, 0);
+    hipLaunchKernelGGL((k_rb_sweep32<0, 0>), grd, blk, 0, st, g, s, fm, e0, e1, r0, omega, zeta,
+                       itmax, t, slots, ctl, dirty, first, nullptr);
+    hipLaunchKernelGGL((k_rb_sweep32<1, 0>), grd, blk, 0, st, g, s, fm, e1, e0, r1, omega, zeta,
+                       itmax, t, slots, ctl, dirty, 0, nullptr);
 }
 
 void launch_rb_sweep32_spec(const KGrid &g, const SplitGrid &s, const FMetric &fm, float *e0,
                             float *e1, const float *r0, const float *r1, float omega, int t,
-                            double *slots, hipStream_t st)
+                            double *slots, double zeta, int *flags, hipStream_t st)
 {
     const int nq = (g.nx + 1) / 2;
     dim3 blk(64, 4), grd((nq + 63) / 64, (g.ny + 3) / 4, (g.nz + RB_KPT - 1) / RB_KPT);
-    hipLaunchKernelGGL((k_rb_sweep32<0, true>), grd, blk, 0, st, g, s, fm, e0, e1, r0, omega, 0.0,
-                       0.0, t, slots, nullptr, nullptr, 0);
-    hipLaunchKernelGGL((k_rb_sweep32<1, true>), grd, blk, 0, st, g, s, fm, e1, e0, r1, omega, 0.0,
-                       0.0, t, slots, nullptr, nullptr, 0);
+    if (flags) {
+        hipLaunchKernelGGL((k_rb_sweep32<0, 2>), grd, blk, 0, st, g, s, fm, e0, e1, r0, omega,
+                           zeta, 0.0, t, slots, nullptr, nullptr, 0, flags);
+        hipLaunchKernelGGL((k_rb_sweep32<1, 2>), grd, blk, 0, st, g, s, fm, e1, e0, r1, omega,
+                           zeta, 0.0, t, slots, nullptr, nullptr, 0, flags);
+    } else {
+        hipLaunchKernelGGL((k_rb_sweep32<0, 1>), grd, blk, 0, st, g, s, fm, e0, e1, r0, omega,
+                           0.0, 0.0, t, slots, nullptr, nullptr, 0, nullptr);
+        hipLaunchKernelGGL((k_rb_sweep32<1, 1>), grd, blk, 0, st, g, s, fm, e1, e0, r1, omega,
+                           0.0, 0.0, t, slots, nullptr, nullptr, 0, nullptr);
+    }
 }
 
 void launch_rb_residual(const KGrid &g, const SplitGrid &s, const double *rx, const double *ry,

@@ -69,8 +69,11 @@ struct Ctx {
     SorControl *ctl = nullptr;
     double *pmax = nullptr;
     int ik_hint = 0;
-    // measurement of the red-black half-sweep
+    // measurement of the red-black half-sweep; sev: one pair per segment of
+    // the FP32-sweep solver's speculative batch
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    static constexpr int NSEV = 64;
+    hipEvent_t sev[2 * NSEV] = {};
     double prof_ms = 0.0;
     long long prof_launches = 0;
     // red-black sweep kernel: one launch per sweep (fused) or per colour,
@@ -226,22 +229,27 @@ long spec_batch(long tmax, long quantum)
     return nb >= 2 * quantum ? nb : 0;
 }
 
-// maxima of sweeps 1..nb to the host; index of the first sweep whose
-// maximum ends the loop (not > zeta; a NaN ends it too), or -1
-int spec_stop(double *slots, long nb, double zeta, long *q)
+// The speculative batch of sweeps 1..nb recorded, for sweeps 1..nb-1, only
+// whether each one's maximum change exceeded zeta (flags, zeroed before the
+// batch), and for sweep nb its maximum in the buckets (the next gated sweep
+// and the reported maximum read it).  Index (0-based) of the first sweep
+// whose maximum ends the loop (not > zeta; a NaN maximum ends it too), or -1.
+int spec_stop(const double *slots, const int *flags, long nb, double zeta, long *q)
 {
-    NEED(dm, "sor_maxima", (size_t)SPEC_MAX, false);
-    std::vector<double> m((size_t)nb);
-    launch_slots_max(slots, 1, (int)nb, dm, C.st);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(m.data(), dm, nb * sizeof(double), hipMemcpyDeviceToHost, C.st));
+    std::vector<int> f((size_t)nb + 1);
+    double bk[SLOT_BUCKETS * SLOT_STRIDE];
+    HIPCHK(hipMemcpyAsync(f.data(), flags, (nb + 1) * sizeof(int), hipMemcpyDeviceToHost, C.st));
+    HIPCHK(hipMemcpyAsync(bk, slots + slot_index(nb, 0), sizeof bk, hipMemcpyDeviceToHost, C.st));
     TRY(sync());
     *q = -1;
-    for (long i = 0; i < nb; i++)
-        if (!(m[i] > zeta)) {
+    for (long i = 0; i + 1 < nb; i++)
+        if (f[(size_t)i + 1] == 0) {
             *q = i;
-            break;
+            return 0;
         }
+    double m = 0.0;
+    for (int b = 0; b < SLOT_BUCKETS; b++) m = bk[b * SLOT_STRIDE] > m ? bk[b * SLOT_STRIDE] : m;
+    if (!(m > zeta)) *q = nb - 1;
     return 0;
 }
 
@@ -336,39 +344,54 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
         const long nb1 = !fused && !graphs && 1.0 > zeta ? spec_batch(tmax, seg) : 0;
         if (nb1 > 0) {
             // sweeps 1..upto in segments counted from sweep 1, each followed
-            // by the FP64 correction, as the gated loop below runs them
+            // by the FP64 correction, as the gated loop below runs them; all
+            // but the last record only whether they exceed zeta
+            NEED(fl, "sor_flags", (size_t)SPEC_MAX / 2 + 2, false);
+            int *flags = (int *)fl;
+            HIPCHK(hipMemsetAsync(flags, 0, (nb1 + 1) * sizeof(int), C.st));
+            // the sweeps of every segment but the first (which follows the
+            // set-up) are timed, up to NSEV segments
+            int nev = 0;
+            long tlaunch = 0;
             auto run = [&](long upto, bool timed) {
                 for (long tt = 1; tt <= upto; tt += seg) {
                     const long n = seg < upto - tt + 1 ? seg : upto - tt + 1;
-                    // the second segment is timed (the first follows the set-up)
-                    const bool ev = timed && tt == 1 + seg;
-                    if (ev) (void)hipEventRecord(C.ev0, C.st);
+                    const bool ev = timed && tt > 1 && nev < Ctx::NSEV;
+                    if (ev) HIPCHK(hipEventRecord(C.sev[2 * nev], C.st));
                     for (long b = 0; b < n; b++)
                         launch_rb_sweep32_spec(g, C.sg, fm, E0, E1, R0, R1, (float)omega,
-                                               (int)(tt + b), slots, C.st);
-                    if (ev) (void)hipEventRecord(C.ev1, C.st);
+                                               (int)(tt + b), slots, zeta,
+                                               tt + b < upto ? flags : nullptr, C.st);
+                    if (ev) {
+                        HIPCHK(hipEventRecord(C.sev[2 * nev + 1], C.st));
+                        nev++;
+                        tlaunch += 2 * n;
+                    }
                     launch_rb_correct(g, C.sg, rx, ry, rz, P0, P1, E0, E1, B0, B1, R0, R1,
                                       nullptr, C.st);
                 }
+                return 0;
             };
             launch_slots_clear(slots, 1, (int)nb1 + 1, C.st);
-            run(nb1, true);
+            TRY(run(nb1, true));
             long q;
-            TRY(spec_stop(slots, nb1, zeta, &q));
+            TRY(spec_stop(slots, flags, nb1, zeta, &q));
             if (q >= 0 && q < nb1 - 1) {
                 launch_rb_gather(g, C.sg, p, P0, P1, C.st);
                 HIPCHK(hipMemsetAsync(E0, 0, np * sizeof(float), C.st));
                 HIPCHK(hipMemsetAsync(E1, 0, np * sizeof(float), C.st));
                 launch_rb_residual(g, C.sg, rx, ry, rz, P0, P1, B0, B1, R0, R1, E1, nullptr,
                                    nullptr, C.st);
-                run(q + 1, false);
+                TRY(run(q + 1, false));
                 HIPCHK(hipGetLastError());
                 TRY(sync());
             } else {
-                float ms = 0.f;
-                HIPCHK(hipEventElapsedTime(&ms, C.ev0, C.ev1));
-                C.prof_ms += ms;
-                C.prof_launches += 2 * seg;
+                for (int e = 0; e < nev; e++) {
+                    float ms = 0.f;
+                    HIPCHK(hipEventElapsedTime(&ms, C.sev[2 * e], C.sev[2 * e + 1]));
+                    C.prof_ms += ms;
+                }
+                C.prof_launches += tlaunch;
             }
             h.ik = (int)(q >= 0 ? q + 1 : nb1);
             t = q >= 0 ? tmax + 1 : nb1 + 1;
@@ -466,17 +489,22 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
                           ? spec_batch(tmax, 8)
                           : 0;
     if (nb64 > 0) {
+        // all but the last sweep of a batch record only whether they exceed
+        // zeta
+        NEED(fl, "sor_flags", (size_t)SPEC_MAX / 2 + 2, false);
+        int *flags = (int *)fl;
+        HIPCHK(hipMemsetAsync(flags, 0, (nb64 + 1) * sizeof(int), C.st));
         auto run = [&](long upto) {
             for (long tt = 1; tt <= upto; tt++)
                 launch_rb_sweep_spec(g, C.sg, rx, ry, rz, P0, P1, B0, B1, omega, (int)tt, slots,
-                                     C.st);
+                                     zeta, tt < upto ? flags : nullptr, C.st);
         };
         launch_slots_clear(slots, 1, (int)nb64 + 1, C.st);
         HIPCHK(hipEventRecord(C.ev0, C.st));
         run(nb64);
         HIPCHK(hipEventRecord(C.ev1, C.st));
         long q;
-        TRY(spec_stop(slots, nb64, zeta, &q));
+        TRY(spec_stop(slots, flags, nb64, zeta, &q));
         if (q >= 0 && q < nb64 - 1) {
             launch_rb_gather(g, C.sg, p, P0, P1, C.st);
             run(q + 1);
@@ -867,6 +895,8 @@ int ntust_finalize(void)
     if (C.pmax) keep(hipFree(C.pmax));
     if (C.ev0) keep(hipEventDestroy(C.ev0));
     if (C.ev1) keep(hipEventDestroy(C.ev1));
+    for (int e = 0; e < 2 * Ctx::NSEV; e++)
+        if (C.sev[e]) keep(hipEventDestroy(C.sev[e]));
     if (C.st) keep(hipStreamDestroy(C.st));
     C = Ctx();
     if (first != hipSuccess)
@@ -885,6 +915,7 @@ static int init_device(int nx, int ny, int nz, const double *iDx, const double *
     HIPCHK(hipStreamCreateWithFlags(&C.st, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&C.ev0));
     HIPCHK(hipEventCreate(&C.ev1));
+    for (int e = 0; e < 2 * Ctx::NSEV; e++) HIPCHK(hipEventCreate(&C.sev[e]));
     const double *src[6] = {iDx, Dxs, iDy, Dys, iDz, Dzs};
     const int len[6] = {nx + 4, nx + 4, ny + 4, ny + 4, nz + 4, nz + 4};
     for (int m = 0; m < 6; m++) {

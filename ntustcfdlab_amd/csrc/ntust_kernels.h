@@ -209,12 +209,14 @@ void launch_rb_sweep32(const KGrid &g, const SplitGrid &s, const FMetric &fm, fl
                        double itmax, int t, double *slots, SorControl *ctl, int *dirty,
                        int first, hipStream_t st);
 // speculative sweeps (no gate): sweep t runs and records its maximum in
-// the buckets of slot t, which launch_slots_clear has zeroed; the host reads
-// the maxima (launch_slots_max) and replays from a checkpoint if the
+// the buckets of slot t, which launch_slots_clear has zeroed -- or, with
+// flags non-null, only whether it exceeds zeta (flags[t] = 1, flags zeroed
+// by the caller); the host reads them and replays from a checkpoint if the
 // reference's loop would have stopped earlier
 void launch_rb_sweep_spec(const KGrid &g, const SplitGrid &s, const double *rx, const double *ry,
                           const double *rz, double *p0, double *p1, const double *b0,
-                          const double *b1, double omega, int t, double *slots, hipStream_t st);
+                          const double *b1, double omega, int t, double *slots, double zeta,
+                          int *flags, hipStream_t st);
 // one launch per sweep on 3-D tiles (k_sor_tile.hip); arguments as the
 // z-streaming fused sweeps
 void launch_rb_sweep_tile(const KGrid &g, const SplitGrid &s, const double *rx, const double *ry,
@@ -229,9 +231,8 @@ void launch_rb_sweep32_tile(const KGrid &g, const SplitGrid &s, const FMetric &f
                             hipStream_t st);
 void launch_rb_sweep32_spec(const KGrid &g, const SplitGrid &s, const FMetric &fm, float *e0,
                             float *e1, const float *r0, const float *r1, float omega, int t,
-                            double *slots, hipStream_t st);
+                            double *slots, double zeta, int *flags, hipStream_t st);
 void launch_slots_clear(double *slots, int t0, int n, hipStream_t st);
-void launch_slots_max(double *slots, int t0, int n, double *m, hipStream_t st);
 // one launch per sweep (red then black on LDS tiles), reading copy "in" of
 // the colour planes and writing copy "out"
 void launch_rb_sweep_fused(const KGrid &g, const SplitGrid &s, const double *rx,
