@@ -229,28 +229,41 @@ long spec_batch(long tmax, long quantum)
     return nb >= 2 * quantum ? nb : 0;
 }
 
-// The speculative batch of sweeps 1..nb recorded, for sweeps 1..nb-1, only
+// A speculative batch of sweeps t0..t1 recorded, for sweeps t0..t1-1, only
 // whether each one's maximum change exceeded zeta (flags, zeroed before the
-// batch), and for sweep nb its maximum in the buckets (the next gated sweep
-// and the reported maximum read it).  Index (0-based) of the first sweep
-// whose maximum ends the loop (not > zeta; a NaN maximum ends it too), or -1.
-int spec_stop(const double *slots, const int *flags, long nb, double zeta, long *q)
+// batch), and for sweep t1 its maximum in the buckets (the next gated sweep
+// and the reported maximum read it).  *stop: the first of these sweeps
+// whose maximum ends the loop (not > zeta; a NaN maximum ends it too), or
+// -1.
+int spec_stop(const double *slots, const int *flags, long t0, long t1, double zeta,
+              long *stop)
 {
-    std::vector<int> f((size_t)nb + 1);
+    std::vector<int> f((size_t)(t1 - t0 + 1));
     double bk[SLOT_BUCKETS * SLOT_STRIDE];
-    HIPCHK(hipMemcpyAsync(f.data(), flags, (nb + 1) * sizeof(int), hipMemcpyDeviceToHost, C.st));
-    HIPCHK(hipMemcpyAsync(bk, slots + slot_index(nb, 0), sizeof bk, hipMemcpyDeviceToHost, C.st));
+    HIPCHK(hipMemcpyAsync(f.data(), flags + t0, (t1 - t0 + 1) * sizeof(int),
+                          hipMemcpyDeviceToHost, C.st));
+    HIPCHK(hipMemcpyAsync(bk, slots + slot_index(t1, 0), sizeof bk, hipMemcpyDeviceToHost, C.st));
     TRY(sync());
-    *q = -1;
-    for (long i = 0; i + 1 < nb; i++)
-        if (f[(size_t)i + 1] == 0) {
-            *q = i;
+    *stop = -1;
+    for (long t = t0; t < t1; t++)
+        if (f[(size_t)(t - t0)] == 0) {
+            *stop = t;
             return 0;
         }
     double m = 0.0;
     for (int b = 0; b < SLOT_BUCKETS; b++) m = bk[b * SLOT_STRIDE] > m ? bk[b * SLOT_STRIDE] : m;
-    if (!(m > zeta)) *q = nb - 1;
+    if (!(m > zeta)) *stop = t1;
     return 0;
+}
+
+// end of the second speculative batch: the previous solve's count plus an
+// eighth, in whole segments of quantum sweeps from sweep 1, within tmax
+long spec_tail(long tmax, long quantum)
+{
+    long hi = (long)C.ik_hint + C.ik_hint / 8;
+    hi = (hi + quantum - 1) / quantum * quantum;
+    hi = hi < tmax ? hi : tmax;
+    return hi < SPEC_MAX ? hi : SPEC_MAX;
 }
 
 int solve_pressure(const double *us, const double *vs, const double *ws, double *p,
@@ -353,8 +366,8 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
             // set-up) are timed, up to NSEV segments
             int nev = 0;
             long tlaunch = 0;
-            auto run = [&](long upto, bool timed) {
-                for (long tt = 1; tt <= upto; tt += seg) {
+            auto run = [&](long from, long upto, bool timed) {
+                for (long tt = from; tt <= upto; tt += seg) {
                     const long n = seg < upto - tt + 1 ? seg : upto - tt + 1;
                     const bool ev = timed && tt > 1 && nev < Ctx::NSEV;
                     if (ev) HIPCHK(hipEventRecord(C.sev[2 * nev], C.st));
@@ -372,17 +385,22 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
                 }
                 return 0;
             };
-            launch_slots_clear(slots, 1, (int)nb1 + 1, C.st);
-            TRY(run(nb1, true));
-            long q;
-            TRY(spec_stop(slots, flags, nb1, zeta, &q));
-            if (q >= 0 && q < nb1 - 1) {
-                launch_rb_gather(g, C.sg, p, P0, P1, C.st);
+            // the state after a segment's correction, from the FP64 planes
+            auto restart = [&]() {
                 HIPCHK(hipMemsetAsync(E0, 0, np * sizeof(float), C.st));
                 HIPCHK(hipMemsetAsync(E1, 0, np * sizeof(float), C.st));
                 launch_rb_residual(g, C.sg, rx, ry, rz, P0, P1, B0, B1, R0, R1, E1, nullptr,
                                    nullptr, C.st);
-                TRY(run(q + 1, false));
+                return 0;
+            };
+            launch_slots_clear(slots, 1, (int)nb1 + 1, C.st);
+            TRY(run(1, nb1, true));
+            long q;
+            TRY(spec_stop(slots, flags, 1, nb1, zeta, &q));
+            if (q >= 1 && q < nb1) {
+                launch_rb_gather(g, C.sg, p, P0, P1, C.st);
+                TRY(restart());
+                TRY(run(1, q, false));
                 HIPCHK(hipGetLastError());
                 TRY(sync());
             } else {
@@ -393,8 +411,34 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
                 }
                 C.prof_launches += tlaunch;
             }
-            h.ik = (int)(q >= 0 ? q + 1 : nb1);
-            t = q >= 0 ? tmax + 1 : nb1 + 1;
+            h.ik = (int)(q >= 1 ? q : nb1);
+            t = q >= 1 ? tmax + 1 : nb1 + 1;
+            // the tail, to the previous count plus an eighth, speculative too:
+            // from a copy of the FP64 planes, so that a stop inside it replays
+            // from there
+            const long hi = t <= tmax ? spec_tail(tmax, seg) : 0;
+            if (hi > nb1) {
+                NEED(k0d, "mix_ck0", np, false);
+                NEED(k1d, "mix_ck1", np, false);
+                HIPCHK(hipMemcpyAsync(k0d, P0, np * sizeof(double), hipMemcpyDeviceToDevice, C.st));
+                HIPCHK(hipMemcpyAsync(k1d, P1, np * sizeof(double), hipMemcpyDeviceToDevice, C.st));
+                HIPCHK(hipMemsetAsync(flags + nb1 + 1, 0, (hi - nb1) * sizeof(int), C.st));
+                launch_slots_clear(slots, (int)nb1 + 1, (int)(hi - nb1) + 1, C.st);
+                TRY(run(nb1 + 1, hi, false));
+                TRY(spec_stop(slots, flags, nb1 + 1, hi, zeta, &q));
+                if (q >= nb1 + 1 && q < hi) {
+                    HIPCHK(hipMemcpyAsync(P0, k0d, np * sizeof(double), hipMemcpyDeviceToDevice,
+                                          C.st));
+                    HIPCHK(hipMemcpyAsync(P1, k1d, np * sizeof(double), hipMemcpyDeviceToDevice,
+                                          C.st));
+                    TRY(restart());
+                    TRY(run(nb1 + 1, q, false));
+                    HIPCHK(hipGetLastError());
+                    TRY(sync());
+                }
+                h.ik = (int)(q >= 1 ? q : hi);
+                t = q >= 1 ? tmax + 1 : hi + 1;
+            }
             want = 2 * seg;
         }
         if (graphs) {
@@ -494,20 +538,20 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
         NEED(fl, "sor_flags", (size_t)SPEC_MAX / 2 + 2, false);
         int *flags = (int *)fl;
         HIPCHK(hipMemsetAsync(flags, 0, (nb64 + 1) * sizeof(int), C.st));
-        auto run = [&](long upto) {
-            for (long tt = 1; tt <= upto; tt++)
+        auto run = [&](long from, long upto) {
+            for (long tt = from; tt <= upto; tt++)
                 launch_rb_sweep_spec(g, C.sg, rx, ry, rz, P0, P1, B0, B1, omega, (int)tt, slots,
                                      zeta, tt < upto ? flags : nullptr, C.st);
         };
         launch_slots_clear(slots, 1, (int)nb64 + 1, C.st);
         HIPCHK(hipEventRecord(C.ev0, C.st));
-        run(nb64);
+        run(1, nb64);
         HIPCHK(hipEventRecord(C.ev1, C.st));
         long q;
-        TRY(spec_stop(slots, flags, nb64, zeta, &q));
-        if (q >= 0 && q < nb64 - 1) {
+        TRY(spec_stop(slots, flags, 1, nb64, zeta, &q));
+        if (q >= 1 && q < nb64) {
             launch_rb_gather(g, C.sg, p, P0, P1, C.st);
-            run(q + 1);
+            run(1, q);
             HIPCHK(hipGetLastError());
             TRY(sync());
         } else {
@@ -516,8 +560,33 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
             C.prof_ms += ms;
             C.prof_launches += 2 * nb64;
         }
-        h.ik = (int)(q >= 0 ? q + 1 : nb64);
-        t = q >= 0 ? tmax + 1 : nb64 + 1;
+        h.ik = (int)(q >= 1 ? q : nb64);
+        t = q >= 1 ? tmax + 1 : nb64 + 1;
+        // the tail, to the previous count plus an eighth, speculative too:
+        // from a copy of the colour planes, so that a stop inside it replays
+        // from there
+        const long hi = t <= tmax ? spec_tail(tmax, 8) : 0;
+        if (hi > nb64) {
+            NEED(k0d, "sor_ck0", np, false);
+            NEED(k1d, "sor_ck1", np, false);
+            HIPCHK(hipMemcpyAsync(k0d, P0, np * sizeof(double), hipMemcpyDeviceToDevice, C.st));
+            HIPCHK(hipMemcpyAsync(k1d, P1, np * sizeof(double), hipMemcpyDeviceToDevice, C.st));
+            HIPCHK(hipMemsetAsync(flags + nb64 + 1, 0, (hi - nb64) * sizeof(int), C.st));
+            launch_slots_clear(slots, (int)nb64 + 1, (int)(hi - nb64) + 1, C.st);
+            run(nb64 + 1, hi);
+            TRY(spec_stop(slots, flags, nb64 + 1, hi, zeta, &q));
+            if (q >= nb64 + 1 && q < hi) {
+                HIPCHK(hipMemcpyAsync(P0, k0d, np * sizeof(double), hipMemcpyDeviceToDevice,
+                                      C.st));
+                HIPCHK(hipMemcpyAsync(P1, k1d, np * sizeof(double), hipMemcpyDeviceToDevice,
+                                      C.st));
+                run(nb64 + 1, q);
+                HIPCHK(hipGetLastError());
+                TRY(sync());
+            }
+            h.ik = (int)(q >= 1 ? q : hi);
+            t = q >= 1 ? tmax + 1 : hi + 1;
+        }
         batch = 16;
     }
     if (graphs && t <= tmax) {
