@@ -256,14 +256,14 @@ int spec_stop(const double *slots, const int *flags, long t0, long t1, double ze
     return 0;
 }
 
-// end of the second speculative batch: the previous solve's count plus an
-// eighth, in whole segments of quantum sweeps from sweep 1, within tmax
-long spec_tail(long tmax, long quantum)
+// end of the second speculative batch: itmax, when the previous solve ran
+// to itmax (a solve that the stopping rule ends varies its count from step
+// to step, and a tail past its stop would be replayed almost every time);
+// else 0, no tail
+long spec_tail(long tmax)
 {
-    long hi = (long)C.ik_hint + C.ik_hint / 8;
-    hi = (hi + quantum - 1) / quantum * quantum;
-    hi = hi < tmax ? hi : tmax;
-    return hi < SPEC_MAX ? hi : SPEC_MAX;
+    if ((long)C.ik_hint < tmax) return 0;
+    return tmax < SPEC_MAX ? tmax : SPEC_MAX;
 }
 
 int solve_pressure(const double *us, const double *vs, const double *ws, double *p,
@@ -413,10 +413,10 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
             }
             h.ik = (int)(q >= 1 ? q : nb1);
             t = q >= 1 ? tmax + 1 : nb1 + 1;
-            // the tail, to the previous count plus an eighth, speculative too:
-            // from a copy of the FP64 planes, so that a stop inside it replays
-            // from there
-            const long hi = t <= tmax ? spec_tail(tmax, seg) : 0;
+            // after a solve that ran to itmax, the tail to itmax speculative
+            // too: from a copy of the FP64 planes, so that a stop inside it
+            // replays from there
+            const long hi = t <= tmax ? spec_tail(tmax) : 0;
             if (hi > nb1) {
                 NEED(k0d, "mix_ck0", np, false);
                 NEED(k1d, "mix_ck1", np, false);
@@ -562,10 +562,10 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
         }
         h.ik = (int)(q >= 1 ? q : nb64);
         t = q >= 1 ? tmax + 1 : nb64 + 1;
-        // the tail, to the previous count plus an eighth, speculative too:
+        // after a solve that ran to itmax, the tail to itmax speculative too:
         // from a copy of the colour planes, so that a stop inside it replays
         // from there
-        const long hi = t <= tmax ? spec_tail(tmax, 8) : 0;
+        const long hi = t <= tmax ? spec_tail(tmax) : 0;
         if (hi > nb64) {
             NEED(k0d, "sor_ck0", np, false);
             NEED(k1d, "sor_ck1", np, false);
