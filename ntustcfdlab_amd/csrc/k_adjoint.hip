@@ -425,8 +425,8 @@ k_adj_black_sum(KGrid g, SplitGrid s, const double *__restrict__ rx,
 //   r = S(z0) - aP z0            (the forward residual with b = 0)
 //   e <- e + omega ((S(e) + r)/aP - e)
 // and (ACC, the red half-sweeps) Ef sums e after every half-sweep, so that
-// at the end of the segment Z += n z0 + Ef and z0 += e (k_adj_fold); the
-// black sum is derived there from Ef (k_adj_fold_black).  The black
+// at the end of the segment Z += n z0 + Ef and z0 += e (k_adj_seg_red); the
+// black sum is derived there from Ef (k_adj_seg_black).  The black
 // correction runs shifted as in the forward, e1' = e1 - s1 with s1 = r1/aP
 // in the r1 plane and the red residual r0 + S(s1) (launch_rb_residual with
 // the shift), so the black half-sweeps read no residual.  The FP32
@@ -493,22 +493,10 @@ k_adj_sweep32(KGrid g, SplitGrid s, FMetric fm, float *__restrict__ e_own,
     }
 }
 
-// end of a segment, one colour plane: Z += n z0 + Ef, z0 += e, e = Ef = 0
-// (every entry; the ghost and padding entries are all zero and stay so)
-__global__ void __launch_bounds__(256)
-k_adj_fold(long n, double nacc, double *__restrict__ z, float *__restrict__ e,
-           float *__restrict__ Ef, double *__restrict__ Z)
-{
-    const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= n) return;
-    const double z0 = z[x];
-    Z[x] = Z[x] + (nacc * z0 + (double)Ef[x]);
-    z[x] = z0 + (double)e[x];
-    e[x] = 0.0f;
-    Ef[x] = 0.0f;
-}
-
-// end of a segment, black plane (before the red fold clears e0 and Ef0).
+// A segment's end, fused with the next segment's start, in two colour
+// launches (k_adj_seg_black, then k_adj_seg_red); with next = 0 (the last
+// segment) only the folds.
+// Colour 1, the black fold (before the red fold clears e0 and Ef0).
 // The black half-sweeps do not accumulate: summing the black update, in
 // the shifted correction e1' = e1 - s1 (s1 in the r1 plane),
 //   e1' <- (1-omega) e1' + omega S(e0)/aP,   e1' = -s1 at the start,
@@ -517,14 +505,20 @@ k_adj_fold(long n, double nacc, double *__restrict__ z, float *__restrict__ e,
 // E the sum of the red corrections they read: Ef0, less the last red
 // correction when the segment ends with a red update (ends_red).  tail is
 // (1-omega)/omega, plus 1 when the last black update does not count.
-// Then Z1 += nacc z1 + sum, z1 += e1_end = e1' + s1, e1' = 0 on the
-// interior black cells (the ghost and padding entries are zero and stay
-// so).  Arithmetic in FP64 on the FP32 metric the sweeps use.
+// Then Z1 += nacc z1 + sum and z1 += e1_end = e1' + s1 on the interior
+// black cells (the ghost and padding entries are zero and stay so), in
+// FP64 arithmetic on the FP32 metric the sweeps use.  Then e1' = 0 (last
+// segment), or the next segment's shifted residual
+// r1 = S(z0 + e0) - aP z1 (b = 0; the red fold is not done yet, so the red
+// values are formed on the fly) with the FP64 metric as launch_rb_residual:
+// s1 = r1/aP into the r1 plane and e1' = -s1.
 __global__ void __launch_bounds__(256)
-k_adj_fold_black(KGrid g, SplitGrid s, FMetric fm, double nb, double nacc, double tail,
-                 int ends_red, const float *__restrict__ Ef0, const float *__restrict__ e0,
-                 const float *__restrict__ s1, float *__restrict__ e1,
-                 double *__restrict__ z1, double *__restrict__ Z1)
+k_adj_seg_black(KGrid g, SplitGrid s, FMetric fm, const double *__restrict__ rx,
+                const double *__restrict__ ry, const double *__restrict__ rz, double nb,
+                double nacc, double tail, int ends_red, int next,
+                const float *__restrict__ Ef0, const float *__restrict__ e0,
+                const double *__restrict__ z0, float *__restrict__ s1, float *__restrict__ e1,
+                double *__restrict__ z1, double *__restrict__ Z1)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
@@ -534,24 +528,96 @@ k_adj_fold_black(KGrid g, SplitGrid s, FMetric fm, double nb, double nacc, doubl
     const long row = (long)s.mp * ((long)j + (long)s.ry * k);
     const long o = row + q;
     const long pl = (long)s.mp * s.ry;
+    // W, E, S, N, B, F
     const long at[6] = {row + ((i - 1) >> 1), row + ((i + 1) >> 1), o - s.mp, o + s.mp,
                         o - pl, o + pl};
-    double E[6];
+    double E[6], zr[6];
 #pragma unroll
-    for (int n = 0; n < 6; n++)
-        E[n] = (double)Ef0[at[n]] - (ends_red ? (double)e0[at[n]] : 0.0);
-    const double ax = fm.iDx[i], ay = fm.iDy[j], az = fm.iDz[k];
-    const double rW = fm.rx[i - 1], rE = fm.rx[i], rS = fm.ry[j - 1], rN = fm.ry[j];
-    const double rB = fm.rz[k - 1], rF = fm.rz[k];
-    const double sum = ay * az * (rE * E[1] + rW * E[0]) + ax * az * (rN * E[3] + rS * E[2]) +
-                       ax * ay * (rF * E[5] + rB * E[4]);
-    const double aP = ay * az * (rE + rW) + ax * az * (rN + rS) + ax * ay * (rF + rB);
-    const double sh = (double)s1[o];
-    const double e = (double)e1[o] + sh;
-    const double z = z1[o];
-    Z1[o] = Z1[o] + (nacc * z + ((sum / aP + nb * sh) - tail * e));
-    z1[o] = z + e;
-    e1[o] = 0.0f;
+    for (int n = 0; n < 6; n++) {
+        const double ee = (double)e0[at[n]];
+        E[n] = (double)Ef0[at[n]] - (ends_red ? ee : 0.0);
+        zr[n] = z0[at[n]] + ee;   // ghost entries: 0 + 0
+    }
+    {
+        const double ax = fm.iDx[i], ay = fm.iDy[j], az = fm.iDz[k];
+        const double rW = fm.rx[i - 1], rE = fm.rx[i], rS = fm.ry[j - 1], rN = fm.ry[j];
+        const double rB = fm.rz[k - 1], rF = fm.rz[k];
+        const double sum = ay * az * (rE * E[1] + rW * E[0]) +
+                           ax * az * (rN * E[3] + rS * E[2]) + ax * ay * (rF * E[5] + rB * E[4]);
+        const double aP = ay * az * (rE + rW) + ax * az * (rN + rS) + ax * ay * (rF + rB);
+        const double sh = (double)s1[o];
+        const double e = (double)e1[o] + sh;
+        const double z = z1[o];
+        Z1[o] = Z1[o] + (nacc * z + ((sum / aP + nb * sh) - tail * e));
+        z1[o] = z + e;
+    }
+    if (!next) {
+        e1[o] = 0.0f;
+        return;
+    }
+    const double ax = g.iDx[i], ay = g.iDy[j], az = g.iDz[k];
+    const double rW = rx[i - 1], rE = rx[i], rS = ry[j - 1], rN = ry[j];
+    const double rB = rz[k - 1], rF = rz[k];
+    const double ayz = ay * az, axz = ax * az, axy = ax * ay;
+    const double t1 = fma(rE, zr[1], rW * zr[0]);
+    const double t2 = fma(rN, zr[3], rS * zr[2]);
+    const double t3 = fma(rF, zr[5], rB * zr[4]);
+    const double sum = fma(ayz, t1, fma(axz, t2, axy * t3));
+    const double aP = fma(ayz, rE + rW, fma(axz, rN + rS, axy * (rF + rB)));
+    const double res = (sum - 0.0) - aP * z1[o];
+    const float sv = (float)(res / aP);
+    s1[o] = sv;
+    e1[o] = -sv;
+}
+
+// Colour 0: the red fold (Z0 += nr z0 + Ef0, z0 += e0, e0 = Ef0 = 0), then
+// the next segment's residual r0 = S(z1) - aP z0 + S(s1) (b = 0; the black
+// values already folded, s1 read as -e1'), FP64 metric.
+__global__ void __launch_bounds__(256)
+k_adj_seg_red(KGrid g, SplitGrid s, const double *__restrict__ rx,
+              const double *__restrict__ ry, const double *__restrict__ rz, double nr, int next,
+              float *__restrict__ e0, float *__restrict__ Ef0, double *__restrict__ z0,
+              double *__restrict__ Z0, const double *__restrict__ z1,
+              const float *__restrict__ e1, float *__restrict__ r0)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    const int i = 2 * q + ((j + k) & 1);
+    if (i < 1 || i > g.nx || j > g.ny) return;
+    const long row = (long)s.mp * ((long)j + (long)s.ry * k);
+    const long o = row + q;
+    const long pl = (long)s.mp * s.ry;
+    const double z = z0[o];
+    Z0[o] = Z0[o] + (nr * z + (double)Ef0[o]);
+    const double zn = z + (double)e0[o];
+    z0[o] = zn;
+    e0[o] = 0.0f;
+    Ef0[o] = 0.0f;
+    if (!next) return;
+    const long at[6] = {row + ((i - 1) >> 1), row + ((i + 1) >> 1), o - s.mp, o + s.mp,
+                        o - pl, o + pl};
+    double zb[6], eb[6];
+#pragma unroll
+    for (int n = 0; n < 6; n++) {
+        zb[n] = z1[at[n]];
+        eb[n] = (double)e1[at[n]];
+    }
+    const double ax = g.iDx[i], ay = g.iDy[j], az = g.iDz[k];
+    const double rW = rx[i - 1], rE = rx[i], rS = ry[j - 1], rN = ry[j];
+    const double rB = rz[k - 1], rF = rz[k];
+    const double ayz = ay * az, axz = ax * az, axy = ax * ay;
+    const double t1 = fma(rE, zb[1], rW * zb[0]);
+    const double t2 = fma(rN, zb[3], rS * zb[2]);
+    const double t3 = fma(rF, zb[5], rB * zb[4]);
+    const double sum = fma(ayz, t1, fma(axz, t2, axy * t3));
+    const double aP = fma(ayz, rE + rW, fma(axz, rN + rS, axy * (rF + rB)));
+    double res = (sum - 0.0) - aP * zn;
+    const double u1 = fma(rE, eb[1], rW * eb[0]);
+    const double u2 = fma(rN, eb[3], rS * eb[2]);
+    const double u3 = fma(rF, eb[5], rB * eb[4]);
+    res -= fma(ayz, u1, fma(axz, u2, axy * u3));
+    r0[o] = (float)res;
 }
 
 // colour planes -> lp (interior: adjoint of the initial guess, the previous
@@ -1027,12 +1093,13 @@ void launch_adj_pressure(const KGrid &g, const SplitGrid &s, const double *rx, c
                            1.0, omega);
         const int nq32 = (g.nx + 1) / 2;
         const dim3 g32((nq32 + 63) / 64, (g.ny + 3) / 4, (g.nz + RB_KPT - 1) / RB_KPT);
-        const dim3 gf((unsigned)((s.plane + 255) / 256));
         const long nh = 2L * sweeps - 1;   // half-sweeps after the first
+        // the first segment's residual; each segment's end computes the next
+        // one's (k_adj_seg_black, k_adj_seg_red)
+        launch_rb_residual(g, s, rx, ry, rz, z0, z1, mx->bzero, mx->bzero, mx->r0, mx->r1,
+                           mx->e1, nullptr, nullptr, st);
         for (long h0 = 0; h0 < nh; h0 += mx->seg) {
             const long h1 = h0 + mx->seg < nh ? h0 + mx->seg : nh;
-            launch_rb_residual(g, s, rx, ry, rz, z0, z1, mx->bzero, mx->bzero, mx->r0, mx->r1,
-                               mx->e1, nullptr, nullptr, st);
             for (long h = h0; h < h1; h++) {
                 if (h & 1)
                     hipLaunchKernelGGL((k_adj_sweep32<0, true>), g32, blk, 0, st, g, s, mx->fm,
@@ -1043,12 +1110,12 @@ void launch_adj_pressure(const KGrid &g, const SplitGrid &s, const double *rx, c
             }
             const long nb = (h1 - h0 + 1) / 2, nr = (h1 - h0) / 2;
             const bool last = h1 == nh;   // ends with the black update that does not count
-            hipLaunchKernelGGL(k_adj_fold_black, grs, blk, 0, st, g, s, mx->fm, (double)nb,
-                               (double)(last ? nb - 1 : nb),
-                               (1.0 - omega) / omega + (last ? 1.0 : 0.0), last ? 0 : 1, mx->f0,
-                               mx->e0, mx->r1, mx->e1, z1, Z1);
-            hipLaunchKernelGGL(k_adj_fold, gf, dim3(256), 0, st, s.plane, (double)nr, z0, mx->e0,
-                               mx->f0, Z0);
+            hipLaunchKernelGGL(k_adj_seg_black, grs, blk, 0, st, g, s, mx->fm, rx, ry, rz,
+                               (double)nb, (double)(last ? nb - 1 : nb),
+                               (1.0 - omega) / omega + (last ? 1.0 : 0.0), last ? 0 : 1,
+                               last ? 0 : 1, mx->f0, mx->e0, z0, mx->r1, mx->e1, z1, Z1);
+            hipLaunchKernelGGL(k_adj_seg_red, grs, blk, 0, st, g, s, rx, ry, rz, (double)nr,
+                               last ? 0 : 1, mx->e0, mx->f0, z0, Z0, z1, mx->e1, mx->r0);
         }
     }
     if (sweeps > 0) {
