@@ -7,13 +7,9 @@
 // values held fixed during the solve.
 //
 // Two orderings:
-//  * lexicographic (exact): the reference's k, j, i order.  Cell (i,j,k)
-//    reads the new values of its three lower neighbours and the old values
-//    of its three upper ones, so all cells on one hyperplane i+j+k = s
-//    depend only on plane s-1; sweeping the planes in increasing s inside
-//    one workgroup reproduces the in-place lexicographic sweep bit for bit.
-//    The whole do-while loop runs in a single kernel launch.
-//  * red-black (fast): colour (i+j+k)&1, red first, on colour-split copies
+//  * lexicographic (exact): the reference's k, j, i order, pipelined over
+//    sweeps on the colour-split planes below (k_sor_lex.hip).
+//  * red-black: colour (i+j+k)&1, red first, on colour-split copies
 //    of p and of the right-hand side b = mChange/dt, so each half-sweep
 //    streams one colour plane of p and b and reads the other colour through
 //    the caches.  Each half-sweep kernel decides by itself whether its sweep
@@ -23,66 +19,6 @@
 #include "ntust_kernels.h"
 
 namespace ntust {
-
-// ---------------------------------------------------------------------
-// exact lexicographic SOR
-// ---------------------------------------------------------------------
-
-__global__ void __launch_bounds__(1024)
-k_sor_lexicographic(KGrid g, const double *__restrict__ us, const double *__restrict__ vs,
-                    const double *__restrict__ ws, double *p, double dt, double omega,
-                    double zeta, double itmax, SorControl *ctl, double *pmax_out)
-{
-    __shared__ double red[16];
-    const int nx = g.nx, ny = g.ny, nz = g.nz;
-    const int tid = threadIdx.x;
-    double last = 1.0;   // pChangeMax_ before the first sweep (GS.f90:9)
-    int ik = 0;
-    while (last > zeta && (double)ik < itmax) {
-        ik++;
-        double lmax = 0.0;
-        for (int s = 3; s <= nx + ny + nz; s++) {
-            const int klo = max(1, s - nx - ny), khi = min(nz, s - 2);
-            const int cnt = ny * (khi - klo + 1);
-            for (int q = tid; q < cnt; q += blockDim.x) {
-                const int k = q / ny + klo;
-                const int j = q - (k - klo) * ny + 1;
-                const int i = s - j - k;
-                if (i < 1 || i > nx) continue;
-                const double ax = g.iDx[i], ay = g.iDy[j], az = g.iDz[k];
-                const long o = hix(g, i, j, k);
-                const double mChange = (us[o] - us[o - 1]) * ay * az
-                                     + (vs[o] - vs[o - g.sx]) * ax * az
-                                     + (ws[o] - ws[o - g.sxy]) * ax * ay;
-                const double pNew = (-p[o + 1] * ay * az / g.Dxs[i]
-                                     - p[o - 1] * ay * az / g.Dxs[i - 1]
-                                     - p[o + g.sx] * ax * az / g.Dys[j]
-                                     - p[o - g.sx] * ax * az / g.Dys[j - 1]
-                                     - p[o + g.sxy] * ax * ay / g.Dzs[k]
-                                     - p[o - g.sxy] * ax * ay / g.Dzs[k - 1]
-                                     + mChange / dt)
-                                  / (-ay * az / g.Dxs[i] - ay * az / g.Dxs[i - 1]
-                                     - ax * az / g.Dys[j] - ax * az / g.Dys[j - 1]
-                                     - ax * ay / g.Dzs[k] - ax * ay / g.Dzs[k - 1]);
-                const double pc = fabs(pNew - p[o]);
-                p[o] = p[o] + (omega * (pNew - p[o]));
-                if (pc > lmax) lmax = pc;
-            }
-            __syncthreads();
-        }
-        lmax = wave_max(lmax);
-        if ((tid & 63) == 0) red[tid >> 6] = lmax;
-        __syncthreads();
-        double m = 0.0;
-        for (int q = 0; q < (int)(blockDim.x >> 6); q++) m = red[q] > m ? red[q] : m;
-        __syncthreads();
-        last = m;
-    }
-    if (tid == 0) {
-        ctl->ik = ik;
-        *pmax_out = last;
-    }
-}
 
 __global__ void k_sor_init(double *slots, SorControl *ctl)
 {
@@ -102,15 +38,6 @@ __global__ void k_sor_init(double *slots, SorControl *ctl)
 void launch_sor_init(double *slots, SorControl *ctl, hipStream_t st)
 {
     hipLaunchKernelGGL(k_sor_init, dim3(1), dim3(64), 0, st, slots, ctl);
-}
-
-void launch_sor_lexicographic(const KGrid &g, const double *us, const double *vs,
-                              const double *ws, double *p, double dt, double omega,
-                              double zeta, double itmax, SorControl *ctl, double *pmax,
-                              hipStream_t st)
-{
-    hipLaunchKernelGGL(k_sor_lexicographic, dim3(1), dim3(1024), 0, st, g, us, vs, ws, p, dt,
-                       omega, zeta, itmax, ctl, pmax);
 }
 
 // ---------------------------------------------------------------------

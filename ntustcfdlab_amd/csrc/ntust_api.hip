@@ -76,6 +76,11 @@ struct Ctx {
     hipEvent_t sev[2 * NSEV] = {};
     double prof_ms = 0.0;
     long long prof_launches = 0;
+    long long prof_sweeps = 0;         // lexicographic: sweeps in the timed launches
+    // lexicographic solver: stamp of the current batch's per-sweep flags,
+    // previous solve's sweep count
+    int lex_epoch = 0;
+    int lex_hint = 0;
     // red-black sweep kernel: one launch per sweep (fused) or per colour,
     // the latter from the stream or replayed from a captured graph
     int sweep_variant = NTUST_SWEEP_TWO_PASS;
@@ -266,25 +271,139 @@ long spec_tail(long tmax)
     return tmax < SPEC_MAX ? tmax : SPEC_MAX;
 }
 
+// GS.f90's lexicographic SOR (k_sor_lex.hip): sweeps in batches, every
+// batch speculative.  Batch t0..t1 ends with the host reading, for sweeps
+// t0..t1-1, whether each one changed some cell by more than zeta (its
+// stamp), and the maximum change of sweep t1.  The loop of GS.f90:11 stops
+// after the first sweep without a stamp (or after t1 when its maximum is
+// not above zeta); if that sweep lies inside the batch, the colour planes
+// go back to the batch's start -- gathered again from p, which the solve
+// leaves alone until the end, or the copy taken before the batch -- and the
+// batch runs again, exactly to that sweep.  First batch: 7/8 of the
+// previous solve's count, or all of itmax when that solve ran to itmax.
+int lex_stop(const int *flags, const double *slots, long t0, long t1, int epoch, double zeta,
+             long *stop)
+{
+    std::vector<int> f((size_t)(t1 - t0 + 1));
+    double bk[SLOT_BUCKETS * SLOT_STRIDE];
+    if (t1 > t0)
+        HIPCHK(hipMemcpyAsync(f.data(), flags + t0, (t1 - t0) * sizeof(int),
+                              hipMemcpyDeviceToHost, C.st));
+    HIPCHK(hipMemcpyAsync(bk, slots + slot_index(t1, 0), sizeof bk, hipMemcpyDeviceToHost, C.st));
+    TRY(sync());
+    *stop = -1;
+    for (long t = t0; t < t1; t++)
+        if (f[(size_t)(t - t0)] != epoch) {
+            *stop = t;
+            return 0;
+        }
+    double m = 0.0;
+    for (int b = 0; b < SLOT_BUCKETS; b++) m = bk[b * SLOT_STRIDE] > m ? bk[b * SLOT_STRIDE] : m;
+    if (!(m > zeta)) *stop = t1;
+    return 0;
+}
+
+int solve_lexicographic(const double *us, const double *vs, const double *ws, double *p,
+                        double *pre, double dt, double omega, double zeta, double itmax,
+                        int *ik_out, double *pmax_out)
+{
+    const KGrid &g = C.g;
+    const long tmax = (long)fmin(itmax, 1.0e8) > 0 ? (long)fmin(itmax, 1.0e8) : 0;
+    if (!(1.0 > zeta) || tmax < 1) {
+        // GS.f90:11 runs no sweep; pre = p (:86-94)
+        if (pre) launch_pack_interior(g, p, pre, C.st);
+        HIPCHK(hipGetLastError());
+        TRY(sync());
+        *ik_out = 0;
+        *pmax_out = 1.0;
+        return 0;
+    }
+    const size_t np = (size_t)C.sg.plane;
+    NEED(P0, "sor_p0", np, true);
+    NEED(P1, "sor_p1", np, true);
+    NEED(B0, "sor_b0", np, true);
+    NEED(B1, "sor_b1", np, true);
+    NEED(fl, "lex_flags", (size_t)tmax / 2 + 2, true);
+    int *flags = (int *)fl;
+    NEED(slots, "sor_slots", (size_t)slot_index(tmax + 2, 0), false);
+    launch_rb_gather(g, C.sg, p, P0, P1, C.st);
+    launch_rb_rhs(g, C.sg, us, vs, ws, dt, B0, B1, C.st);
+    const long hint = C.lex_hint;
+    long nb = hint >= tmax ? tmax : (hint > 0 ? hint - hint / 8 : 32);
+    long t0 = 1, ik = 0;
+    double *K0 = nullptr, *K1 = nullptr;
+    for (;;) {
+        const long t1 = t0 + (nb > 1 ? nb : 1) - 1 < tmax ? t0 + (nb > 1 ? nb : 1) - 1 : tmax;
+        if (t0 > 1) {
+            if (!K0) {
+                NEED(a, "lex_k0", np, false);
+                NEED(b, "lex_k1", np, false);
+                K0 = a;
+                K1 = b;
+            }
+            HIPCHK(hipMemcpyAsync(K0, P0, np * sizeof(double), hipMemcpyDeviceToDevice, C.st));
+            HIPCHK(hipMemcpyAsync(K1, P1, np * sizeof(double), hipMemcpyDeviceToDevice, C.st));
+        }
+        int ep = ++C.lex_epoch;
+        launch_slots_clear(slots, (int)t1, 1, C.st);
+        HIPCHK(hipEventRecord(C.ev0, C.st));
+        launch_lex_sweeps(g, C.sg, P0, P1, B0, B1, omega, zeta, (int)t0, (int)t1, flags, ep,
+                          slots, C.st);
+        HIPCHK(hipEventRecord(C.ev1, C.st));
+        HIPCHK(hipGetLastError());
+        long q;
+        TRY(lex_stop(flags, slots, t0, t1, ep, zeta, &q));
+        if (q >= t0 && q < t1) {
+            if (t0 == 1) {
+                launch_rb_gather(g, C.sg, p, P0, P1, C.st);
+            } else {
+                HIPCHK(hipMemcpyAsync(P0, K0, np * sizeof(double), hipMemcpyDeviceToDevice,
+                                      C.st));
+                HIPCHK(hipMemcpyAsync(P1, K1, np * sizeof(double), hipMemcpyDeviceToDevice,
+                                      C.st));
+            }
+            ep = ++C.lex_epoch;
+            launch_slots_clear(slots, (int)q, 1, C.st);
+            launch_lex_sweeps(g, C.sg, P0, P1, B0, B1, omega, zeta, (int)t0, (int)q, flags, ep,
+                              slots, C.st);
+            HIPCHK(hipGetLastError());
+            ik = q;
+            break;
+        }
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, C.ev0, C.ev1));
+        C.prof_ms += ms;
+        C.prof_launches += lex_launches(g, t0, t1);
+        C.prof_sweeps += t1 - t0 + 1;
+        if (q == t1 || t1 == tmax) {
+            ik = t1;
+            break;
+        }
+        t0 = t1 + 1;
+        nb = hint / 4 > 32 ? hint / 4 : 32;
+    }
+    C.lex_hint = (int)ik;
+    double bk[SLOT_BUCKETS * SLOT_STRIDE];
+    HIPCHK(hipMemcpyAsync(bk, slots + slot_index(ik, 0), sizeof bk, hipMemcpyDeviceToHost,
+                          C.st));
+    launch_rb_scatter(g, C.sg, P0, P1, p, pre, C.st);
+    HIPCHK(hipGetLastError());
+    TRY(sync());
+    double m = 0.0;
+    for (int b = 0; b < SLOT_BUCKETS; b++) m = bk[b * SLOT_STRIDE] > m ? bk[b * SLOT_STRIDE] : m;
+    *pmax_out = m;
+    *ik_out = (int)ik;
+    return 0;
+}
+
 int solve_pressure(const double *us, const double *vs, const double *ws, double *p,
                    double *pre, double dt, double omega, double zeta, double itmax, int mode,
                    int *ik_out, double *pmax_out)
 {
     const KGrid &g = C.g;
-    if (mode == NTUST_SOR_LEXICOGRAPHIC) {
-        HIPCHK(hipMemsetAsync(C.ctl, 0, sizeof(SorControl), C.st));
-        launch_sor_lexicographic(g, us, vs, ws, p, dt, omega, zeta, itmax, C.ctl, C.pmax, C.st);
-        HIPCHK(hipGetLastError());
-        SorControl h;
-        HIPCHK(hipMemcpyAsync(&h, C.ctl, sizeof h, hipMemcpyDeviceToHost, C.st));
-        HIPCHK(hipMemcpyAsync(pmax_out, C.pmax, sizeof(double), hipMemcpyDeviceToHost, C.st));
-        // pre = p on the interior (GS.f90:86-94)
-        if (pre) launch_pack_interior(g, p, pre, C.st);
-        HIPCHK(hipGetLastError());
-        TRY(sync());
-        *ik_out = h.ik;
-        return 0;
-    }
+    if (mode == NTUST_SOR_LEXICOGRAPHIC)
+        return solve_lexicographic(us, vs, ws, p, pre, dt, omega, zeta, itmax, ik_out,
+                                   pmax_out);
     if (mode != NTUST_SOR_REDBLACK && mode != NTUST_SOR_MIXED)
         return fail(NTUST_E_ARG, "unsupported SOR mode %d", mode);
 
@@ -1916,6 +2035,7 @@ int ntust_prof_reset(void)
 {
     C.prof_ms = 0.0;
     C.prof_launches = 0;
+    C.prof_sweeps = 0;
     return 0;
 }
 
@@ -1934,6 +2054,16 @@ int ntust_prof_kernel(double *bytes_per_cell)
     // and write p and read b (24 B FP64); FP32: read and write e, read r on
     // the colour-0 cells only (colour 1 runs shifted, with no right-hand
     // side): 8 + 2 = 10 B.  A two-pass launch is half a sweep.
+    if (C.prm.sor == NTUST_SOR_LEXICOGRAPHIC) {
+        // a lexicographic step launch advances about half the cells of the
+        // grid by one sweep: 24 B per cell update, averaged over the timed
+        // launches (the first and last steps of a batch touch fewer cells)
+        if (bytes_per_cell)
+            *bytes_per_cell = C.prof_launches ? 24.0 * (double)C.prof_sweeps /
+                                                    (double)C.prof_launches
+                                              : 12.0;
+        return 6;
+    }
     const bool mixed = C.prm.sor == NTUST_SOR_MIXED;
     const bool tile = C.sweep_variant == NTUST_SWEEP_TILE;
     const bool fused = C.sweep_variant == NTUST_SWEEP_FUSED || tile;

@@ -182,10 +182,15 @@ __device__ __forceinline__ void clear_next_buckets(double *slots, int t, int tid
 {
     if (tid < SLOT_BUCKETS) slots[slot_index(t + 1, tid)] = 0.0;
 }
-void launch_sor_lexicographic(const KGrid &g, const double *us, const double *vs,
-                              const double *ws, double *p, double dt, double omega,
-                              double zeta, double itmax, SorControl *ctl, double *pmax,
-                              hipStream_t st);
+// GS.f90's lexicographic SOR pipelined over sweeps (k_sor_lex.hip): sweeps
+// t0..t1 on the colour planes, the launches tau = 3+2*t0 .. nx+ny+nz+2*t1;
+// flags[t] = epoch for every sweep t < t1 that changed some cell by more
+// than zeta, the maximum change of sweep t1 in its buckets (zeroed by the
+// caller)
+void launch_lex_sweeps(const KGrid &g, const SplitGrid &s, double *p0, double *p1,
+                       const double *b0, const double *b1, double omega, double zeta, int t0,
+                       int t1, int *flags, int epoch, double *slots, hipStream_t st);
+long lex_launches(const KGrid &g, long t0, long t1);
 // members of the Krylov pressure path
 void launch_poisson_coeffs(const KGrid &g, double *coef, hipStream_t st);
 void launch_poisson_rhs(const KGrid &g, const double *us, const double *vs, const double *ws,

@@ -33,7 +33,8 @@ PROF_KERNELS = {0: ("k_rb_sweep<C>", "two-pass FP64 half-sweep"),
                 2: ("k_rb_sweep32<C>", "two-pass FP32 half-sweep"),
                 3: ("k_rb_sweep_fused<float>", "fused FP32 sweep"),
                 4: ("k_rb_sweep_tile<double>", "3-D-tile FP64 sweep"),
-                5: ("k_rb_sweep_tile<float>", "3-D-tile FP32 sweep")}
+                5: ("k_rb_sweep_tile<float>", "3-D-tile FP32 sweep"),
+                6: ("k_lex_sweep<C>", "lexicographic FP64 step: one colour, pipelined sweeps")}
 
 _D = ct.POINTER(ct.c_double)
 _F = ct.POINTER(ct.c_float)

@@ -1,0 +1,204 @@
+"""Parity with GS.f90's own (lexicographic) ordering at the configurations'
+sizes.
+
+The lexicographic GPU solver (k_sor_lex.hip: the reference's k, j, i sweep
+pipelined over sweeps) is compared
+  * bit for bit with the oracle on shapes and stopping patterns that reach
+    every branch of its batching: a stop inside a speculative batch (replayed
+    from the batch's start), a solve that needs a second batch, the itmax cap,
+    no sweep at all, rows wide enough for two x-blocks;
+  * bit for bit with golden fixtures the oracle wrote in the build container
+    (tests/golden/make_golden.py): the reference's own cavity (40^3, Re 10,
+    upwind, 100 steps) step by step, and the first step of the 128^3 LES cavity.
+The red-black FP64 and the FP32-sweep solvers are then measured against it
+(the reference's ordering) on the 128^3 LES cavity after three steps:
+relative velocity difference <= 1e-6 (BASELINE.json's bar for the FP32-sweep
+solver), with the pressure and force-coefficient differences printed."""
+import hashlib
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from conftest import rand_halo, stretched_grid
+from ntustcfdlab_amd.grid import Grid, bc_codes, noise_ic
+from ntustcfdlab_amd.hip import SOR_LEXICOGRAPHIC, SOR_MIXED, SOR_REDBLACK, UPWIND, \
+    QUICK_CENTRE, ProjectionSolver
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def same(a, b):
+    return np.array_equal(np.asarray(a).view(np.uint64), np.asarray(b).view(np.uint64))
+
+
+def digest(a):
+    return hashlib.sha256(np.asfortranarray(a).tobytes(order="F")).hexdigest()
+
+
+# (shape, [(zeta, itmax), ...]) -- the solves run one after the other on one
+# solver, so each starts its first batch from the previous solve's count
+SEQUENCES = {
+    # count 7/8 of the previous one, then a stop inside the batch, a solve
+    # needing a second batch, a cap inside the first batch, no sweep
+    "12x10x9": ((12, 10, 9), [(1e-7, 400.0), (1e-4, 400.0), (1e-9, 400.0), (1e-12, 60.0),
+                              (1e-12, 400.0), (2.0, 50.0), (1e-8, 400.0)]),
+    # two x-blocks of 64 cell pairs: lanes 60..63 carry sweeps through the
+    # high word of the per-wavefront stamp mask
+    "130x5x9": ((130, 5, 9), [(1e-6, 200.0), (1e-8, 200.0), (1e-5, 200.0)]),
+    "17x6x11": ((17, 6, 11), [(1e-30, 37.0), (1e-9, 300.0), (1e-6, 300.0)]),
+}
+
+
+@pytest.mark.parametrize("name", list(SEQUENCES))
+def test_lexicographic_batches_match_oracle(name, hip_lib):
+    shape, solves = SEQUENCES[name]
+    g = stretched_grid(*shape)
+    o = Oracle(g)
+    rng = np.random.default_rng(sum(shape))
+    s = ProjectionSolver(g)
+    try:
+        us, vs, ws = (rand_halo(g, rng) for _ in range(3))
+        p0 = rand_halo(g, rng)
+        for zeta, itmax in solves:
+            p, pr = p0.copy(order="F"), p0.copy(order="F")
+            pre, prer = g.interior(), g.interior()
+            ik, pm, _ = s.gauss_seidel(us, vs, ws, p, pre, 1e-3, 1.5, zeta, itmax,
+                                       SOR_LEXICOGRAPHIC)
+            ikr, pmr = o.gauss_seidel(us, vs, ws, pr, prer, 1e-3, 1.5, zeta, itmax)
+            print(f"{name} zeta {zeta:g} itmax {itmax:g}: ik {ik} (oracle {ikr})", flush=True)
+            assert ik == ikr
+            assert pm == pmr
+            assert same(p, pr)
+            assert same(pre, prer)
+    finally:
+        s.close()
+
+
+def resident(g, prm, sor, state):
+    s = ProjectionSolver(g, bc_codes())
+    s.configure(dt=prm["dt"], nu=1.0 / prm["Re"], Cs=prm["Cs"], LES=prm["LES"],
+                scheme=UPWIND if prm["scheme"] == 1 else QUICK_CENTRE, omega=prm["omega"],
+                zeta=prm["zeta"], itmax=prm["itmax"], sor=sor, lx=1.0, force_every=1)
+    s.upload(*state)
+    return s
+
+
+def rest_state(g, s):
+    u, v, w, p, eta = (g.halo() for _ in range(5))
+    s.boundary_conditions(u, v, w, p)
+    return [u, v, w, p, eta]
+
+
+def test_c1_cavity_40_100_steps_matches_oracle(hip_lib):
+    """The reference's cavity as it ships (variables_module.f90:9-23,
+    main.f90:44-70): every step's sweep count, maximum change and force
+    coefficients, and the state after steps 1, 10 and 100, bit for bit."""
+    fx = json.loads((GOLDEN / "c1_cavity_40_100steps.json").read_text())
+    prm = fx["params"]
+    g = Grid(prm["n"], prm["n"], prm["n"])
+    probe = ProjectionSolver(g, bc_codes())
+    state = rest_state(g, probe)
+    probe.close()
+    s = resident(g, prm, SOR_LEXICOGRAPHIC, state)
+    try:
+        iks, pms, cds, cls = [], [], [], []
+        done = 0
+        for upto in (1, 10, prm["steps"]):
+            ik, pm, cd, cl, _ = s.step(upto - done)
+            iks += list(ik); pms += list(pm); cds += list(cd); cls += list(cl)
+            done = upto
+            out = {n: g.halo() for n in ("u", "v", "w", "p")}
+            if upto < prm["steps"]:
+                s.download(**out)
+                for n, a in out.items():
+                    assert digest(a) == fx["hash_after"][str(upto)][n], (upto, n)
+        assert iks == fx["ik"]
+        assert [float(x).hex() for x in pms] == fx["pmax"]
+        assert [float(x).hex() for x in cds] == fx["cd"]
+        assert [float(x).hex() for x in cls] == fx["cl"]
+        out = {n: g.halo() for n in ("u", "v", "w", "p")}
+        out.update({n: g.interior() for n in ("pre", "uc", "vc", "wc")})
+        s.download(**out)
+        for n, a in out.items():
+            assert digest(a) == fx["hash_final"][n], n
+        print(f"C1 40^3, 100 steps: sweeps {iks[0]} .. {iks[-1]}, all bits equal", flush=True)
+    finally:
+        s.close()
+
+
+def les_state(g, prm, s):
+    u, v, w = noise_ic(g, prm["noise"], prm["seed"])
+    p, eta = g.halo(), g.halo()
+    s.boundary_conditions(u, v, w, p)
+    return [u, v, w, p, eta]
+
+
+def test_c2_les_cavity_128_first_step_matches_oracle(hip_lib):
+    """The first step of the 128^3 LES cavity from the seed-2024 noise state
+    (3000 sweeps, itmax-capped), bit for bit against the oracle's."""
+    fx = json.loads((GOLDEN / "c2_les_cavity_128_step1.json").read_text())
+    prm = fx["params"]
+    g = Grid(prm["n"], prm["n"], prm["n"])
+    probe = ProjectionSolver(g, bc_codes())
+    state = les_state(g, prm, probe)
+    probe.close()
+    s = resident(g, prm, SOR_LEXICOGRAPHIC, state)
+    try:
+        ik, pm, cd, cl, _ = s.step(1)
+        out = {n: g.halo() for n in ("u", "v", "w", "p")}
+        out["pre"] = g.interior()
+        s.download(**out)
+    finally:
+        s.close()
+    planes = np.load(GOLDEN / "c2_les_cavity_128_step1_planes.npz")
+    k = prm["n"] // 2 + 2
+    assert same(out["p"][:, :, k], planes["p_k64"]), "p differs on the k = 64 plane"
+    assert same(out["w"][:, :, k], planes["w_k64"]), "w differs on the k = 64 plane"
+    assert int(ik[0]) == fx["ik"]
+    assert float(pm[0]).hex() == fx["pmax"]
+    assert float(cd[0]).hex() == fx["cd"] and float(cl[0]).hex() == fx["cl"]
+    for n, a in out.items():
+        assert digest(a) == fx["hash"][n], n
+
+
+def test_c2_les_cavity_128_fast_solvers_vs_reference_order(hip_lib):
+    """Red-black FP64 and FP32 sweeps + FP64 refine against the lexicographic
+    solver (the reference's order, bit-identical to the oracle) after three
+    128^3 LES steps from the same state: velocity within 1e-6 relative."""
+    fx = json.loads((GOLDEN / "c2_les_cavity_128_step1.json").read_text())
+    prm = fx["params"]
+    g = Grid(prm["n"], prm["n"], prm["n"])
+    probe = ProjectionSolver(g, bc_codes())
+    state = les_state(g, prm, probe)
+    probe.close()
+    K = 3
+    res = {}
+    for name, sor in (("lex", SOR_LEXICOGRAPHIC), ("rb", SOR_REDBLACK), ("mixed", SOR_MIXED)):
+        s = resident(g, prm, sor, state)
+        try:
+            ik, pm, cd, cl, _ = s.step(K)
+            out = {n: g.halo() for n in ("u", "v", "w", "p")}
+            s.download(**out)
+        finally:
+            s.close()
+        res[name] = (ik, cd, cl, out)
+    ref = res["lex"][3]
+    vel = max(np.abs(ref[n]).max() for n in ("u", "v", "w"))
+    pmag = np.abs(ref["p"]).max()
+    for name in ("rb", "mixed"):
+        ik, cd, cl, out = res[name]
+        ev = max(np.abs(out[n] - ref[n]).max() for n in ("u", "v", "w")) / vel
+        ep = np.abs(out["p"] - ref["p"]).max() / pmag
+        ecd = np.max(np.abs(cd - res["lex"][1]))
+        ecl = np.max(np.abs(cl - res["lex"][2]))
+        print(f"{name} vs lexicographic after {K} steps: sweeps {list(ik)} "
+              f"(lex {list(res['lex'][0])}), velocity {ev:.3e}, p {ep:.3e}, "
+              f"C_D {ecd:.3e}, C_L {ecl:.3e}", flush=True)
+        # the orderings converge at different rates: a red-black solve may
+        # stop before itmax where the lexicographic one runs to it
+        assert ev <= 1e-6
