@@ -40,9 +40,12 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "Mcell-steps/s, FP32+refine vs FP64 plain @≤1e-6 rel err; adjoint/fwd ratio"
 
 
+ITMAX = 3000.0
+
+
 def workload(n):
-    return dict(nx=n, ny=n, nz=n, Re=1000.0, dt=5e-4, Cs=0.1, LES=1, scheme=0, omega=1.5,
-                zeta=1e-10, itmax=3000.0, noise=1e-3, seed=2024)
+    from ntustcfdlab_amd.configs import les_cavity
+    return les_cavity(n, ITMAX)
 
 
 def initial_state(grid, w, solver):
@@ -351,31 +354,15 @@ def airfoil_leg(hip, calls, cpu):
 
 
 def airfoil_case(hip):
-    """The large airfoil configuration: NACA 0012 (5000 vertices) on 512 x 384
-    x 128 cells (span x, normal y, chord z; y and z clustered at the airfoil
-    with Roberts' stretching, finest spacing 0.0086 / 0.026 there, domain 1 x
-    8 x 8), Re = 5000, QUICK, Smagorinsky LES Cs = 0.1, dt = 1e-4, SOR omega
-    1.5, zeta 1e-10, itmax 3000.  Uniform inflow w = 1 through the back face,
-    Neumann outflow at the front, Neumann tangential components on the other
-    faces (this configuration's own choice: the reference ships only the
-    cavity's boundary conditions).  Returns the grid, the boundary codes,
-    the polygon and a configure(s, sor) function."""
-    from ntustcfdlab_amd.airfoil import naca4_polygon
-    from ntustcfdlab_amd.grid import Grid, bc_codes, tanh_nodes
-    nx, ny, nz = 512, 384, 128
-    g = Grid(nx, ny, nz, lx=1.0, ly=8.0, lz=8.0, Y=tanh_nodes(ny, 8.0, 4.0, 5.0),
-             Z=tanh_nodes(nz, 8.0, 4.0, 5.0))
-    bc = bc_codes(BackWall_w="Dirichlet", FrontWall_u="Neumann", FrontWall_v="Neumann",
-                  FrontWall_w="Neumann", NorthWall_u="Neumann", NorthWall_w="Neumann",
-                  SouthWall_u="Neumann", SouthWall_w="Neumann", WestWall_v="Neumann",
-                  WestWall_w="Neumann", EastWall_v="Neumann", EastWall_w="Neumann")
-    az0, ay0 = naca4_polygon(poly=5000)
+    """The large airfoil configuration (ntustcfdlab_amd.configs.airfoil_c5):
+    the grid, the boundary codes, the polygon and a configure(s, sor)
+    function."""
+    from ntustcfdlab_amd import configs
+    g, bc, az0, ay0, _, _, _, prm = configs.airfoil_c5()
 
     def configure(s, sor, force_every=1):
-        s.configure(dt=1e-4, nu=1.0 / 5000.0, Cs=0.1, LES=1, scheme=hip.QUICK_CENTRE, omega=1.5,
-                    zeta=1e-10, itmax=3000.0, sor=sor, lx=1.0, force_every=force_every)
+        configs.configure(s, prm, sor, force_every)
     return g, bc, az0, ay0, configure
-
 
 def airfoil_les_leg(hip, hiptime, steps, warmup):
     """The large airfoil configuration (airfoil_case) at 8 degrees, forward:
@@ -488,31 +475,8 @@ def airfoil_c3_leg(hip, hiptime, steps, warmup):
 
 
 def actuator_strip(g, aoa, seed):
-    """The actuator control of the large configuration: a seeded strip 0.05
-    chord wide at x/c = 0.1, next to the surface (normal offset within 0.1
-    chord of the chord line), uniform along the span like the extruded
-    airfoil, pushing along the chord.  Chord frame of the cast airfoil
-    (quarter chord at (z, y) = (4, 4), rotated by -AOA): s = (z-4) cos a -
-    (y-4) sin a + 0.25 from the leading edge.  Returns (fu, fv, fw) halo
-    arrays with seeded U(0, 1) amplitudes.  Uniform along x keeps the flow
-    x-invariant, so u stays exactly 0: an x-dependent field would make it
-    non-zero with either sign, and the QUICK upwind choice at a zero face
-    velocity (discretisation_QUICK_centre.f90:18-102, 0 takes the else
-    branch) puts a kink in the map there that a central difference
-    straddles."""
-    _, ys, zs = g.centres()
-    a = np.deg2rad(aoa)
-    Z, Y = np.meshgrid(zs, ys, indexing="xy")          # (ny, nz)
-    sc = (Z - 4.0) * np.cos(a) - (Y - 4.0) * np.sin(a) + 0.25
-    nr = (Z - 4.0) * np.sin(a) + (Y - 4.0) * np.cos(a)
-    mask = (sc >= 0.075) & (sc <= 0.125) & (np.abs(nr) <= 0.1)
-    amp = np.random.default_rng(seed).uniform(0.0, 1.0, (g.ny, g.nz)) * mask
-    amp = np.broadcast_to(amp[None, :, :], (g.nx, g.ny, g.nz))
-    fu, fv, fw = g.halo(), g.halo(), g.halo()
-    fv[2:-2, 2:-2, 2:-2] = -np.sin(a) * amp
-    fw[2:-2, 2:-2, 2:-2] = np.cos(a) * amp
-    return fu, fv, fw, int(mask.sum()) * g.nx
-
+    from ntustcfdlab_amd.configs import actuator_strip as strip
+    return strip(g, aoa, seed)
 
 def airfoil_adjoint_leg(hip, hiptime, steps, every, navg, fd=True):
     """The adjoint of the large configuration (airfoil_case, FP32 sweeps +
@@ -723,8 +687,13 @@ def timed_run(s, hip, hiptime, state, sor, steps, warmup):
     return ms, ik, prof, final, wall
 
 
+SOLVER_NAMES = ("fp64", "mixed", "lex")
+
+
 def measure(hip, hiptime, size, steps, warmup, sweep, only, calibrate):
-    """Both solvers on the LES cavity of `size`^3 from the same state."""
+    """The three pressure solvers on the LES cavity of `size`^3 from the same
+    state: red-black FP64 ("fp64"), FP32 sweeps + FP64 refine ("mixed") and
+    GS.f90's lexicographic order ("lex", the oracle's bits)."""
     from ntustcfdlab_amd.grid import Grid, bc_codes
     w = workload(size)
     g = Grid(w["nx"], w["ny"], w["nz"])
@@ -732,7 +701,7 @@ def measure(hip, hiptime, size, steps, warmup, sweep, only, calibrate):
     s.set_sweep_variant({"fused": hip.SWEEP_FUSED, "two-pass": hip.SWEEP_TWO_PASS,
                          "graph": hip.SWEEP_GRAPH, "tile": hip.SWEEP_TILE}[sweep])
     if calibrate:
-        hiptime.copy_calibration()
+        s.prof_copy_calibration()
 
     def configure_sor(sor):
         s.configure(dt=w["dt"], nu=1.0 / w["Re"], Cs=w["Cs"], LES=w["LES"], scheme=w["scheme"],
@@ -742,22 +711,23 @@ def measure(hip, hiptime, size, steps, warmup, sweep, only, calibrate):
     configure_sor(hip.SOR_REDBLACK)
     state = initial_state(g, w, s)
     runs = {}
-    for name, sor in (("fp64", hip.SOR_REDBLACK), ("mixed", hip.SOR_MIXED)):
-        if only and only != name:
+    sors = {"fp64": hip.SOR_REDBLACK, "mixed": hip.SOR_MIXED, "lex": hip.SOR_LEXICOGRAPHIC}
+    for name in SOLVER_NAMES:
+        if only and name not in only:
             continue
         print(f"# {size}^3 {name}: warmup {warmup}, timing {steps} steps", file=sys.stderr,
               flush=True)
-        runs[name] = timed_run(s, hip, hiptime, state, sor, steps, warmup)
+        runs[name] = timed_run(s, hip, hiptime, state, sors[name], steps, warmup)
     s.close()
 
     cells = g.cells
     # HBM bytes per launch of the sweep kernels, from the counter profile of
-    # this workload (FETCH_SIZE/WRITE_SIZE scaled by a plain copy)
+    # this workload, when it has been validated against a copy of known size
     traffic = {}
     tfile = ROOT / "profiles" / f"pmc_traffic_{size}.json"
     if tfile.exists():
         t = json.loads(tfile.read_text())
-        if t.get("cells") == cells:
+        if t.get("cells") == cells and t.get("validated"):
             traffic = {k: v["read_bytes"] + v["write_bytes"] for k, v in t["per_launch"].items()}
 
     solvers, roofs = {}, {}
@@ -769,32 +739,51 @@ def measure(hip, hiptime, size, steps, warmup, sweep, only, calibrate):
             achieved = per_cell * cells / (avg_ms / 1e3) / 1e9
             kname, kdesc = hip.PROF_KERNELS[kid]
             tr = traffic.get(kname)
+            # a counter figure that would move faster than HBM's peak does not
+            # count HBM bytes (cache-resident working sets): reported as null
+            if tr and tr / (avg_ms / 1e3) / 1e9 > HBM_PEAK_GBS:
+                tr = None
             roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": round(tr) if tr else None,
                     "kernel": f"{kname} ({kdesc})",
-                    "bytes_per_launch": per_cell * cells,
+                    "bytes_per_launch": round(per_cell * cells, 1),
                     "avg_launch_us": round(avg_ms * 1e3, 3), "launches_timed": launches}
         roofs[name] = roof
         solvers[name] = {"value": round(cells * steps / (ms / 1e3) / 1e6, 3),
                          "ms_per_step": round(ms / steps, 3),
                          "sweeps_per_step": round(float(np.mean(ik)), 1),
+                         "sweeps": [int(x) for x in ik],
                          "roofline_frac": roof and roof["frac"]}
         wall = wl
-    err = None
-    if "fp64" in runs and "mixed" in runs:
-        ref = runs["fp64"][3]
+
+    def rel_err(a, b):
+        ref = runs[b][3]
         vel = max(np.abs(x).max() for x in ref[:3])
-        err = max(np.abs(x - y).max() for x, y in zip(runs["mixed"][3][:3], ref[:3])) / vel
+        pm = np.abs(ref[3]).max()
+        return (max(np.abs(x - y).max() for x, y in zip(runs[a][3][:3], ref[:3])) / vel,
+                np.abs(runs[a][3][3] - ref[3]).max() / pm)
+
+    errs = {}
+    for a_, b_ in (("mixed", "lex"), ("fp64", "lex"), ("mixed", "fp64")):
+        if a_ in runs and b_ in runs:
+            ev, ep = rel_err(a_, b_)
+            errs[f"{a_}_vs_{'oracle' if b_ == 'lex' else b_}_rel_velocity_err"] = ev
+            errs[f"{a_}_vs_{'oracle' if b_ == 'lex' else b_}_rel_pressure_err"] = ep
     # the headline is the FP32-sweep + FP64-refine solver when it meets the
-    # 1e-6 relative velocity bar against the FP64 solver, else the FP64 one
-    head = "mixed" if ("mixed" in runs and (err is None or err <= 1e-6)) else "fp64"
+    # 1e-6 relative velocity bar against GS.f90's order (the oracle's bits),
+    # else the FP64 red-black solver
+    e = errs.get("mixed_vs_oracle_rel_velocity_err")
+    head = "mixed" if ("mixed" in runs and e is not None and e <= 1e-6) else "fp64"
+
+    def speed(a_, b_):
+        return (round(solvers[a_]["value"] / solvers[b_]["value"], 3)
+                if a_ in solvers and b_ in solvers else None)
     return {"grid": g, "workload": w, "state": state,
             "summary": {"head": head, "solvers": solvers, "roofline": roofs,
-                        "wall_s": round(wall, 3), "mixed_vs_fp64_rel_velocity_err": err,
-                        "mixed_speedup_vs_fp64": (round(solvers["mixed"]["value"] /
-                                                        solvers["fp64"]["value"], 3)
-                                                  if len(solvers) == 2 else None)}}
+                        "wall_s": round(wall, 3), **errs,
+                        "mixed_speedup_vs_fp64": speed("mixed", "fp64"),
+                        "mixed_speedup_vs_reference_order": speed("mixed", "lex")}}
 
 
 def main():
@@ -802,17 +791,24 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--size", type=int, default=128)
-    ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
-                    help="time one solver only")
+    ap.add_argument("--size", type=int, default=256,
+                    help="grid of the headline LES cavity (the 128^3 cavity is a leg)")
+    ap.add_argument("--only", default=None,
+                    help="comma-separated solvers to time (fp64, mixed, lex); also skips "
+                         "the other legs")
+    ap.add_argument("--itmax", type=float, default=None,
+                    help="override the SOR sweep cap of the cavity legs (counter profiles "
+                         "of the same kernels with fewer launches; not a benchmark setting)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate", action="store_true",
-                    help="one 256 MiB device copy before timing (counter-profile scaling)")
+                    help="plain device copies of known size before timing (64 MiB and "
+                         "1 GiB each way), for the counter profiles")
     ap.add_argument("--adjoint-size", type=int, default=96)
     ap.add_argument("--adjoint-steps", type=int, default=128)
     ap.add_argument("--fd-directions", type=int, default=2)
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--no-256", action="store_true", help="skip the 256^3 leg")
+    ap.add_argument("--no-128", action="store_true", help="skip the 128^3 leg")
+    ap.add_argument("--no-256", action="store_true", help="skip the 256^3 adjoint leg")
     ap.add_argument("--no-airfoil", action="store_true",
                     help="skip the airfoil ray-casting leg")
     ap.add_argument("--only-airfoil", action="store_true",
@@ -848,6 +844,9 @@ def main():
         sys.exit(2)
 
     from ntustcfdlab_amd import hip, hiptime
+    if a.itmax is not None:
+        global ITMAX
+        ITMAX = a.itmax
 
     if a.only_krylov:
         print(json.dumps({"krylov_bicgstab": krylov_leg(hip, 128)}))
@@ -876,7 +875,8 @@ def main():
                                               fd=not a.no_airfoil_adjoint_fd)}))
         return
 
-    head_run = measure(hip, hiptime, a.size, a.steps, a.warmup, a.sweep, a.only, a.calibrate)
+    only = tuple(a.only.split(",")) if a.only else None
+    head_run = measure(hip, hiptime, a.size, a.steps, a.warmup, a.sweep, only, a.calibrate)
     g, w, state, sweeps = head_run["grid"], head_run["workload"], head_run["state"], None
     res = head_run["summary"]
     head = res["head"]
@@ -889,24 +889,28 @@ def main():
         "dtype": "f64" if head == "fp64" else "f32 sweeps + f64 refine",
         "data": "synthetic",
         "config": {"workload": (f"LES cavity Re=1000 {a.size}^3 uniform, QUICK, Smagorinsky "
-                                f"Cs=0.1, dt=5e-4, SOR omega=1.5 zeta=1e-10 itmax=3000, "
+                                f"Cs=0.1, dt=5e-4, SOR omega=1.5 zeta=1e-10 itmax={w['itmax']:g}, "
                                 f"red-black, {'FP64' if head == 'fp64' else 'FP32 sweeps in FP64 defect correction'}"),
                    "cells": g.cells, "solver": head, "sweeps_per_step": sweeps,
                    "wall_s": res["wall_s"]},
         "roofline": res["roofline"][head],
         "solvers": res["solvers"],
-        "mixed_vs_fp64_rel_velocity_err": res["mixed_vs_fp64_rel_velocity_err"],
+        "solver_roofline": {k: v for k, v in res["roofline"].items() if k != head},
+        **{k: v for k, v in res.items() if k.endswith("_err")},
         "mixed_speedup_vs_fp64": res["mixed_speedup_vs_fp64"],
+        "mixed_speedup_vs_reference_order": res["mixed_speedup_vs_reference_order"],
     }
-    if a.size != 256 and not a.no_256 and not a.only:
-        print("# 256^3 LES cavity", file=sys.stderr, flush=True)
-        big = measure(hip, hiptime, 256, min(a.steps, 3), 1, a.sweep, None, False)["summary"]
-        out["les_cavity_256"] = {
-            "workload": "the same LES cavity on uniform 256^3 (16.8 M cells)",
-            "steps": min(a.steps, 3), "warmup": 1, "solvers": big["solvers"],
-            "roofline": big["roofline"],
-            "mixed_speedup_vs_fp64": big["mixed_speedup_vs_fp64"],
-            "mixed_vs_fp64_rel_velocity_err": big["mixed_vs_fp64_rel_velocity_err"]}
+    if a.size != 128 and not a.no_128 and not a.only:
+        print("# 128^3 LES cavity", file=sys.stderr, flush=True)
+        small = measure(hip, hiptime, 128, a.steps, min(a.warmup, 2), a.sweep, None,
+                        False)["summary"]
+        out["les_cavity_128"] = {
+            "workload": "the same LES cavity on uniform 128^3 (2.1 M cells)",
+            "steps": a.steps, "warmup": min(a.warmup, 2), "solvers": small["solvers"],
+            "roofline": small["roofline"],
+            **{k: v for k, v in small.items() if k.endswith("_err")},
+            "mixed_speedup_vs_fp64": small["mixed_speedup_vs_fp64"],
+            "mixed_speedup_vs_reference_order": small["mixed_speedup_vs_reference_order"]}
     if not a.no_airfoil and not a.only:
         print("# airfoil geometry", file=sys.stderr, flush=True)
         out["airfoil_ray_casting"] = airfoil_leg(hip, 10, not a.no_cpu_baseline)
@@ -915,14 +919,14 @@ def main():
         if not a.no_adjoint:
             print("# airfoil adjoint", file=sys.stderr, flush=True)
             out["airfoil_adjoint_512x384x128"] = airfoil_adjoint_leg(hip, hiptime, *aa_args)
-    if not a.no_adjoint:
+    if not a.no_adjoint and not a.only:
         print("# adjoint", file=sys.stderr, flush=True)
         out["adjoint"] = adjoint_leg(hip, hiptime, a.adjoint_size, a.adjoint_steps,
                                      a.fd_directions)
-        if not a.no_256 and not a.only:
+        if not a.no_256:
             print("# adjoint 256^3", file=sys.stderr, flush=True)
             out["adjoint_les_cavity_256"] = adjoint_cavity_leg(hip, hiptime, 256, 4, 2, 2, 1)
-    if not a.no_cpu_baseline:
+    if not a.no_cpu_baseline and not a.only:
         print("# cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(g, w, state, sweeps)
     print(json.dumps(out))

@@ -44,7 +44,7 @@ enum {
 
 /* pressure-solver orderings (ntust_configure sor_mode) */
 enum {
-    NTUST_SOR_LEXICOGRAPHIC = 0,  /* GS.f90 order, bit-identical, one workgroup */
+    NTUST_SOR_LEXICOGRAPHIC = 0,  /* GS.f90 order, bit-identical, pipelined over sweeps */
     NTUST_SOR_REDBLACK = 1,       /* red-black, FP64 */
     NTUST_SOR_MIXED = 2           /* red-black, FP32 sweeps inside FP64 defect correction */
 };
@@ -396,11 +396,21 @@ int ntust_prof_reset(void);
 int ntust_prof_read(double *ms, long long *launches);
 
 /* The timed kernel: returns 0 two-pass FP64, 1 fused FP64, 2 two-pass FP32,
- * 3 fused FP32 sweep, 4 tile FP64 sweep, 5 tile FP32 sweep; *bytes_per_cell
- * gets its algorithmic bytes per grid cell and launch, averaged over a
- * sweep (FP64: read p and b, write p; FP32: read and write e, read r on the
- * colour-0 cells, the colour-1 sweeps being shifted). */
+ * 3 fused FP32 sweep, 4 tile FP64 sweep, 5 tile FP32 sweep, 6 the
+ * lexicographic step; *bytes_per_cell gets its algorithmic bytes per grid
+ * cell and launch, averaged over a sweep (FP64: read p and b, write p;
+ * FP32: read and write e, read r on the colour-0 cells, the colour-1 sweeps
+ * being shifted; lexicographic: 24 B per cell update, over the timed
+ * launches). */
 int ntust_prof_kernel(double *bytes_per_cell);
+
+/* Measurement aid (no counterpart in the reference): copies nbytes (rounded
+ * down to 16-byte words, 16 B .. 64 GB) between two fresh device buffers
+ * with a plain streaming kernel, k_stream_copy, each byte read once and
+ * written once, and returns its time (*ms may be NULL).  A counter profile
+ * of the same process then holds a copy of known traffic to check the
+ * memory counters against.  Allocates and frees the two buffers. */
+int ntust_prof_copy(double nbytes, double *ms);
 
 /* Selects the red-black sweep kernel (NTUST_SWEEP_*); not a reference
  * setting, the iterates are bit-identical either way. */

@@ -132,4 +132,29 @@ void launch_unpack_q(const KGrid &g, const float *q, double *u, double *v, doubl
     hipLaunchKernelGGL(k_centre, interior_grid(g), dim3(64, 4), 0, st, g, u, v, w, uc, vc, wc);
 }
 
+// A plain streaming copy of n 16-byte words (each read once and written
+// once): the known traffic the memory counters of a profile are checked
+// against.  Grid-stride, 4 words in flight per thread.
+__global__ void __launch_bounds__(256)
+k_stream_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, long n)
+{
+    const long stride = (long)gridDim.x * blockDim.x;
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; q + 3 * stride < n; q += 4 * stride) {
+        const double2 a = src[q], b = src[q + stride], c = src[q + 2 * stride],
+                      d = src[q + 3 * stride];
+        dst[q] = a;
+        dst[q + stride] = b;
+        dst[q + 2 * stride] = c;
+        dst[q + 3 * stride] = d;
+    }
+    for (; q < n; q += stride) dst[q] = src[q];
+}
+
+void launch_stream_copy(const void *src, void *dst, long words16, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_stream_copy, dim3(2048), dim3(256), 0, st, (const double2 *)src,
+                       (double2 *)dst, words16);
+}
+
 }  // namespace ntust

@@ -280,7 +280,8 @@ long spec_tail(long tmax)
 // go back to the batch's start -- gathered again from p, which the solve
 // leaves alone until the end, or the copy taken before the batch -- and the
 // batch runs again, exactly to that sweep.  First batch: 7/8 of the
-// previous solve's count, or all of itmax when that solve ran to itmax.
+// previous solve's count, or all of itmax when that solve ran to itmax;
+// then a quarter of that count (at least 32 sweeps), doubling each time.
 int lex_stop(const int *flags, const double *slots, long t0, long t1, int epoch, double zeta,
              long *stop)
 {
@@ -330,7 +331,7 @@ int solve_lexicographic(const double *us, const double *vs, const double *ws, do
     launch_rb_rhs(g, C.sg, us, vs, ws, dt, B0, B1, C.st);
     const long hint = C.lex_hint;
     long nb = hint >= tmax ? tmax : (hint > 0 ? hint - hint / 8 : 32);
-    long t0 = 1, ik = 0;
+    long t0 = 1, ik = 0, grow = 0;
     double *K0 = nullptr, *K1 = nullptr;
     for (;;) {
         const long t1 = t0 + (nb > 1 ? nb : 1) - 1 < tmax ? t0 + (nb > 1 ? nb : 1) - 1 : tmax;
@@ -379,8 +380,11 @@ int solve_lexicographic(const double *us, const double *vs, const double *ws, do
             ik = t1;
             break;
         }
+        // later batches: a quarter of the previous count, doubling, so a
+        // solve with no useful count pays O(log itmax) pipeline fills
         t0 = t1 + 1;
-        nb = hint / 4 > 32 ? hint / 4 : 32;
+        grow = grow ? 2 * grow : (hint / 4 > 32 ? hint / 4 : 32);
+        nb = grow;
     }
     C.lex_hint = (int)ik;
     double bk[SLOT_BUCKETS * SLOT_STRIDE];
@@ -2076,6 +2080,37 @@ int ntust_prof_read(double *ms, long long *launches)
 {
     if (ms) *ms = C.prof_ms;
     if (launches) *launches = C.prof_launches;
+    return 0;
+}
+
+int ntust_prof_copy(double nbytes, double *ms)
+{
+    NEED_INIT();
+    if (!(nbytes >= 16.0) || nbytes > 64.0e9)
+        return fail(NTUST_E_ARG, "copy size %g bytes out of range", nbytes);
+    const long words = (long)(nbytes / 16.0);
+    double *a = nullptr, *b = nullptr;
+    HIPCHK(hipMalloc(&a, words * 16));
+    if (hipMalloc(&b, words * 16) != hipSuccess) {
+        hipFree(a);
+        return fail(NTUST_E_ALLOC, "device allocation of the copy buffers failed");
+    }
+    int rc = 0;
+    float t = 0.f;
+    // the source filled by a kernel of its own first, so the timed copy
+    // reads memory that is not in flight
+    if (hipMemsetAsync(a, 0, words * 16, C.st) != hipSuccess ||
+        hipEventRecord(C.ev0, C.st) != hipSuccess) rc = -1;
+    if (rc == 0) launch_stream_copy(a, b, words, C.st);
+    if (rc == 0 && (hipEventRecord(C.ev1, C.st) != hipSuccess ||
+                    hipStreamSynchronize(C.st) != hipSuccess ||
+                    hipEventElapsedTime(&t, C.ev0, C.ev1) != hipSuccess ||
+                    hipGetLastError() != hipSuccess))
+        rc = -1;
+    hipFree(a);
+    hipFree(b);
+    if (rc) return fail(NTUST_E_HIP, "calibration copy failed");
+    if (ms) *ms = t;
     return 0;
 }
 

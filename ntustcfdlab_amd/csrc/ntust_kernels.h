@@ -371,6 +371,8 @@ void launch_pack_q(const KGrid &g, const double *pre, const double *uc, const do
                    const double *wc, const double *eta, float *q, hipStream_t st);
 void launch_steady(const KGrid &g, const double *wc, double *last, int compare,
                    unsigned long long *vmax, hipStream_t st);
+// plain copy of words16 16-byte words (counter calibration)
+void launch_stream_copy(const void *src, void *dst, long words16, hipStream_t st);
 void launch_unpack_q(const KGrid &g, const float *q, double *u, double *v, double *w, double *p,
                      double *pre, double *uc, double *vc, double *wc, hipStream_t st);
 

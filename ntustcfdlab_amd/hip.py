@@ -74,6 +74,7 @@ def load_library() -> ct.CDLL:
         "ntust_step": [i, _I, _D, _D, _D],
         "ntust_prof_reset": [],
         "ntust_prof_read": [_D, ct.POINTER(ct.c_longlong)],
+        "ntust_prof_copy": [ct.c_double, _D],
         "ntust_set_body_force": [_D, _D, _D],
         "ntust_kinetic_energy": [_D],
         "ntust_checkpoint_reserve": [i],
@@ -288,6 +289,16 @@ class ProjectionSolver:
         b = ct.c_double()
         k = self.lib.ntust_prof_kernel(ct.byref(b))
         return k, b.value
+
+    def prof_copy_calibration(self, sizes=(64 << 20, 1 << 30)):
+        """Plain device copies of the given sizes (ntust_prof_copy), for the
+        counter profiles.  Returns [(bytes, ms)]."""
+        out = []
+        for n in sizes:
+            ms = ct.c_double()
+            _check(self.lib.ntust_prof_copy(float(n), ct.byref(ms)))
+            out.append((n, ms.value))
+        return out
 
     def prof_read(self):
         ms = ct.c_double()
