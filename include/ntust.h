@@ -143,6 +143,21 @@ int ntust_updating_velocity(const double *u2, const double *v2, const double *w2
 int ntust_virtual_force_integrator(const double *FY, const double *FZ, double lx,
                                    double *cDrag, double *cLift);
 
+/* No counterpart in the reference.  ntust_virtual_force_integrator for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc): it calls no
+ * hipMalloc, hipFree or allocating library routine.  FY and FZ are DEVICE
+ * halo arrays of the grid of ntust_init (unlike every other entry point),
+ * coef a DEVICE array of 2 doubles that gets cDrag, cLift.  The line sums go
+ * to a static device scratch array of 2^20 doubles (8 MiB, held by each GPU
+ * the library runs on until the process ends): grids with
+ * 2(nx+2)(ny+3) > 2^20 are refused (NTUST_E_ARG).  Calls on one GPU share
+ * that scratch, so calls that overlap on one GPU race; each call clears the
+ * entries it needs zero.  Stream ordered on the library stream and
+ * synchronous at return, like the other entry points. */
+int ntust_virtual_force_integrator_noalloc(const double *FY, const double *FZ, double lx,
+                                           double *coef);
+
 /* boundary_conditions.f90:1-376 with the codes of ntust_set_bc: velocity
  * faces, whole-array copies into u1/u2/u_star (v, w alike; any target may
  * be NULL) and the Neumann pressure ghosts. */

@@ -209,6 +209,35 @@ void launch_copy_interior(const KGrid &g, const double *a, const double *b, cons
     hipLaunchKernelGGL(k_copy_interior, grd, blk, 0, st, g, a, b, c, x, y, z, k0, k1);
 }
 
+// The entries of the line sums that k_force_y and k_force_x read but no
+// kernel writes -- rows j = 0 and ny+1 of FXz, FYz and entries 0 and nx+1 of
+// FXy, FYy, the reference's out-of-bounds reads (virtualForceIntegrator.f90:
+// 49,53,68,72) -- set to 0 for scratch memory that a previous call, possibly
+// of another grid, has left holding other values.
+__global__ void __launch_bounds__(64)
+k_force_clear(int nx, int ny, double *__restrict__ FXz, double *__restrict__ FYz,
+              double *__restrict__ FXy, double *__restrict__ FYy)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > nx + 1) return;
+    const long top = (long)(nx + 2) * (ny + 1);
+    FXz[i] = 0.0;
+    FYz[i] = 0.0;
+    FXz[i + top] = 0.0;
+    FYz[i + top] = 0.0;
+    if (i == 0 || i == nx + 1) {
+        FXy[i] = 0.0;
+        FYy[i] = 0.0;
+    }
+}
+
+void launch_force_clear(const KGrid &g, double *FXz, double *FYz, double *FXy, double *FYy,
+                        hipStream_t st)
+{
+    hipLaunchKernelGGL(k_force_clear, dim3((g.nx + 2 + 63) / 64), dim3(64), 0, st, g.nx, g.ny,
+                       FXz, FYz, FXy, FYy);
+}
+
 void launch_force_integral(const KGrid &g, const double *FY, const double *FZ, double lx,
                            double *FXz, double *FYz, double *FXy, double *FYy,
                            double *coef_out, hipStream_t st)

@@ -21,7 +21,19 @@
 
 using namespace ntust;
 
+// Scratch of the allocation-free force integral
+// (ntust_virtual_force_integrator_noalloc): the line sums FXz, FYz
+// ((nx+2)(ny+2) doubles each) and FXy, FYy (nx+2 each).  Each GPU the library
+// runs on holds its own copy until the process ends, and every call on that
+// GPU shares it.  Host code uses the per-device address that
+// hipGetSymbolAddress returns at set-up (g_scratch), never this name.
+constexpr size_t NOALLOC_SCRATCH_DOUBLES = (size_t)1 << 20;   // 8 MiB
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
 namespace {
+
+constexpr int MAX_DEVICES = 64;
+double *g_scratch[MAX_DEVICES] = {};   // device address of scratch, per device
 
 char g_err[512] = "";
 
@@ -50,6 +62,7 @@ struct Params {
 
 struct Ctx {
     bool ready = false;
+    int dev = 0;                       // the device current at ntust_init
     int nx = 0, ny = 0, nz = 0;
     long nh = 0, ni = 0;
     hipStream_t st = nullptr;
@@ -1102,6 +1115,13 @@ static int init_device(int nx, int ny, int nz, const double *iDx, const double *
                        const double *Dzs)
 {
     C.nx = nx; C.ny = ny; C.nz = nz;
+    HIPCHK(hipGetDevice(&C.dev));
+    if (C.dev < 0 || C.dev >= MAX_DEVICES)
+        return fail(NTUST_E_ARG, "device %d: at most %d devices", C.dev, MAX_DEVICES);
+    // the allocation-free entry point's scratch on this device, looked up
+    // once per device with the device current
+    if (!g_scratch[C.dev])
+        HIPCHK(hipGetSymbolAddress((void **)&g_scratch[C.dev], HIP_SYMBOL(scratch)));
     C.nh = (long)(nx + 4) * (ny + 4) * (nz + 4);
     C.ni = (long)nx * ny * nz;
     HIPCHK(hipStreamCreateWithFlags(&C.st, hipStreamNonBlocking));
@@ -1433,6 +1453,27 @@ int ntust_virtual_force_integrator(const double *FY, const double *FZ, double lx
     *cDrag = h[0];
     *cLift = h[1];
     return 0;
+}
+
+int ntust_virtual_force_integrator_noalloc(const double *FY, const double *FZ, double lx,
+                                           double *coef)
+{
+    NEED_INIT();
+    if (!FY || !FZ || !coef) return fail(NTUST_E_ARG, "NULL argument");
+    const size_t a = (size_t)(C.nx + 2) * (C.ny + 2);
+    const size_t need = 2 * a + 2 * (size_t)(C.nx + 2);
+    if (need > NOALLOC_SCRATCH_DOUBLES)
+        return fail(NTUST_E_ARG,
+                    "grid %dx%d: the allocation-free force integral needs %zu scratch doubles "
+                    "(2(nx+2)(ny+3)), at most %zu",
+                    C.nx, C.ny, need, NOALLOC_SCRATCH_DOUBLES);
+    double *s = g_scratch[C.dev];
+    if (!s) return fail(NTUST_E_STATE, "scratch of device %d not set up", C.dev);
+    double *fxz = s, *fyz = s + a, *fxy = s + 2 * a, *fyy = fxy + (C.nx + 2);
+    launch_force_clear(C.g, fxz, fyz, fxy, fyy, C.st);
+    launch_force_integral(C.g, FY, FZ, lx, fxz, fyz, fxy, fyy, coef, C.st);
+    HIPCHK(hipGetLastError());
+    return sync();
 }
 
 int ntust_boundary_conditions(double *u, double *v, double *w, double *p, double *u1,

@@ -278,6 +278,10 @@ void launch_project(const KGrid &g, const double *us, const double *vs, const do
 void launch_copy_interior(const KGrid &g, const double *a, const double *b, const double *c,
                           double *x, double *y, double *z, int k0, int k1, hipStream_t st);
 void launch_pack_interior(const KGrid &g, const double *a, double *x, hipStream_t st);
+// zeroes the entries of the line sums that the force kernels read and do not
+// write (rows 0 and ny+1 of FXz, FYz; entries 0 and nx+1 of FXy, FYy)
+void launch_force_clear(const KGrid &g, double *FXz, double *FYz, double *FXy, double *FYy,
+                        hipStream_t st);
 void launch_force_integral(const KGrid &g, const double *FY, const double *FZ, double lx,
                            double *FXz, double *FYz, double *FXy, double *FYy,
                            double *coef_out, hipStream_t st);

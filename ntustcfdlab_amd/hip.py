@@ -67,6 +67,7 @@ def load_library() -> ct.CDLL:
         "ntust_calcul_new_velocity": [_D, _D, _D, _D, _D, d, d, d, d, i, i] + [_D] * 12,
         "ntust_updating_velocity": [_D, _D, _D, i, i, _D, _D, _D],
         "ntust_virtual_force_integrator": [_D, _D, d, _D, _D],
+        "ntust_virtual_force_integrator_noalloc": [vp, vp, d, vp],
         "ntust_boundary_conditions": [_D] * 13,
         "ntust_configure": [d, d, d, i, i, d, d, d, i, d, i],
         "ntust_state_upload": [_D] * 5,
@@ -222,6 +223,14 @@ class ProjectionSolver:
         _check(self.lib.ntust_virtual_force_integrator(_p(FY), _p(FZ), lx, ct.byref(cd),
                                                        ct.byref(cl)))
         return cd.value, cl.value
+
+    def virtualForceIntegrator_noalloc(self, FY_dev: int, FZ_dev: int, lx: float,
+                                       coef_dev: int):
+        """virtualForceIntegrator.f90:3-85 on device arrays, allocating nothing
+        (ntust_virtual_force_integrator_noalloc): FY_dev, FZ_dev the device
+        addresses of halo arrays, coef_dev of 2 doubles (cDrag, cLift)."""
+        _check(self.lib.ntust_virtual_force_integrator_noalloc(
+            ct.c_void_p(FY_dev), ct.c_void_p(FZ_dev), lx, ct.c_void_p(coef_dev)))
 
     def boundary_conditions(self, u, v, w, p=None, u1=None, v1=None, w1=None, u2=None,
                             v2=None, w2=None, u_star=None, v_star=None, w_star=None):
