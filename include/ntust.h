@@ -61,6 +61,16 @@ enum {
                                      recomputed in LDS, no plane loop) */
 };
 
+/* arithmetic of the resident step's QUICK predictor and eddy viscosity
+ * (ntust_set_arithmetic); the pressure solve, projection, forcing and force
+ * integral are unaffected */
+enum {
+    NTUST_ARITH_EXACT = 0,        /* the reference's operation order, bit-identical (default) */
+    NTUST_ARITH_FAST = 1,         /* FP64, reciprocal metric tables, fused multiply-adds */
+    NTUST_ARITH_FP32 = 2          /* the increment u* - u and nu_t in FP32 from FP64
+                                     differences; u* = u + increment in FP64 */
+};
+
 /* advection schemes (ntust_configure scheme) */
 enum {
     NTUST_QUICK_CENTRE = 0,       /* discretisation_QUICK_centre.f90 */
@@ -418,6 +428,15 @@ int ntust_prof_read(double *ms, long long *launches);
  * being shifted; lexicographic: 24 B per cell update, over the timed
  * launches). */
 int ntust_prof_kernel(double *bytes_per_cell);
+
+/* Arithmetic of the resident step's QUICK predictor and Smagorinsky
+ * viscosity (NTUST_ARITH_*; no counterpart in the reference, whose order
+ * NTUST_ARITH_EXACT keeps).  The fast modes evaluate the same
+ * discretisation (discretisation_QUICK_centre.f90:18-112,
+ * smagorinsky.f90:14-24,60-79) with precomputed reciprocal tables and
+ * fused multiply-adds; the upwind predictor and the drop-in entry points
+ * always run exact.  Kept until ntust_finalize. */
+int ntust_set_arithmetic(int mode);
 
 /* Measurement aid (no counterpart in the reference): copies nbytes (rounded
  * down to 16-byte words, 16 B .. 64 GB) between two fresh device buffers

@@ -73,6 +73,13 @@ struct Ctx {
     double *recip[3] = {};             // 1/Dxs, 1/Dys, 1/Dzs, base
     float *fmetric[6] = {};            // FP32 iDx, iDy, iDz, 1/Dxs, 1/Dys, 1/Dzs, base
     long mix_segment = 64;             // FP32 sweeps between FP64 corrections
+    // arithmetic of the resident step's predictor and eddy viscosity
+    // (NTUST_ARITH_*), with their tables (base pointers of device arrays)
+    int arith = NTUST_ARITH_EXACT;
+    QCoef<double> *qtab64[3][2] = {};
+    QCoef<float> *qtab32[3][2] = {};
+    double *rmet64[6] = {};            // 1/iDx 1/Dxs 1/iDy 1/Dys 1/iDz 1/Dzs
+    float *rmet32[6] = {};
     BcTable bc{};
     Params prm;
     // named device buffers
@@ -846,6 +853,27 @@ int step_buffers(StepBufs &b, bool force)
     return 0;
 }
 
+template <typename T>
+QTables<T> qtables()
+{
+    QTables<T> q;
+    for (int a = 0; a < 3; a++)
+        for (int k = 0; k < 2; k++) {
+            if constexpr (sizeof(T) == 8) q.t[a][k] = C.qtab64[a][k];
+            else q.t[a][k] = C.qtab32[a][k];
+        }
+    return q;
+}
+
+template <typename T>
+RMetric<T> rmetric()
+{
+    T *const *m;
+    if constexpr (sizeof(T) == 8) m = C.rmet64;
+    else m = C.rmet32;
+    return RMetric<T>{m[0] + 1, m[1] + 1, m[2] + 1, m[3] + 1, m[4] + 1, m[5] + 1};
+}
+
 // One step of main.f90:209-259 on the device state.  force: compute FY, FZ
 // and the force integral (synchronises for cd/cl); centre: write uc, vc, wc.
 int one_step(const StepBufs &b, bool force, bool centre, int *ik, double *pm, double *cd,
@@ -856,11 +884,27 @@ int one_step(const StepBufs &b, bool force, bool centre, int *ik, double *pm, do
     // main.f90:209 -- the reference evaluates nu_t every step and scales it
     // by LES; with LES = 0 the product is nu + 0 for any finite nu_t, so the
     // evaluation is skipped.
-    if (q.les)
-        launch_smagorinsky(g, b.u, b.v, b.w, nullptr, nullptr, nullptr, b.filt, b.vis, nullptr,
-                           nullptr, nullptr, 1, C.nz, C.st);
+    if (q.les) {
+        if (C.arith == NTUST_ARITH_FAST)
+            launch_smagorinsky_fast<double>(g, rmetric<double>(), b.u, b.v, b.w, b.filt, b.vis,
+                                            1, C.nz, C.st);
+        else if (C.arith == NTUST_ARITH_FP32)
+            launch_smagorinsky_fast<float>(g, rmetric<float>(), b.u, b.v, b.w, b.filt, b.vis,
+                                           1, C.nz, C.st);
+        else
+            launch_smagorinsky(g, b.u, b.v, b.w, nullptr, nullptr, nullptr, b.filt, b.vis,
+                               nullptr, nullptr, nullptr, 1, C.nz, C.st);
+    }
     // main.f90:213
-    if (q.scheme == NTUST_QUICK_CENTRE)
+    if (q.scheme == NTUST_QUICK_CENTRE && C.arith == NTUST_ARITH_FAST)
+        launch_predict_quick_fast<double>(g, qtables<double>(), b.u, b.v, b.w,
+                                          q.les ? b.vis : nullptr, q.dt, q.nu, q.les, b.us,
+                                          b.vs, b.ws, 1, C.nz, C.st);
+    else if (q.scheme == NTUST_QUICK_CENTRE && C.arith == NTUST_ARITH_FP32)
+        launch_predict_quick_fast<float>(g, qtables<float>(), b.u, b.v, b.w,
+                                         q.les ? b.vis : nullptr, q.dt, q.nu, q.les, b.us, b.vs,
+                                         b.ws, 1, C.nz, C.st);
+    else if (q.scheme == NTUST_QUICK_CENTRE)
         launch_predict_quick(g, b.u, b.v, b.w, q.les ? b.vis : nullptr, q.dt, q.nu, q.les, b.us,
                              b.vs, b.ws, 1, C.nz, C.st);
     else
@@ -1096,6 +1140,15 @@ int ntust_finalize(void)
         if (C.recip[m]) keep(hipFree(C.recip[m]));
     for (int m = 0; m < 6; m++)
         if (C.fmetric[m]) keep(hipFree(C.fmetric[m]));
+    for (int a = 0; a < 3; a++)
+        for (int q = 0; q < 2; q++) {
+            if (C.qtab64[a][q]) keep(hipFree(C.qtab64[a][q]));
+            if (C.qtab32[a][q]) keep(hipFree(C.qtab32[a][q]));
+        }
+    for (int m = 0; m < 6; m++) {
+        if (C.rmet64[m]) keep(hipFree(C.rmet64[m]));
+        if (C.rmet32[m]) keep(hipFree(C.rmet32[m]));
+    }
     if (C.ctl) keep(hipFree(C.ctl));
     if (C.pmax) keep(hipFree(C.pmax));
     if (C.ev0) keep(hipEventDestroy(C.ev0));
@@ -1148,6 +1201,54 @@ static int init_device(int nx, int ny, int nz, const double *iDx, const double *
         HIPCHK(hipMemcpy(C.fmetric[a], fw.data(), fw.size() * sizeof(float), hipMemcpyHostToDevice));
         HIPCHK(hipMalloc(&C.fmetric[3 + a], fr.size() * sizeof(float)));
         HIPCHK(hipMemcpy(C.fmetric[3 + a], fr.data(), fr.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    // tables of the fast predictor and eddy viscosity: QUICK face
+    // coefficients per axis and kind, reciprocal metrics (FP64 and FP32)
+    for (int a = 0; a < 3; a++) {
+        const int n = a == 0 ? nx : (a == 1 ? ny : nz);
+        const std::vector<double> &iD = C.h_metric[2 * a], &Ds = C.h_metric[2 * a + 1];
+        for (int q = 0; q < 2; q++) {
+            // element m of a metric is at index m+1 (m = -1 .. n+2)
+            auto L = [&](int m) { return q == 0 ? iD[m + 2] : Ds[m + 1]; };
+            auto Cw = [&](int m) { return q == 0 ? Ds[m + 1] : iD[m + 1]; };
+            std::vector<QCoef<double>> t((size_t)n + 3, QCoef<double>{});
+            std::vector<QCoef<float>> tf((size_t)n + 3, QCoef<float>{});
+            for (int m = 0; m <= n; m++) {
+                QCoef<double> c;
+                const double la = 0.125 * L(m) * L(m) / Cw(m), lb = 0.125 * L(m) * L(m) / Cw(m + 1);
+                c.am1 = -la / L(m - 1);
+                c.a0 = 0.5 + la / L(m) + la / L(m - 1);
+                c.ap1 = 0.5 - la / L(m);
+                c.b0 = 0.5 - lb / L(m);
+                c.bp1 = 0.5 + lb / L(m + 1) + lb / L(m);
+                c.bp2 = -lb / L(m + 1);
+                c.rL = 1.0 / L(m);
+                c.rC = 1.0 / Cw(m);
+                t[(size_t)m + 1] = c;
+                tf[(size_t)m + 1] = QCoef<float>{(float)c.am1, (float)c.a0, (float)c.ap1,
+                                                 (float)c.b0,  (float)c.bp1, (float)c.bp2,
+                                                 (float)c.rL,  (float)c.rC};
+            }
+            HIPCHK(hipMalloc(&C.qtab64[a][q], t.size() * sizeof(t[0])));
+            HIPCHK(hipMemcpy(C.qtab64[a][q], t.data(), t.size() * sizeof(t[0]),
+                             hipMemcpyHostToDevice));
+            HIPCHK(hipMalloc(&C.qtab32[a][q], tf.size() * sizeof(tf[0])));
+            HIPCHK(hipMemcpy(C.qtab32[a][q], tf.data(), tf.size() * sizeof(tf[0]),
+                             hipMemcpyHostToDevice));
+        }
+    }
+    for (int m = 0; m < 6; m++) {
+        const std::vector<double> &M = C.h_metric[m];
+        std::vector<double> r(M.size());
+        std::vector<float> rf(M.size());
+        for (size_t e = 0; e < M.size(); e++) {
+            r[e] = 1.0 / M[e];
+            rf[e] = (float)r[e];
+        }
+        HIPCHK(hipMalloc(&C.rmet64[m], r.size() * sizeof(double)));
+        HIPCHK(hipMemcpy(C.rmet64[m], r.data(), r.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMalloc(&C.rmet32[m], rf.size() * sizeof(float)));
+        HIPCHK(hipMemcpy(C.rmet32[m], rf.data(), rf.size() * sizeof(float), hipMemcpyHostToDevice));
     }
     HIPCHK(hipMalloc(&C.ctl, sizeof(SorControl)));
     HIPCHK(hipMalloc(&C.pmax, sizeof(double)));
@@ -2081,6 +2182,14 @@ int ntust_prof_reset(void)
     C.prof_ms = 0.0;
     C.prof_launches = 0;
     C.prof_sweeps = 0;
+    return 0;
+}
+
+int ntust_set_arithmetic(int mode)
+{
+    if (mode != NTUST_ARITH_EXACT && mode != NTUST_ARITH_FAST && mode != NTUST_ARITH_FP32)
+        return fail(NTUST_E_ARG, "unknown arithmetic mode %d", mode);
+    C.arith = mode;
     return 0;
 }
 

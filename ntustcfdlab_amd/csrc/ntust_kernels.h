@@ -128,6 +128,30 @@ void launch_smagorinsky(const KGrid &g, const double *u, const double *v, const 
                         const double *filt, double *visc, double *dudx, double *dvdx,
                         double *dwdx, int k0, int k1, hipStream_t st);
 
+// fast arithmetic modes of the predictor and the eddy viscosity (k_fast.hip).
+// QCoef: the QUICK face between nodes m and m+1 of one axis -- coefficients
+// of phi(m-1), phi(m), phi(m+1) for a positive advecting velocity (am1, a0,
+// ap1), of phi(m), phi(m+1), phi(m+2) otherwise (b0, bp1, bp2) -- and the
+// reciprocal node spacing 1/L(m) and control width 1/C(m).  QTables: per
+// axis, kind 0 the component's own axis (L(m) = iD(m+1), C(m) = D_s(m)),
+// kind 1 a transverse one (L(m) = D_s(m), C(m) = iD(m)); entry m+1 holds
+// face m.  RMetric: reciprocal metrics, offset by +1 like KGrid.
+template <typename T>
+struct QCoef { T am1, a0, ap1, b0, bp1, bp2, rL, rC; };
+template <typename T>
+struct QTables { const QCoef<T> *t[3][2]; };
+template <typename T>
+struct RMetric { const T *riDx, *rDxs, *riDy, *rDys, *riDz, *rDzs; };
+template <typename T>
+void launch_predict_quick_fast(const KGrid &g, const QTables<T> &q, const double *u,
+                               const double *v, const double *w, const double *visc, double dt,
+                               double nu, int les, double *us, double *vs, double *ws, int k0,
+                               int k1, hipStream_t st);
+template <typename T>
+void launch_smagorinsky_fast(const KGrid &g, const RMetric<T> &r, const double *u,
+                             const double *v, const double *w, const double *filt,
+                             double *visc, int k0, int k1, hipStream_t st);
+
 // predictor
 void launch_predict_quick(const KGrid &g, const double *u, const double *v, const double *w,
                           const double *visc, double dt, double nu, int les, double *us,

@@ -24,6 +24,7 @@ LIB = Path(__file__).resolve().parent / "libntust_hip.so"
 
 SOR_LEXICOGRAPHIC, SOR_REDBLACK, SOR_MIXED = 0, 1, 2
 QUICK_CENTRE, UPWIND = 0, 1
+ARITH_EXACT, ARITH_FAST, ARITH_FP32 = 0, 1, 2
 WARN_ITMAX = 1
 E_STATE = -5
 E_IO = -6
@@ -76,6 +77,7 @@ def load_library() -> ct.CDLL:
         "ntust_prof_reset": [],
         "ntust_prof_read": [_D, ct.POINTER(ct.c_longlong)],
         "ntust_prof_copy": [ct.c_double, _D],
+        "ntust_set_arithmetic": [i],
         "ntust_set_body_force": [_D, _D, _D],
         "ntust_kinetic_energy": [_D],
         "ntust_checkpoint_reserve": [i],
@@ -298,6 +300,11 @@ class ProjectionSolver:
         b = ct.c_double()
         k = self.lib.ntust_prof_kernel(ct.byref(b))
         return k, b.value
+
+    def set_arithmetic(self, mode: int):
+        """ARITH_EXACT (default), ARITH_FAST or ARITH_FP32 for the resident
+        step's QUICK predictor and Smagorinsky viscosity."""
+        _check(self.lib.ntust_set_arithmetic(int(mode)))
 
     def prof_copy_calibration(self, sizes=(64 << 20, 1 << 30)):
         """Plain device copies of the given sizes (ntust_prof_copy), for the
