@@ -316,6 +316,31 @@ int ntust_ray_casting(const double *az0, const double *ay0, int poly, double AOA
  * only; the caller then calls ntust_ray_casting (main.f90:144-155). */
 int ntust_dynamic_aoa(double AOA1, double time, double dt, double StartDynamic, double *AOA);
 
+/* ---- Lagrangian-marker direct forcing --------------------------------------
+ *
+ * No counterpart in the reference, whose immersed boundary is the Eulerian
+ * volume fraction ETA blended into the velocity after the projection
+ * (calcul_new_velocity.f90:61-100).  With markers set, the resident step
+ * forces the post-projection velocity u1 at the same place instead:
+ * U = I u1 at the markers, F = -U/dt (a resting body), u2 = u1 + dt S F,
+ * and FY, FZ = S F feed the force integral (virtualForceIntegrator.f90:22-81)
+ * as before.  Keep ETA = 0 with markers (the two forcings are alternatives).
+ * Markers lie on a curve in the (z, y) plane and are extruded along x: one
+ * marker (m, ix) at (xc(ix), ym[m], zm[m]) for every x cell ix, of weight
+ * wm[m] * iDx(ix).  I and S use the cubic B-spline kernel (support 2
+ * widths, twice differentiable) with widths hx, hy, hz per axis, on each component's
+ * staggered points, and are transposes in the volume-weighted products;
+ * all sums run in a fixed order (reproducible bit for bit).  Xn, Yn, Zn:
+ * the node arrays X(-1:nx+3), Y(-1:ny+3), Z(-1:nz+3) of the gridder.  A
+ * grid finer than a kernel width at a marker (more than 4 points within
+ * 2 h) is refused (NTUST_E_ARG).  n = 0 removes the markers. */
+int ntust_set_markers(int n, const double *zm, const double *ym, const double *wm, double hx,
+                      double hy, double hz, const double *Xn, const double *Yn, const double *Zn);
+
+/* The marker velocities U of the last step, U[(a*n + m)*nx + ix-1] for
+ * component a (0 u, 1 v, 2 w). */
+int ntust_marker_velocity(double *U);
+
 /* ---- discrete adjoint of the resident step ------------------------------
  *
  * No counterpart in the reference: gradients of an objective of the flow
@@ -377,6 +402,13 @@ int ntust_adjoint_gradient(double *gfu, double *gfv, double *gfw);
  * chain rule.  Default: (1, 0, 0, 0), the kinetic energy alone.  Kept
  * until changed or ntust_finalize. */
 int ntust_adjoint_objective(double w_ke, double w_cd, double w_cl, int eta_gradient);
+
+/* With markers and eta_gradient set: dJ/dZ_m and dJ/dY_m, the sensitivity
+ * of J to the position of marker m in the (z, y) plane (its x copies
+ * summed), accumulated like dJ/dETA; with the markers of a geometry and
+ * their motion dX_m/dAOA it gives dJ/dAOA by the chain rule.  No
+ * counterpart in the reference. */
+int ntust_adjoint_marker_gradient(double *gz, double *gy);
 
 /* dJ/dETA accumulated by the last ntust_adjoint_run (or by the
  * ntust_adjoint_step calls since ntust_adjoint_set), a halo array with the

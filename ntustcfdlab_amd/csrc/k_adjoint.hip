@@ -174,25 +174,6 @@ void launch_adj_boundary_conditions(const KGrid &g, const BcTable &bc, double *l
 // and C_L alike in FY with FYz, FYy.  FZ = (w2 - w1)/dt with
 // w2 = ETA w_solid + (1-ETA) w1 and w_solid = 0 in the resident step, so
 // dFZ/dw1 = ((1-ETA) - 1)/dt and dFZ/dETA = -w1/dt.
-__device__ __forceinline__ double nz_mask(const double *a, long s)
-{
-    return a[s] != 0.0 ? 1.0 : 0.0;
-}
-
-__device__ __forceinline__ double force_weight(const KGrid &g, const double *fz, const double *fy,
-                                               int i, int j, int k, double lx)
-{
-    const long px = g.nx + 2;
-    double ax = 4.0 * nz_mask(fy, i);
-    if (i > 1) ax += nz_mask(fy, i - 1);
-    if (i < g.nx) ax += nz_mask(fy, i + 1);
-    double ay = 4.0 * nz_mask(fz, i + px * j);
-    if (j > 1) ay += nz_mask(fz, i + px * (j - 1));
-    if (j < g.ny) ay += nz_mask(fz, i + px * (j + 1));
-    const double az = 4.0 + (k > 1 ? 1.0 : 0.0) + (k < g.nz ? 1.0 : 0.0);
-    return (-2.0 / lx) * (g.Dxs[i] / 6.0 * ax) * (g.Dys[j] / 6.0 * ay) * (g.Dzs[k] / 6.0 * az);
-}
-
 // u = (1-ETA) u1 + ETA u_solid,  u1 = u* - dt (p(i+1) - p(i)) / Dxs(i):
 // lus = (1-ETA) lu on the interior, plus the drag / lift terms of the
 // objective (F.sd = dJ/dC_D, F.sl = dJ/dC_L of this step, with the step's

@@ -111,6 +111,12 @@ k_centre(KGrid g, const double *__restrict__ u, const double *__restrict__ v,
     wc[c] = 0.5 * (w[o] + w[o - g.sxy]);
 }
 
+void launch_centre(const KGrid &g, const double *u, const double *v, const double *w, double *uc,
+                   double *vc, double *wc, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_centre, interior_grid(g), dim3(64, 4), 0, st, g, u, v, w, uc, vc, wc);
+}
+
 void launch_pack_q(const KGrid &g, const double *pre, const double *uc, const double *vc,
                    const double *wc, const double *eta, float *q, hipStream_t st)
 {

@@ -117,6 +117,9 @@ struct Ctx {
     // C_D and C_L over the averaging window; ETA gradient wanted
     double adj_w[3] = {1.0, 0.0, 0.0};
     bool adj_eta = false;
+    // Lagrangian markers of the direct forcing (device arrays in the named
+    // buffers "mk_*"); off when mk.n == 0
+    MarkerSet mk{};
 };
 
 Ctx C;
@@ -874,6 +877,12 @@ RMetric<T> rmetric()
     return RMetric<T>{m[0] + 1, m[1] + 1, m[2] + 1, m[3] + 1, m[4] + 1, m[5] + 1};
 }
 
+// dJ/dZ_m, dJ/dY_m of the markers (2n doubles), zeroed at allocation
+double *marker_gradient()
+{
+    return C.mk.n > 0 ? buf("mk_G", 2 * (size_t)C.mk.n, true) : nullptr;
+}
+
 // One step of main.f90:209-259 on the device state.  force: compute FY, FZ
 // and the force integral (synchronises for cd/cl); centre: write uc, vc, wc.
 int one_step(const StepBufs &b, bool force, bool centre, int *ik, double *pm, double *cd,
@@ -920,6 +929,15 @@ int one_step(const StepBufs &b, bool force, bool centre, int *ik, double *pm, do
                    nullptr, nullptr, nullptr, nullptr, force ? b.FY : nullptr,
                    force ? b.FZ : nullptr, centre ? b.uc : nullptr, centre ? b.vc : nullptr,
                    centre ? b.wc : nullptr, b.u, b.v, b.w, b.u2, b.v2, b.w2, 1, C.nz, C.st);
+    // marker direct forcing on the post-projection velocity (no counterpart
+    // in the reference; with ETA = 0 the blend above left u = u1)
+    if (C.mk.n > 0) {
+        NEED(U, "mk_U", 3 * (size_t)C.mk.n * C.nx, false);
+        launch_marker_interp(g, C.mk, b.u, b.v, b.w, U, C.st);
+        launch_marker_force(g, C.mk, U, q.dt, b.u, b.v, b.w, force ? b.FY : nullptr,
+                            force ? b.FZ : nullptr, C.st);
+        if (centre) launch_centre(g, b.u, b.v, b.w, b.uc, b.vc, b.wc, C.st);
+    }
     // main.f90:251
     if (force) {
         launch_force_integral(g, b.FY, b.FZ, q.lx, b.fxz, b.fyz, b.fxy, b.fyy, b.coef, C.st);
@@ -989,6 +1007,17 @@ int adjoint_of_step(const AdjBufs &a, const double *eta, const double *filt, con
     const KGrid &g = C.g;
     const double *rx = C.recip[0] + 1, *ry = C.recip[1] + 1, *rz = C.recip[2] + 1;
     launch_adj_boundary_conditions(g, C.bc, a.lu, a.lv, a.lw, a.lp, C.st);
+    if (C.mk.n > 0) {
+        // the marker forcing, transposed: lambda_u1 = lambda_u2 - (S I)^T q;
+        // with the step's u1 (geometry gradient wanted) dJ/d(marker position)
+        const size_t nm = (size_t)C.mk.n * C.nx;
+        NEED(L, "mk_L", 3 * nm, false);
+        NEED(GP, "mk_GP", 2 * nm, false);
+        double *G = marker_gradient();
+        if (!G) return fail(NTUST_E_ALLOC, "device allocation of mk_G failed");
+        launch_marker_adjoint(g, C.mk, F, q.dt, a.lu, a.lv, a.lw, F.u1, F.v1, F.w1, L, GP, G,
+                              G + C.mk.n, C.st);
+    }
     launch_adj_project(g, a.lu, a.lv, a.lw, eta, q.dt, F, a.lus, a.lvs, a.lws, a.lp, a.lu, a.lv,
                        a.lw, C.st);
     // the FP32-sweep solver's adjoint runs its transposed sweeps in FP32
@@ -1850,6 +1879,11 @@ int ntust_adjoint_set(const double *lu, const double *lv, const double *lw, cons
         NEED(ge, "adj_geta", C.nh, true);
         HIPCHK(hipMemsetAsync(ge, 0, C.nh * sizeof(double), C.st));
     }
+    if (C.mk.n > 0) {
+        double *G = marker_gradient();
+        if (!G) return fail(NTUST_E_ALLOC, "device allocation of mk_G failed");
+        HIPCHK(hipMemsetAsync(G, 0, 2 * (size_t)C.mk.n * sizeof(double), C.st));
+    }
     return sync();
 }
 
@@ -1894,6 +1928,128 @@ int ntust_adjoint_eta_gradient(double *geta)
                                    "was not set before the adjoint");
     NEED(ge, "adj_geta", C.nh, true);
     if (geta) TRY(d2h(geta, ge, C.nh));
+    return sync();
+}
+
+int ntust_set_markers(int n, const double *zm, const double *ym, const double *wm, double hx,
+                      double hy, double hz, const double *Xn, const double *Yn, const double *Zn)
+{
+    NEED_INIT();
+    if (n < 0) return fail(NTUST_E_ARG, "marker count %d", n);
+    C.mk = MarkerSet{};
+    if (n == 0) return 0;
+    if (!zm || !ym || !wm || !Xn || !Yn || !Zn) return fail(NTUST_E_ARG, "NULL argument");
+    if (!(hx > 0.0 && hy > 0.0 && hz > 0.0))
+        return fail(NTUST_E_ARG, "kernel widths must be > 0");
+    const int nx = C.nx, ny = C.ny, nz = C.nz;
+    auto pt = [](const double *N, int i, bool node) {
+        return node ? N[i + 2] : 0.5 * (N[i + 1] + N[i + 2]);
+    };
+    // window starts: the first point inside the kernel's support (|d| < 2 h),
+    // and no point past the window's last one inside it
+    const double rad = MK_RADIUS;
+    std::vector<int> win(6 * (size_t)n);   // jw[a][m] at a*n+m, kw at (3+a)*n+m
+    for (int a = 0; a < 3; a++)
+        for (int m = 0; m < n; m++) {
+            int j = 1;
+            while (j <= ny && !(pt(Yn, j, a == 1) > ym[m] - rad * hy)) j++;
+            int k = 1;
+            while (k <= nz && !(pt(Zn, k, a == 2) > zm[m] - rad * hz)) k++;
+            if ((j + MK_WIN <= ny && pt(Yn, j + MK_WIN, a == 1) < ym[m] + rad * hy) ||
+                (k + MK_WIN <= nz && pt(Zn, k + MK_WIN, a == 2) < zm[m] + rad * hz))
+                return fail(NTUST_E_ARG,
+                            "marker %d: the grid is finer than the kernel width there (more "
+                            "than %d points within %g h); narrow hy/hz", m, MK_WIN, rad);
+            win[(size_t)a * n + m] = j;
+            win[(size_t)(3 + a) * n + m] = k;
+        }
+    // spread lists: per component, the interior points (j, k) of the windows
+    // in increasing (k, j), each with its markers in increasing order
+    std::vector<int> ints(win);
+    int ne[3], off_sj[3], off_sk[3], off_so[3], off_sm[3];
+    for (int a = 0; a < 3; a++) {
+        std::vector<std::vector<int>> lists((size_t)(ny + 2) * (nz + 2));
+        for (int m = 0; m < n; m++)
+            for (int dk = 0; dk < MK_WIN; dk++)
+                for (int dj = 0; dj < MK_WIN; dj++) {
+                    const int j = win[(size_t)a * n + m] + dj, k = win[(size_t)(3 + a) * n + m] + dk;
+                    if (j >= 1 && j <= ny && k >= 1 && k <= nz)
+                        lists[(size_t)k * (ny + 2) + j].push_back(m);
+                }
+        std::vector<int> sj, sk, so{0}, sm;
+        for (int k = 1; k <= nz; k++)
+            for (int j = 1; j <= ny; j++) {
+                const auto &l = lists[(size_t)k * (ny + 2) + j];
+                if (l.empty()) continue;
+                sj.push_back(j);
+                sk.push_back(k);
+                sm.insert(sm.end(), l.begin(), l.end());
+                so.push_back((int)sm.size());
+            }
+        ne[a] = (int)sj.size();
+        off_sj[a] = (int)ints.size(); ints.insert(ints.end(), sj.begin(), sj.end());
+        off_sk[a] = (int)ints.size(); ints.insert(ints.end(), sk.begin(), sk.end());
+        off_so[a] = (int)ints.size(); ints.insert(ints.end(), so.begin(), so.end());
+        off_sm[a] = (int)ints.size(); ints.insert(ints.end(), sm.begin(), sm.end());
+    }
+    // the kernels take markers and list entries along the launch's y extent
+    if (n > 65535 || ne[0] > 65535 || ne[1] > 65535 || ne[2] > 65535)
+        return fail(NTUST_E_ARG, "%d markers, %d/%d/%d spread points: at most 65535 each", n,
+                    ne[0], ne[1], ne[2]);
+    const size_t nd = 3 * (size_t)n + (nx + 5) + (ny + 5) + (nz + 5);
+    std::vector<double> dbl;
+    dbl.reserve(nd);
+    dbl.insert(dbl.end(), zm, zm + n);
+    dbl.insert(dbl.end(), ym, ym + n);
+    dbl.insert(dbl.end(), wm, wm + n);
+    dbl.insert(dbl.end(), Xn, Xn + nx + 5);
+    dbl.insert(dbl.end(), Yn, Yn + ny + 5);
+    dbl.insert(dbl.end(), Zn, Zn + nz + 5);
+    NEED(dd, "mk_dbl", nd, false);
+    NEED(di, "mk_int", ints.size() / 2 + 1, false);
+    HIPCHK(hipMemcpyAsync(dd, dbl.data(), nd * sizeof(double), hipMemcpyHostToDevice, C.st));
+    HIPCHK(hipMemcpyAsync(di, ints.data(), ints.size() * sizeof(int), hipMemcpyHostToDevice,
+                          C.st));
+    TRY(sync());
+    const int *ip = (const int *)di;
+    MarkerSet &M = C.mk;
+    M.hx = hx; M.hy = hy; M.hz = hz;
+    M.zm = dd; M.ym = dd + n; M.wm = dd + 2 * n;
+    M.Xn = dd + 3 * n; M.Yn = M.Xn + nx + 5; M.Zn = M.Yn + ny + 5;
+    for (int a = 0; a < 3; a++) {
+        M.jw[a] = ip + (size_t)a * n;
+        M.kw[a] = ip + (size_t)(3 + a) * n;
+        M.ne[a] = ne[a];
+        M.sj[a] = ip + off_sj[a];
+        M.sk[a] = ip + off_sk[a];
+        M.so[a] = ip + off_so[a];
+        M.sm[a] = ip + off_sm[a];
+    }
+    M.n = n;
+    double *G = marker_gradient();
+    if (!G) return fail(NTUST_E_ALLOC, "device allocation of mk_G failed");
+    HIPCHK(hipMemsetAsync(G, 0, 2 * (size_t)n * sizeof(double), C.st));
+    return sync();
+}
+
+int ntust_marker_velocity(double *U)
+{
+    NEED_INIT();
+    if (C.mk.n <= 0) return fail(NTUST_E_STATE, "no markers set");
+    if (!U) return fail(NTUST_E_ARG, "NULL argument");
+    NEED(dU, "mk_U", 3 * (size_t)C.mk.n * C.nx, false);
+    TRY(d2h(U, dU, 3 * (size_t)C.mk.n * C.nx));
+    return sync();
+}
+
+int ntust_adjoint_marker_gradient(double *gz, double *gy)
+{
+    NEED_INIT();
+    if (C.mk.n <= 0) return fail(NTUST_E_STATE, "no markers set");
+    double *G = marker_gradient();
+    if (!G) return fail(NTUST_E_ALLOC, "device allocation of mk_G failed");
+    if (gz) TRY(d2h(gz, G, C.mk.n));
+    if (gy) TRY(d2h(gy, G + C.mk.n, C.mk.n));
     return sync();
 }
 
@@ -2120,6 +2276,11 @@ int ntust_adjoint_run(int nsteps, int checkpoint_every, int navg, double *J, int
     double *clear[7] = {a.lu, a.lv, a.lw, a.lp, a.gu, a.gv, a.gw};
     for (double *x : clear) HIPCHK(hipMemsetAsync(x, 0, nh * sizeof(double), C.st));
     if (geta) HIPCHK(hipMemsetAsync(geta, 0, nh * sizeof(double), C.st));
+    if (C.mk.n > 0) {
+        double *G = marker_gradient();
+        if (!G) return fail(NTUST_E_ALLOC, "device allocation of mk_G failed");
+        HIPCHK(hipMemsetAsync(G, 0, 2 * (size_t)C.mk.n * sizeof(double), C.st));
+    }
     for (int sg = nseg - 1; sg >= 0; sg--) {
         const int s0 = sg * K, s1 = s0 + K < N ? s0 + K : N;
         if (sg != nseg - 1) {

@@ -319,6 +319,103 @@ void launch_copy_shell(const KGrid &g, const double *u, const double *v, const d
                        double *a0, double *a1, double *a2, double *b0, double *b1,
                        double *b2, hipStream_t st);
 
+// weight of a cell's FZ (FY) in C_D (C_L) with the step's skip pattern:
+// virtualForceIntegrator.f90:22-81 with the line sums fz = FXz (FYz), fy =
+// FXy (FYy); a line sample that is exactly zero is skipped, so only the
+// masks of the line sums enter
+__device__ __forceinline__ double nz_mask(const double *a, long s)
+{
+    return a[s] != 0.0 ? 1.0 : 0.0;
+}
+
+__device__ __forceinline__ double force_weight(const KGrid &g, const double *fz, const double *fy,
+                                               int i, int j, int k, double lx)
+{
+    const long px = g.nx + 2;
+    double ax = 4.0 * nz_mask(fy, i);
+    if (i > 1) ax += nz_mask(fy, i - 1);
+    if (i < g.nx) ax += nz_mask(fy, i + 1);
+    double ay = 4.0 * nz_mask(fz, i + px * j);
+    if (j > 1) ay += nz_mask(fz, i + px * (j - 1));
+    if (j < g.ny) ay += nz_mask(fz, i + px * (j + 1));
+    const double az = 4.0 + (k > 1 ? 1.0 : 0.0) + (k < g.nz ? 1.0 : 0.0);
+    return (-2.0 / lx) * (g.Dxs[i] / 6.0 * ax) * (g.Dys[j] / 6.0 * ay) * (g.Dzs[k] / 6.0 * az);
+}
+
+// ---- Lagrangian markers (k_marker.hip) -----------------------------------
+//
+// Markers on a curve in the (z, y) plane, extruded along x: marker (m, ix)
+// sits at (xc(ix), Y_m, Z_m), ix = 1..nx, with weight W = wm[m] * iDx(ix).
+// The discrete delta of component a at its staggered point c is
+//   w = (phi(dx/hx)/hx * phi(dy/hy)/hy) * phi(dz/hz)/hz,  d = x_c - X_m,
+// phi the cubic B-spline (support |r| < 2; weights over unit-spaced points
+// sum to 1, first moment 0, twice continuously differentiable, so the
+// forcing is twice differentiable in the marker positions).  Each marker
+// reads the points of a 5 (x) x 4 (y) x 4 (z) window, the y and z windows
+// starting at jw[a][m], kw[a][m]; the points outside the interior
+// are left out.  Spreading gathers, for each interior point of the
+// spread lists, the markers whose windows hold it (CSR per component), so
+// interpolation I u = sum_c w u_c dv_c and spreading S F = sum_m w W F_m
+// are exact transposes in the dv- and W-weighted inner products.
+constexpr int MK_WIN = 4;          // window points per axis in y and z
+constexpr double MK_RADIUS = 2.0;  // support of the kernel, in kernel widths
+
+__host__ __device__ inline double mk_phi(double r)
+{
+    const double a = fabs(r);
+    if (a <= 1.0) return (4.0 - 6.0 * (a * a) + 3.0 * (a * a * a)) / 6.0;
+    if (a < 2.0) {
+        const double b = 2.0 - a;
+        return (b * b * b) / 6.0;
+    }
+    return 0.0;
+}
+
+// d phi / d r
+__host__ __device__ inline double mk_dphi(double r)
+{
+    const double a = fabs(r);
+    double d = 0.0;
+    if (a <= 1.0) {
+        d = (-12.0 * a + 9.0 * (a * a)) / 6.0;
+    } else if (a < 2.0) {
+        const double b = 2.0 - a;
+        d = -(b * b) / 2.0;
+    }
+    return r < 0.0 ? -d : d;
+}
+
+struct AdjForce;
+
+struct MarkerSet {
+    int n;                                  // markers in the (z, y) plane
+    double hx, hy, hz;                      // kernel widths
+    const double *zm, *ym, *wm;             // positions, weights per unit span
+    const int *jw[3], *kw[3];               // window starts per component
+    const double *Xn, *Yn, *Zn;             // node coordinates, element i at [i+1]
+    // spread lists per component: entry e covers the points (i, sj[e], sk[e])
+    // for all i, with markers sm[so[e] .. so[e+1]-1] in increasing order
+    int ne[3];
+    const int *sj[3], *sk[3], *so[3], *sm[3];
+};
+
+// U[(a*n + m)*nx + ix-1] = (I u)_a at marker (m, ix) from the velocity u, v, w
+void launch_marker_interp(const KGrid &g, const MarkerSet &M, const double *u, const double *v,
+                          const double *w, double *U, hipStream_t st);
+// forcing F = -U/dt spread and applied: u += dt S F; FY, FZ (may be null)
+// get the spread force of v and w on the listed points
+void launch_marker_force(const KGrid &g, const MarkerSet &M, const double *U, double dt,
+                         double *u, double *v, double *w, double *FY, double *FZ,
+                         hipStream_t st);
+// adjoint of the forcing of one step: q = lambda + dJ/dF / dt on the listed
+// points (the force weights of F, as in the transposed projection);
+// lambda -= (S I)^T q; with u1 (the step's post-projection velocity, may be
+// null: no marker gradient) gz, gy += dJ/dZ_m, dJ/dY_m
+void launch_marker_adjoint(const KGrid &g, const MarkerSet &M, const AdjForce &F, double dt,
+                           double *lu, double *lv, double *lw, const double *u1,
+                           const double *v1, const double *w1, double *L, double *GP,
+                           double *gz, double *gy, hipStream_t st);
+
 // discrete adjoint of the resident step (transposed stages, reverse order)
 void launch_adj_boundary_conditions(const KGrid &g, const BcTable &bc, double *lu, double *lv,
                                     double *lw, double *lp, hipStream_t st);
@@ -401,6 +498,9 @@ void launch_steady(const KGrid &g, const double *wc, double *last, int compare,
                    unsigned long long *vmax, hipStream_t st);
 // plain copy of words16 16-byte words (counter calibration)
 void launch_stream_copy(const void *src, void *dst, long words16, hipStream_t st);
+// uc, vc, wc = the cell-centred velocity of u, v, w (calcul_new_velocity.f90:171-181)
+void launch_centre(const KGrid &g, const double *u, const double *v, const double *w, double *uc,
+                   double *vc, double *wc, hipStream_t st);
 void launch_unpack_q(const KGrid &g, const float *q, double *u, double *v, double *w, double *p,
                      double *pre, double *uc, double *vc, double *wc, hipStream_t st);
 

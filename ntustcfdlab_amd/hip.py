@@ -78,6 +78,9 @@ def load_library() -> ct.CDLL:
         "ntust_prof_read": [_D, ct.POINTER(ct.c_longlong)],
         "ntust_prof_copy": [ct.c_double, _D],
         "ntust_set_arithmetic": [i],
+        "ntust_set_markers": [i, _D, _D, _D, d, d, d, _D, _D, _D],
+        "ntust_marker_velocity": [_D],
+        "ntust_adjoint_marker_gradient": [_D, _D],
         "ntust_set_body_force": [_D, _D, _D],
         "ntust_kinetic_energy": [_D],
         "ntust_checkpoint_reserve": [i],
@@ -300,6 +303,34 @@ class ProjectionSolver:
         b = ct.c_double()
         k = self.lib.ntust_prof_kernel(ct.byref(b))
         return k, b.value
+
+    def set_markers(self, zm=None, ym=None, wm=None, h=(1.0, 1.0, 1.0)):
+        """Lagrangian-marker direct forcing (ntust_set_markers): marker
+        positions zm, ym and weights wm per unit span in the (z, y) plane,
+        kernel widths h = (hx, hy, hz); no arguments remove the markers."""
+        if zm is None:
+            _check(self.lib.ntust_set_markers(0, None, None, None, 1.0, 1.0, 1.0, None, None,
+                                              None))
+            self.n_markers = 0
+            return
+        zm, ym, wm = (np.ascontiguousarray(a, dtype=np.float64) for a in (zm, ym, wm))
+        X, Y, Z = (np.ascontiguousarray(a, dtype=np.float64)
+                   for a in (self.grid.X, self.grid.Y, self.grid.Z))
+        _check(self.lib.ntust_set_markers(len(zm), _p(zm), _p(ym), _p(wm), float(h[0]),
+                                          float(h[1]), float(h[2]), _p(X), _p(Y), _p(Z)))
+        self.n_markers = len(zm)
+
+    def marker_velocity(self):
+        """U[a, m, ix] of the last step."""
+        U = np.zeros(3 * self.n_markers * self.grid.nx)
+        _check(self.lib.ntust_marker_velocity(_p(U)))
+        return U.reshape(3, self.n_markers, self.grid.nx)
+
+    def adjoint_marker_gradient(self):
+        """(dJ/dZ_m, dJ/dY_m) of the markers."""
+        gz, gy = np.zeros(self.n_markers), np.zeros(self.n_markers)
+        _check(self.lib.ntust_adjoint_marker_gradient(_p(gz), _p(gy)))
+        return gz, gy
 
     def set_arithmetic(self, mode: int):
         """ARITH_EXACT (default), ARITH_FAST or ARITH_FP32 for the resident

@@ -961,6 +961,10 @@ int or_step(const or_grid *g, const or_params *prm, or_fields *f,
     or_calcul_new_velocity(g, f->us, f->vs, f->ws, f->p, f->ETA, 0.0, 0.0, 0.0,
                            prm->dt, 1, nz, f->u1, f->v1, f->w1, f->u2, f->v2,
                            f->w2, f->FX, f->FY, f->FZ, f->uc, f->vc, f->wc);
+    /* Lagrangian-marker forcing of the library, when markers are set (no
+     * counterpart in the reference; ntust_markers.c) */
+    if (prm->markers && or_marker_count() > 0)
+        or_marker_forcing(g, prm->dt, f->u2, f->v2, f->w2, f->FY, f->FZ, f->uc, f->vc, f->wc);
     /* main.f90:251 */
     or_virtual_force_integrator(g, f->FY, f->FZ, prm->lx, cd, cl);
     /* main.f90:255 */

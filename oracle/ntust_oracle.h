@@ -155,6 +155,7 @@ typedef struct {
     double dt, nu, Cs, omega, zeta, itmax, lx;
     int LES, scheme, sor;
     int bc[18];
+    int markers;   /* apply the marker forcing of or_set_markers (ntust_markers.c) */
 } or_params;
 
 int or_step(const or_grid *g, const or_params *prm, or_fields *f,
@@ -215,6 +216,21 @@ void or_matmul_ell(double *x, const double *coef, const int *jcoef,
 /* solver.f90:1-144 solver_BICG3, serial sums; returns the iteration count */
 int or_solver_bicg3(const double *coef, const int *jcoef, const double *b,
                     double *x, int n, int m, double eps, int itmax, double *normout);
+
+/* ---- Lagrangian-marker direct forcing (ntust_markers.c) --------------
+ * The library's marker forcing restated (no counterpart in the reference);
+ * or_step applies it after the projection when markers are set. */
+double or_mk_phi(double r);
+double or_mk_dphi(double r);
+int or_set_markers(const or_grid *g, int n, const double *zm, const double *ym,
+                   const double *wm, double hx, double hy, double hz, const double *Xn,
+                   const double *Yn, const double *Zn);
+int or_marker_count(void);
+void or_marker_interp(const or_grid *g, const double *u, const double *v, const double *w,
+                      double *U);
+void or_marker_spread(const or_grid *g, const double *F, double *fu, double *fv, double *fw);
+void or_marker_forcing(const or_grid *g, double dt, double *u2, double *v2, double *w2,
+                       double *FY, double *FZ, double *uc, double *vc, double *wc);
 
 #ifdef __cplusplus
 }

@@ -36,7 +36,7 @@ class OrParams(ct.Structure):
     _fields_ = [("dt", ct.c_double), ("nu", ct.c_double), ("Cs", ct.c_double),
                 ("omega", ct.c_double), ("zeta", ct.c_double), ("itmax", ct.c_double),
                 ("lx", ct.c_double), ("LES", ct.c_int), ("scheme", ct.c_int),
-                ("sor", ct.c_int), ("bc", ct.c_int * 18)]
+                ("sor", ct.c_int), ("bc", ct.c_int * 18), ("markers", ct.c_int)]
 
 
 def _load():
@@ -220,6 +220,43 @@ class Oracle:
         return x, it, nrm.value
 
     # reading_data.f90:47-83
+    # Lagrangian-marker forcing (ntust_markers.c)
+    def set_markers(self, zm, ym, wm, h):
+        zm, ym, wm = (np.ascontiguousarray(a, dtype=np.float64) for a in (zm, ym, wm))
+        self._mk = (zm, ym, wm)
+        X, Y, Z = (np.ascontiguousarray(a, dtype=np.float64)
+                   for a in (self.grid.X, self.grid.Y, self.grid.Z))
+        self._mkn = (X, Y, Z)
+        _lib.or_set_markers.restype = ct.c_int
+        return _lib.or_set_markers(ct.byref(self.g), len(zm), _p(zm), _p(ym), _p(wm),
+                                   ct.c_double(h[0]), ct.c_double(h[1]), ct.c_double(h[2]),
+                                   _p(X), _p(Y), _p(Z))
+
+    def clear_markers(self):
+        _lib.or_set_markers(ct.byref(self.g), 0, None, None, None, ct.c_double(1.0),
+                            ct.c_double(1.0), ct.c_double(1.0), None, None, None)
+
+    def marker_interp(self, u, v, w, n):
+        U = np.zeros(3 * n * self.grid.nx)
+        _lib.or_marker_interp(ct.byref(self.g), _p(u), _p(v), _p(w), _p(U))
+        return U.reshape(3, n, self.grid.nx)
+
+    def marker_spread(self, F):
+        F = np.ascontiguousarray(F, dtype=np.float64).ravel()
+        out = [self.grid.halo() for _ in range(3)]
+        _lib.or_marker_spread(ct.byref(self.g), _p(F), *[_p(a) for a in out])
+        return out
+
+    @staticmethod
+    def mk_phi(r):
+        _lib.or_mk_phi.restype = ct.c_double
+        return _lib.or_mk_phi(ct.c_double(r))
+
+    @staticmethod
+    def mk_dphi(r):
+        _lib.or_mk_dphi.restype = ct.c_double
+        return _lib.or_mk_dphi(ct.c_double(r))
+
     def reading_variables(self, q, u, v, w, p, pre, uc, vc, wc):
         assert q.dtype == np.float32 and q.flags.f_contiguous
         _lib.or_reading_variables(ct.byref(self.g), q.ctypes.data_as(ct.POINTER(ct.c_float)),
@@ -239,8 +276,14 @@ class OracleRun:
                                    "us", "vs", "ws", "FX", "FY", "FZ")}
         self.f.update({n: In() for n in ("Viseff", "uc", "vc", "wc", "pre")})
         self.prm = OrParams(dt, nu, Cs, omega, zeta, itmax, lx, LES, scheme, sor,
-                            (ct.c_int * 18)(*[int(c) for c in bc]))
+                            (ct.c_int * 18)(*[int(c) for c in bc]), 0)
         self.bc = bc
+
+    def set_markers(self, zm, ym, wm, h):
+        """The library's marker forcing in every following step."""
+        rc = self.o.set_markers(zm, ym, wm, h)
+        self.prm.markers = 1 if rc == 0 else 0
+        return rc
 
     def apply_bc(self):
         f = self.f
