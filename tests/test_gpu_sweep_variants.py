@@ -143,3 +143,50 @@ def test_speculative_first_batch_and_replay(mode, hip_lib):
         assert a[0] == b[0] and a[1] == b[1]
         assert np.array_equal(a[2].view(np.uint64), b[2].view(np.uint64))
         assert np.array_equal(a[3].view(np.uint64), b[3].view(np.uint64))
+
+
+@pytest.mark.parametrize("mode", [SOR_REDBLACK, SOR_MIXED])
+def test_speculative_tail_replay(mode, hip_lib):
+    """After a solve that ran to itmax, the two-pass solve runs its first
+    batch (7/8 of itmax) and then the tail up to itmax without the device
+    gate, from a copy of the colour planes.  A stop inside the tail restores
+    the planes from that copy and replays the tail's sweeps up to the stop.
+    zeta is the change of a probe's sweep 650, so the stop lands between
+    7/8 * 700 and 700: every solve must give the gated graph replay's bits."""
+    g = stretched_grid(26, 22, 20)
+    rng = np.random.default_rng(37)
+    us, vs, ws = (rand_halo(g, rng, 1e-2) for _ in range(3))
+    p0 = rand_halo(g, rng, 1e-3)
+    probe = ProjectionSolver(g)
+    try:
+        probe.set_sweep_variant(SWEEP_GRAPH)
+        p = p0.copy(order="F")
+        _, pm650, _ = probe.gauss_seidel(us, vs, ws, p, g.interior(), 1e-3, 1.5, 0.0, 650.0,
+                                         mode)
+    finally:
+        probe.close()
+    z = pm650 * (1.0 + 1e-9)
+    calls = [(0.0, 700.0), (z, 700.0), (0.0, 700.0), (z, 700.0)]
+    res = {}
+    for v in (SWEEP_TWO_PASS, SWEEP_GRAPH):
+        s = ProjectionSolver(g)
+        try:
+            s.set_sweep_variant(v)
+            out = []
+            for zeta, itmax in calls:
+                p = p0.copy(order="F")
+                pre = g.interior()
+                ik, pm, _ = s.gauss_seidel(us, vs, ws, p, pre, 1e-3, 1.5, zeta, itmax, mode)
+                out.append((ik, pm, p, pre))
+            res[v] = out
+        finally:
+            s.close()
+    iks = [r[0] for r in res[SWEEP_TWO_PASS]]
+    print(f"mode {mode}: sweeps {iks}", flush=True)
+    assert iks[0] == 700 and iks[2] == 700
+    for q in (iks[1], iks[3]):
+        assert 0.875 * 700 < q < 700                          # stopped inside the tail
+    for a, b in zip(res[SWEEP_TWO_PASS], res[SWEEP_GRAPH]):
+        assert a[0] == b[0] and a[1] == b[1]
+        assert np.array_equal(a[2].view(np.uint64), b[2].view(np.uint64))
+        assert np.array_equal(a[3].view(np.uint64), b[3].view(np.uint64))
