@@ -12,11 +12,15 @@ dynamic_AOA (airfoil.f90:311-321) by what the geometry fixes:
     over nsub^2;
   - dynamic_AOA equals AOA1 at the start of the pitching and AOA1 + 10 a
     quarter period later, and is even in time about the start.
+  - the reference's own table NACA0012.DAT agrees with the generated
+    polygon (where the reference's files are on the machine).
 CPU only."""
+import os
+
 import numpy as np
 import pytest
 
-from ntustcfdlab_amd.airfoil import naca4_polygon, polygon_area
+from ntustcfdlab_amd.airfoil import naca4_polygon, polygon_area, read_polygon
 from ntustcfdlab_amd.grid import Grid, tanh_nodes
 from oracle import Oracle, dynamic_aoa
 
@@ -150,3 +154,56 @@ def test_dynamic_aoa():
         assert dynamic_aoa(5.0, t0 + tau, dt, start) == pytest.approx(
             dynamic_aoa(5.0, t0 - tau, dt, start), abs=1e-12)
         assert 5.0 <= dynamic_aoa(5.0, t0 + tau, dt, start) <= 15.0
+
+
+REFERENCE_TABLE = "/root/reference/NACA0012.DAT"
+
+
+@pytest.mark.skipif(not os.path.exists(REFERENCE_TABLE),
+                    reason="the reference's airfoil table is not on this machine")
+def test_reference_table_matches_the_generated_polygon():
+    """The reference casts NACA0012.DAT (airfoil.f90:587-591, poly = 5000 at
+    variables_module.f90:161); configurations here use naca4_polygon(5000).
+    The table: 5000 rows, chord 0..1 along z, upper surface then lower, the
+    last row repeating the first.  Both polygons follow the NACA 0012
+    thickness law (open trailing edge; the table to 1e-4 in y -- 5e-5 on
+    its lower surface near z = 0.02 -- the generated one to 1e-6 between
+    its vertices) away from the leading edge, enclose the same area to
+    1e-5, and cast ETA fields that differ by less than 1 % of a cell
+    anywhere and 1e-3 of the area in total.  The closed-trailing-edge law
+    (z^4 coefficient 0.1036) is off by 1.3e-3 and fails."""
+    az, ay = read_polygon(REFERENCE_TABLE, 5000)
+    bz, by = naca4_polygon(poly=5000)
+    assert len(az) == 5000 and az[0] == az[-1] and ay[0] == ay[-1]
+    te = int(np.argmax(az))
+    assert abs(az[te] - 1.0) < 1e-8 and az.min() == 0.0
+    assert np.all(np.diff(az[:te + 1]) >= 0) and np.all(np.diff(az[te:]) <= 0)
+    # the open trailing edge is closed by rows at z = 1 from +y to -y
+    assert np.all(ay[1:te] > 0)
+    lower = az[te:-1] < 1.0 - 1e-6
+    assert np.all(ay[te:-1][lower] < 0) and np.all(ay[te:-1][~lower] <= ay[te])
+    h = (5000 - 2) // 2
+    zz = np.linspace(0.0, 1.0, 20001)
+    law = 5 * 0.12 * (0.2969 * np.sqrt(zz) - 0.1260 * zz - 0.3516 * zz**2 + 0.2843 * zz**3
+                      - 0.1015 * zz**4)
+    away = (zz >= 0.01) & (zz < 1.0)      # z = 1: the closing edge
+    for tz, ty, gz, gy, sign in ((az[:te + 1], ay[:te + 1], bz[:h + 1], by[:h + 1], 1.0),
+                                 (az[te:][::-1], ay[te:][::-1], bz[h:][::-1], by[h:][::-1], -1.0)):
+        yt, yg = np.interp(zz, tz, ty), np.interp(zz, gz, gy)
+        assert np.abs(yt - yg)[away].max() < 1e-4
+        assert np.abs(yt - yg).max() < 5e-4
+        assert np.abs(yt - sign * law)[away].max() < 1e-4
+        assert np.abs(yg - sign * law)[away].max() < 1e-6
+    assert abs(polygon_area(az, ay) / polygon_area(bz, by) - 1.0) < 1e-5
+    g = airfoil_grid(192)
+    o = Oracle(g)
+    etas = []
+    for z0, y0 in ((az, ay), (bz, by)):
+        pz, py = o.airfoil_transform(z0, y0, 8.0, 4.0, 4.0)
+        etas.append(o.ray_casting(pz, py, g.Y, g.Z, 50, g.halo())[0])
+    d = np.abs(etas[0] - etas[1])
+    dz = g.Z[3:g.nz + 3] - g.Z[2:g.nz + 2]
+    dy = g.Y[3:g.ny + 3] - g.Y[2:g.ny + 2]
+    cell = dz[:, None] * dy[None, :]
+    assert d.max() < 0.01
+    assert float((d * cell).sum()) < 1e-3 * float((etas[0] * cell).sum())

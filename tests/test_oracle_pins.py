@@ -432,6 +432,60 @@ def test_projection_identities_and_divergence():
     np.testing.assert_allclose(outs[6][sl], (0.3 - outs[0][sl]) / dt, rtol=1e-15, atol=1e-12)
 
 
+def test_centring_of_linear_fields():
+    """calcul_new_velocity.f90:171-181: uc, vc, wc average a component's two
+    faces of the cell, so a field linear in x, y, z gives its value at the
+    cell centre on any stretched grid (the faces of cell i along x are
+    X(i) and X(i+1), whose mean is the centre).  Averaging faces i, i+1
+    instead, or the wrong axis, is off by a cell width times the slope.  With
+    p = 0 and ETA = 0 the projection leaves u* unchanged, so u2 = u*; the
+    ghost faces i = 0 (j = 0, k = 0) come from the arrays passed as u2."""
+    g = stretched_grid(7, 6, 8)
+    o = Oracle(g)
+    nx, ny, nz = g.shape
+    X, Y, Z = (np.asarray(a) for a in (g.X, g.Y, g.Z))
+    coefs = [(0.3, 1.7, -0.6, 0.9), (-0.2, 0.4, 2.3, -1.1), (0.5, -0.8, 0.35, 1.9)]
+
+    def lin(c, x, y, z):
+        return c[0] + c[1] * x[:, None, None] + c[2] * y[None, :, None] + c[3] * z[None, None, :]
+    xc, yc, zc = (halo_coord(N, n, "c") for N, n in ((X, nx), (Y, ny), (Z, nz)))
+    xf, yf, zf = (halo_coord(N, n, "f") for N, n in ((X, nx), (Y, ny), (Z, nz)))
+    star = [np.asfortranarray(lin(coefs[0], xf, yc, zc)),
+            np.asfortranarray(lin(coefs[1], xc, yf, zc)),
+            np.asfortranarray(lin(coefs[2], xc, yc, zf))]
+    p = g.halo()
+    outs = [g.halo() for _ in range(3)] + [s.copy(order="F") for s in star] + \
+        [g.halo() for _ in range(3)] + [g.interior() for _ in range(3)]
+    o.calcul_new_velocity(*star, p, g.halo(), (0.0, 0.0, 0.0), 1e-2, *outs)
+    sl = (slice(2, -2),) * 3
+    for a in range(3):
+        assert np.array_equal(outs[3 + a][sl], star[a][sl])
+    cx, cy, cz = centres(X, nx), centres(Y, ny), centres(Z, nz)
+    for a in range(3):
+        want = lin(coefs[a], cx, cy, cz)
+        scale = np.abs(want).max()
+        assert np.abs(outs[9 + a] - want).max() < 1e-14 * scale, a
+
+
+def test_updating_velocity_copies_the_interior():
+    """calcul_new_velocity.f90:195-243: u, v, w take u2, v2, w2 on cells
+    1..nx x 1..ny x istart..iend and nothing else changes (ghosts, and the
+    planes outside the slab)."""
+    g = stretched_grid(5, 6, 7)
+    o = Oracle(g)
+    rng = np.random.default_rng(21)
+    src = [np.asfortranarray(rng.standard_normal(g.halo_shape)) for _ in range(3)]
+    for ks, ke in ((1, g.nz), (3, 5)):
+        dst = [np.asfortranarray(rng.standard_normal(g.halo_shape)) for _ in range(3)]
+        before = [d.copy() for d in dst]
+        o.updating_velocity(*src, *dst, kstart=ks, kend=ke)
+        inside = np.zeros(g.halo_shape, bool)
+        inside[2:g.nx + 2, 2:g.ny + 2, ks + 1:ke + 2] = True
+        for s, d, b in zip(src, dst, before):
+            assert np.array_equal(d[inside], s[inside])
+            assert np.array_equal(d[~inside], b[~inside])
+
+
 # ---------------------------------------------------------------------------
 # virtualForceIntegrator.f90
 # ---------------------------------------------------------------------------
