@@ -1,32 +1,38 @@
 #!/usr/bin/env python3
 """Throughput of the projection-method time step on one MI355X.
 
-Workload (the reference's Re=1000 LES cavity): unit cube, uniform 128^3
-finite-volume grid, Re = 1000, QUICK advection, Smagorinsky LES with Cs = 0.1
-and filter width (dx dy dz)^(1/3), dt = 5e-4, lid w = 1 on the north wall,
-no-slip elsewhere; rest plus seeded U(-1e-3, 1e-3) noise on u, v, w; SOR with
-omega = 1.5, zeta = 1e-10, itmax = 3000.  Synthetic data (no files).
+Workload (the reference's Re=1000 LES cavity): unit cube, uniform 256^3
+finite-volume grid (16.8 M cells), Re = 1000, QUICK advection, Smagorinsky
+LES with Cs = 0.1 and filter width (dx dy dz)^(1/3), dt = 5e-4, lid w = 1 on
+the north wall, no-slip elsewhere; rest plus seeded U(-1e-3, 1e-3) noise on
+u, v, w; SOR with omega = 1.5, zeta = 1e-10, itmax = 3000.  Synthetic data
+(no files).
 
 A step is one full pass of the hot path (LES viscosity, predictor, pressure
 solve, projection with direct forcing, force integral, boundary conditions)
 with the state resident in HBM.  W warm-up steps, then K timed steps
 bracketed by HIP events on the library's stream (device synchronised on both
-sides).  Both pressure solvers run from the same state: FP64 red-black SOR and
-FP32 sweeps inside FP64 defect correction; the headline is the latter when its
-velocity is within 1e-6 (relative) of the FP64 solver's.
+sides).  Three solvers run from the same state: red-black SOR in FP64
+("fp64"), the FP32 pattern -- predictor, eddy viscosity and SOR sweeps in
+FP32 inside FP64 defect correction -- ("mixed"), and GS.f90's lexicographic
+order in FP64 ("lex", bit for bit the CPU restatement's result).  The
+headline is "mixed" when its velocity is within 1e-6 (relative, max norm)
+of "lex"; each solver's error against "lex" is reported.
 
-Further legs in the same JSON line: the same cavity at 256^3 (the size the
->= 70 % HBM and >= 2x FP32-refine targets refer to), the airfoil's ETA ray
-casting on a 512x384x128 grid (the per-step cost of the pitching mode) and the
-airfoil LES on that grid (both solvers), the
-discrete adjoint on a 96^3 Re=400 cavity (cost relative to the forward,
-central-difference check), and the CPU restatement timed on a bounded
-sample.  One JSON line on stdout.
+Further legs in the same JSON line: the same cavity at 128^3, the airfoil's
+ETA ray casting on a 512x384x128 grid (the per-step cost of the pitching
+mode), the airfoil LES on that grid and on the 64x192x384 grid, the airfoil
+adjoint (cost, actuator-strip and dC_D/dAOA checks, through the cast ETA and
+through Lagrangian markers), the discrete adjoint of a 96^3 Re=400 cavity
+and of the 256^3 LES cavity, and the CPU restatement timed on bounded
+samples (red-black on all threads, GS.f90's order on one).  One JSON line on
+stdout.
 """
 from __future__ import annotations
 
 import argparse
 import json
+import os
 import sys
 import time
 from pathlib import Path
@@ -57,19 +63,32 @@ def initial_state(grid, w, solver):
     return u, v, ww, p, eta
 
 
+def cpu_threads():
+    """Threads for the all-core CPU figure: OMP_NUM_THREADS when set (the
+    CPU share a job is given), else the cores this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return int(env) if env.isdigit() and int(env) > 0 else len(os.sched_getaffinity(0))
+
+
 def cpu_baseline(grid, w, state, sweeps_per_step):
-    """The CPU restatement (one core) timed on a bounded sample of the same
-    workload: one step from the same state with the sweep cap at 4 and at 12;
-    the difference gives the time per SOR sweep, the rest the time of the
-    other kernels; the step time is extrapolated to the GPU run's mean sweep
-    count."""
+    """The CPU restatement timed on bounded samples of the same workload:
+    one step from the same state at two sweep caps; the difference gives the
+    time per SOR sweep, the rest the time of the other kernels, and the step
+    time is extrapolated to the GPU run's mean sweep count.  Two forms:
+      - red-black SOR with the restatement's OpenMP loops on all threads
+        (every independent-cell loop over k; results equal the serial
+        loop's) -- the reported value;
+      - GS.f90's lexicographic sweep on one thread (the reference's own
+        order; its OpenMP loop at GS.f90:36 shares pNew and the maxima
+        between threads, so a race-free parallel form of that order does
+        not exist) -- under "one_thread_lexicographic"."""
     sys.path.insert(0, str(ROOT / "oracle"))
-    from oracle import OracleRun
+    from oracle import OracleRun, set_threads
     from ntustcfdlab_amd.grid import bc_codes
 
-    def one(cap):
+    def one(cap, sor):
         r = OracleRun(grid, dt=w["dt"], nu=1.0 / w["Re"], Cs=w["Cs"], omega=w["omega"],
-                      zeta=w["zeta"], itmax=cap, LES=w["LES"], scheme=w["scheme"], sor=0,
+                      zeta=w["zeta"], itmax=cap, LES=w["LES"], scheme=w["scheme"], sor=sor,
                       bc=bc_codes())
         u, v, ww, p, eta = state
         r.f["u"][...], r.f["v"][...], r.f["w"][...], r.f["p"][...] = u, v, ww, p
@@ -79,17 +98,29 @@ def cpu_baseline(grid, w, state, sweeps_per_step):
         ik = r.step()[0]
         return time.perf_counter() - t0, ik
 
-    t_a, ik_a = one(4.0)
-    t_b, ik_b = one(12.0)
-    t_sweep = (t_b - t_a) / max(ik_b - ik_a, 1)
-    t_rest = max(t_a - ik_a * t_sweep, 0.0)
-    t_step = t_rest + sweeps_per_step * t_sweep
-    return {"value": grid.cells / t_step / 1e6, "unit": "Mcell-steps/s", "cores": 1,
-            "kind": "port",
-            "sample": (f"CPU restatement, 1 thread, one {grid.nx}^3 step at sweep caps 4 and 12 "
-                       f"({t_a + t_b:.1f} s): {t_sweep * 1e3:.1f} ms/SOR sweep (lexicographic), "
-                       f"{t_rest:.2f} s other kernels, extrapolated to "
-                       f"{sweeps_per_step:.0f} sweeps/step")}
+    def leg(sor, threads, caps):
+        used = set_threads(threads)
+        t_a, ik_a = one(float(caps[0]), sor)
+        t_b, ik_b = one(float(caps[1]), sor)
+        t_sweep = (t_b - t_a) / max(ik_b - ik_a, 1)
+        t_rest = max(t_a - ik_a * t_sweep, 0.0)
+        t_step = t_rest + sweeps_per_step * t_sweep
+        order = "lexicographic (GS.f90)" if sor == 0 else "red-black"
+        return {"value": grid.cells / t_step / 1e6, "unit": "Mcell-steps/s", "cores": used,
+                "kind": "port", "extrapolated": True,
+                "sample": (f"CPU restatement, {used} thread{'s' if used > 1 else ''}, one "
+                           f"{grid.nx}^3 step at sweep caps {caps[0]} and {caps[1]} "
+                           f"({t_a + t_b:.1f} s): {t_sweep * 1e3:.2f} ms/SOR sweep ({order}), "
+                           f"{t_rest:.2f} s other kernels, extrapolated to "
+                           f"{sweeps_per_step:.0f} sweeps/step")}
+
+    before = set_threads()
+    try:
+        rb = leg(1, cpu_threads(), (20, 200))
+        lex = leg(0, 1, (4, 12))
+    finally:
+        set_threads(before)
+    return {**rb, "one_thread_lexicographic": lex}
 
 
 def adjoint_workload(n=96):
@@ -478,6 +509,59 @@ def actuator_strip(g, aoa, seed):
     from ntustcfdlab_amd.configs import actuator_strip as strip
     return strip(g, aoa, seed)
 
+def marker_aoa(hip, steps, navg):
+    """dC_D/dAOA through the airfoil's Lagrangian markers (the marker direct
+    forcing replaces the cast ETA) on the 512x384x128 configuration, FP32
+    sweeps + FP64 refine: J = mean C_D over the last navg of `steps` steps
+    from the marker-forced state two steps after the inflow; the chain rule
+    sum_m dJ/dX_m . dX_m/dAOA against central differences of J at +-0.01
+    and +-0.005 degrees (marker model -- kernel widths, weights -- of the
+    base angle) and their Richardson extrapolation."""
+    from ntustcfdlab_amd import configs
+    from ntustcfdlab_amd.markers import airfoil_marker_set
+    g, bc, az0, ay0, aoa, zc, yc, prm = configs.airfoil_c5()
+    mk = airfoil_marker_set(g, az0, ay0, aoa, zc, yc, 0.01)
+    s = hip.ProjectionSolver(g, bc)
+    try:
+        configs.configure(s, prm, hip.SOR_MIXED)
+        eta = g.halo()
+        u, v, w, p = g.halo(), g.halo(), g.halo(), g.halo()
+        w[...] = 1.0
+        s.boundary_conditions(u, v, w, p)
+        s.set_markers(mk["zm"], mk["ym"], mk["wm"], mk["h"])
+        s.upload(u, v, w, p, eta)
+        s.step(2)
+        state = [g.halo() for _ in range(4)]
+        s.download(*state)
+
+        def J_at(a):
+            m = airfoil_marker_set(g, az0, ay0, a, zc, yc, 0.01, mk["h"], mk["hn"])
+            s.set_markers(m["zm"], m["ym"], m["wm"], m["h"])
+            s.upload(*[x.copy(order="F") for x in state], eta)
+            tot = 0.0
+            for k in range(steps):
+                _, _, cd, _, _ = s.step(1)
+                if k >= steps - navg:
+                    tot += float(cd[0])
+            return tot * (1.0 / navg)
+        s.set_markers(mk["zm"], mk["ym"], mk["wm"], mk["h"])
+        s.adjoint_objective(0.0, 1.0, 0.0, eta_gradient=True)
+        s.upload(*[x.copy(order="F") for x in state], eta)
+        J, ik, ms = s.adjoint_run(steps, steps, navg)
+        gz, gy = s.adjoint_marker_gradient()
+        chain = float(np.dot(gz, mk["dzm"]) + np.dot(gy, mk["dym"]))
+        fds = [(J_at(aoa + h) - J_at(aoa - h)) / (2 * h) for h in (0.01, 0.005)]
+    finally:
+        s.close()
+    rich = (4.0 * fds[1] - fds[0]) / 3.0
+    return {"markers": int(len(mk["zm"])), "kernel_widths": [float(x) for x in mk["h"]],
+            "J": J, "sweeps_per_step": float(np.mean(ik)),
+            "chain_rule_per_deg": chain, "fd_per_deg_0p01": fds[0], "fd_per_deg_0p005": fds[1],
+            "fd_richardson": rich,
+            "rel_err_fd_0p01": abs(chain - fds[0]) / abs(fds[0]) if fds[0] else None,
+            "rel_err_richardson": abs(chain - rich) / abs(rich) if rich else None}
+
+
 def airfoil_adjoint_leg(hip, hiptime, steps, every, navg, fd=True):
     """The adjoint of the large configuration (airfoil_case, FP32 sweeps +
     FP64 refine): J = mean C_D over the last navg of `steps` steps from the
@@ -495,7 +579,8 @@ def airfoil_adjoint_leg(hip, hiptime, steps, every, navg, fd=True):
     AOA: the force integral skips samples that are exactly zero
     (virtualForceIntegrator.f90:41-75), so a row that starts or stops
     touching the body makes it jump; the chain rule gives the derivative
-    between such jumps."""
+    between such jumps.  Under "aoa_markers", the same derivative with the
+    body forced by Lagrangian markers instead (marker_aoa), smooth in AOA."""
     g, bc, az0, ay0, configure = airfoil_case(hip)
     aoa, h_aoa = 8.0, 0.01
     s = hip.ProjectionSolver(g, bc)
@@ -630,6 +715,7 @@ def airfoil_adjoint_leg(hip, hiptime, steps, every, navg, fd=True):
         "aoa": {"step_deg": h_aoa, "eta_cells_changed": int((ep != em).sum()),
                 "chain_rule_per_deg": chain, "fd_per_deg": fd_a,
                 "rel_err": abs(fd_a - chain) / abs(fd_a) if fd_a else None},
+        "aoa_markers": marker_aoa(hip, 3, 3),
     }
 
 

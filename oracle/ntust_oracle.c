@@ -10,10 +10,20 @@
  * both associate * / + - left to right, and a leading unary minus that C
  * binds tighter than * yields the same bits because IEEE rounding is sign
  * symmetric.
+ *
+ * Loops whose cells are independent (the predictors, the LES gradients,
+ * the right-hand side, one colour of the red-black sweep, the projection)
+ * run over k with OpenMP, each cell computed exactly as in the serial loop;
+ * the only reduction is a maximum, so the results do not depend on the
+ * thread count.  GS.f90's lexicographic sweep and every floating-point sum
+ * stay serial.  GS.f90:36 itself parallelises its sweep with PRIVATE(i)
+ * only (pNew, pChange, mChange and both maxima shared, a race on the
+ * order of updates); the red-black restatement here is the race-free form.
  */
 #include "ntust_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -33,6 +43,13 @@
     const int nx = (g)->nx, ny = (g)->ny, nz = (g)->nz; \
     const int sx = nx + 4, sy = ny + 4;   \
     (void)nz; (void)sx; (void)sy
+
+/* threads of the OpenMP loops (n <= 0: leave as is); returns the count */
+int or_set_threads(int n)
+{
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+}
 
 /* ------------------------------------------------------------------ */
 /* boundary_conditions.f90                                             */
@@ -205,6 +222,7 @@ void or_gradient_phi_gauss(const or_grid *g, const double *Phi,
     GRID_LOCALS(g);
     const long nxy = (long)nx * ny;
     const size_t n3 = (size_t)nx * ny * nz;
+    #pragma omp parallel for schedule(static)
     for (int k = kstart; k <= kend; k++)
         for (int j = 1; j <= ny; j++)
             for (int i = 1; i <= nx; i++) {
@@ -237,6 +255,7 @@ void or_smag_filter_table(const or_grid *g, double Cs, double *tab)
 {
     GRID_LOCALS(g);
     const double third = (double)(1.0f / 3.0f);
+    #pragma omp parallel for schedule(static)
     for (int k = 1; k <= nz; k++)
         for (int j = 1; j <= ny; j++)
             for (int i = 1; i <= nx; i++) {
@@ -263,6 +282,7 @@ void or_smagorinsky(const or_grid *g, const double *u, const double *v,
     or_gradient_phi_gauss(g, v, pre_v, tv, g->iDx, g->Dys, g->iDz, kstart, kend);
     or_gradient_phi_gauss(g, w, pre_w, tw, g->iDx, g->iDy, g->Dzs, kstart, kend);
     or_smag_filter_table(g, Cs, tab);
+    #pragma omp parallel for schedule(static)
     for (int k = kstart; k <= kend; k++)
         for (int j = 1; j <= ny; j++)
             for (int i = 1; i <= nx; i++) {
@@ -311,6 +331,7 @@ void or_predictor_quick(const or_grid *g, const double *u, const double *v,
 #define W(a, b, c) w[H(a, b, c)]
 
     /* u_star, :7-117 */
+    #pragma omp parallel for private(i, j) schedule(static)
     for (k = kstart; k <= kend; k++)
         for (j = 1; j <= ny; j++)
             for (i = 1; i <= nx; i++) {
@@ -382,6 +403,7 @@ void or_predictor_quick(const or_grid *g, const double *u, const double *v,
             }
 
     /* v_star, :123-230 */
+    #pragma omp parallel for private(i, j) schedule(static)
     for (k = kstart; k <= kend; k++)
         for (j = 1; j <= ny; j++)
             for (i = 1; i <= nx; i++) {
@@ -453,6 +475,7 @@ void or_predictor_quick(const or_grid *g, const double *u, const double *v,
             }
 
     /* w_star, :236-342 */
+    #pragma omp parallel for private(i, j) schedule(static)
     for (k = kstart; k <= kend; k++)
         for (j = 1; j <= ny; j++)
             for (i = 1; i <= nx; i++) {
@@ -540,6 +563,7 @@ void or_predictor_upwind(const or_grid *g, const double *u, const double *v,
     int i, j, k;
 
     /* u_star, :8-93 */
+    #pragma omp parallel for private(i, j) schedule(static)
     for (k = 1; k <= nz; k++)
         for (j = 1; j <= ny; j++)
             for (i = 1; i <= nx - 1; i++) {
@@ -566,6 +590,7 @@ void or_predictor_upwind(const or_grid *g, const double *u, const double *v,
             }
 
     /* v_star, :99-184 */
+    #pragma omp parallel for private(i, j) schedule(static)
     for (k = 1; k <= nz; k++)
         for (j = 1; j <= ny - 1; j++)
             for (i = 1; i <= nx; i++) {
@@ -593,6 +618,7 @@ void or_predictor_upwind(const or_grid *g, const double *u, const double *v,
 
     /* w_star, :188-273.  The x-face advecting v of :194 averages
      * v(i,j,k) and v(i+1,j,k+1), as the reference writes it. */
+    #pragma omp parallel for private(i, j) schedule(static)
     for (k = 1; k <= nz - 1; k++)
         for (j = 1; j <= ny; j++)
             for (i = 1; i <= nx; i++) {
@@ -679,6 +705,7 @@ void or_poisson_b(const or_grid *g, const double *us, const double *vs,
                   const double *ws, double dt, double *b)
 {
     GRID_LOCALS(g);
+    #pragma omp parallel for schedule(static)
     for (int k = 1; k <= nz; k++)
         for (int j = 1; j <= ny; j++)
             for (int i = 1; i <= nx; i++) {
@@ -707,6 +734,7 @@ int or_sor_redblack(const or_grid *g, const double *b, double *p,
         ik++;
         double pChangeMax = 0.0;
         for (int colour = 0; colour < 2; colour++)
+            #pragma omp parallel for reduction(max : pChangeMax) schedule(static)
             for (int k = 1; k <= nz; k++)
                 for (int j = 1; j <= ny; j++)
                     for (int i = 1; i <= nx; i++) {
@@ -729,6 +757,7 @@ int or_sor_redblack(const or_grid *g, const double *b, double *p,
         pChangeMax_ = pChangeMax;
     }
     if (pre)
+        #pragma omp parallel for schedule(static)
         for (int k = 1; k <= nz; k++)
             for (int j = 1; j <= ny; j++)
                 for (int i = 1; i <= nx; i++) pre[I3(i, j, k)] = p[H(i, j, k)];
@@ -756,6 +785,7 @@ void or_calcul_new_velocity(const or_grid *g, const double *us,
     GRID_LOCALS(g);
     int i, j, k;
     /* :13-49 */
+    #pragma omp parallel for private(i, j) schedule(static)
     for (k = kstart; k <= kend; k++)
         for (j = 1; j <= ny; j++)
             for (i = 1; i <= nx; i++) {
@@ -765,6 +795,7 @@ void or_calcul_new_velocity(const or_grid *g, const double *us,
                 w1[o] = ws[o] - dt * (p[H(i, j, k + 1)] - p[o]) / M(g->Dzs, k);
             }
     /* :61-100 */
+    #pragma omp parallel for private(i, j) schedule(static)
     for (k = kstart; k <= kend; k++)
         for (j = 1; j <= ny; j++)
             for (i = 1; i <= nx; i++) {
@@ -778,6 +809,7 @@ void or_calcul_new_velocity(const or_grid *g, const double *us,
                 FZ[o] = (w2[o] - w1[o]) / dt;
             }
     /* :171-181 */
+    #pragma omp parallel for private(i, j) schedule(static)
     for (k = kstart; k <= kend; k++)
         for (j = 1; j <= ny; j++)
             for (i = 1; i <= nx; i++) {
@@ -793,6 +825,7 @@ void or_updating_velocity(const or_grid *g, const double *u2,
                           int kend, double *u, double *v, double *w)
 {
     GRID_LOCALS(g);
+    #pragma omp parallel for schedule(static)
     for (int k = kstart; k <= kend; k++)
         for (int j = 1; j <= ny; j++)
             for (int i = 1; i <= nx; i++) {
@@ -935,6 +968,7 @@ int or_step(const or_grid *g, const or_params *prm, or_fields *f,
                             f->us, f->vs, f->ws);
     if (f->fu) {
         const long sx = g->nx + 4, sxy = sx * (g->ny + 4);
+        #pragma omp parallel for schedule(static)
         for (int k = 1; k <= nz; k++)
             for (int j = 1; j <= g->ny; j++)
                 for (int i = 1; i <= g->nx; i++) {

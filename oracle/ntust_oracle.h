@@ -35,6 +35,10 @@ typedef struct {
     const double *iDx, *Dxs, *iDy, *Dys, *iDz, *Dzs; /* each (-1:n+2) */
 } or_grid;
 
+/* threads of the OpenMP loops (n <= 0: unchanged); returns the count.
+ * Results do not depend on it. */
+int or_set_threads(int n);
+
 /* boundary-condition codes, one per (face, component) in the order
  * W_u W_v W_w  E_u E_v E_w  S_u S_v S_w  N_u N_v N_w  B_u B_v B_w  F_u F_v F_w
  * value: 0 'no-slip', 1 'Neumann', 2 'Dirichlet', any other: no rule. */

@@ -49,6 +49,12 @@ _lib = _load()
 _lib.or_dynamic_aoa.restype = ct.c_double
 
 
+def set_threads(n: int = 0) -> int:
+    """Threads of the restatement's OpenMP loops (n <= 0: unchanged); the
+    count in force.  Results are the same at every count."""
+    return int(_lib.or_set_threads(int(n)))
+
+
 def dynamic_aoa(AOA1, time, dt, StartDynamic):
     """airfoil.f90:311-321"""
     return _lib.or_dynamic_aoa(ct.c_double(AOA1), ct.c_double(time), ct.c_double(dt),
