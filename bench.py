@@ -505,6 +505,71 @@ def airfoil_c3_leg(hip, hiptime, steps, warmup):
     return out
 
 
+def c3_series_leg(hip, hiptime, steps, chunk=100, every=10):
+    """The immersed-boundary configuration (configs.airfoil_c3: NACA 0012 at
+    5 degrees, 64x192x384, Re 1000, QUICK, dt 2e-4, ETA cast by the library)
+    run for `steps` steps from the inflow state with GS.f90's lexicographic
+    SOR (the CPU restatement's bits) and with the FP32 pattern inside FP64
+    defect correction: the C_D and C_L series of both (every `every`-th step
+    kept), their largest differences relative to the lexicographic series,
+    and the velocity of the final states against each other.  Runs in
+    chunks of `chunk` steps, one progress line each."""
+    from ntustcfdlab_amd import configs
+    g, bc, az0, ay0, aoa, zc, yc, prm = configs.airfoil_c3()
+    s = hip.ProjectionSolver(g, bc)
+    out = {"workload": (f"NACA 0012 at {aoa} deg, Re 1000, QUICK, dt 2e-4, 64x192x384 clustered, "
+                        f"inflow w = 1, slip side walls, ETA cast on the grid; {steps} steps from "
+                        f"the inflow state"), "steps": steps, "solvers": {}}
+    series, finals = {}, {}
+    try:
+        u, v, w, p = g.halo(), g.halo(), g.halo(), g.halo()
+        w[...] = 1.0
+        s.boundary_conditions(u, v, w, p)
+        eta = g.halo()
+        s.ray_casting(az0, ay0, aoa, zc, yc, ETA=eta)
+        for name, sor in (("lex", hip.SOR_LEXICOGRAPHIC), ("mixed", hip.SOR_MIXED)):
+            configs.configure(s, prm, sor)
+            s.set_arithmetic(hip.ARITH_FP32 if name == "mixed" else hip.ARITH_EXACT)
+            s.upload(u.copy(order="F"), v.copy(order="F"), w.copy(order="F"), p.copy(order="F"),
+                     eta)
+            cds, cls, iks, ms = [], [], [], 0.0
+            for c0 in range(0, steps, chunk):
+                n = min(chunk, steps - c0)
+                e0, e1 = hiptime.Event(), hiptime.Event()
+                hiptime.device_synchronize()
+                e0.record(s.stream())
+                ik, pm, cd, cl, rc = s.step(n)
+                e1.record(s.stream())
+                ms += e0.elapsed_ms(e1)
+                cds += [float(x) for x in cd]
+                cls += [float(x) for x in cl]
+                iks += [int(x) for x in ik]
+                print(f"#   C3 {name}: step {c0 + n}/{steps}, C_D {cds[-1]:.6f}, C_L {cls[-1]:.6f}, "
+                      f"{ms / (c0 + n):.1f} ms/step", file=sys.stderr, flush=True)
+            st = [g.halo() for _ in range(4)]
+            s.download(*st)
+            finals[name] = st
+            series[name] = (np.array(cds), np.array(cls))
+            out["solvers"][name] = {"ms_per_step": round(ms / steps, 3),
+                                    "value": round(g.cells * steps / (ms / 1e3) / 1e6, 3),
+                                    "sweeps_per_step": round(float(np.mean(iks)), 1),
+                                    "sweeps_first_last": [iks[0], iks[-1]],
+                                    "cD": cds[::every], "cL": cls[::every],
+                                    "cD_last": cds[-1], "cL_last": cls[-1]}
+    finally:
+        s.close()
+    (dl, ll), (dm, lm) = series["lex"], series["mixed"]
+    out["mixed_vs_oracle_max_rel_diff_cD"] = float(np.max(np.abs(dm - dl)) / np.max(np.abs(dl)))
+    out["mixed_vs_oracle_max_rel_diff_cL"] = float(np.max(np.abs(lm - ll)) / np.max(np.abs(ll)))
+    out["mixed_vs_oracle_last_rel_diff_cD"] = float(abs(dm[-1] - dl[-1]) / abs(dl[-1]))
+    out["mixed_vs_oracle_last_rel_diff_cL"] = float(abs(lm[-1] - ll[-1]) / abs(ll[-1]))
+    vel = max(np.abs(x).max() for x in finals["lex"][:3])
+    out["mixed_vs_oracle_final_rel_velocity_err"] = float(
+        max(np.abs(a - b).max() for a, b in zip(finals["mixed"][:3], finals["lex"][:3])) / vel)
+    out["series_every"] = every
+    return out
+
+
 def actuator_strip(g, aoa, seed):
     from ntustcfdlab_amd.configs import actuator_strip as strip
     return strip(g, aoa, seed)
@@ -562,7 +627,7 @@ def marker_aoa(hip, steps, navg):
             "rel_err_richardson": abs(chain - rich) / abs(rich) if rich else None}
 
 
-def airfoil_adjoint_leg(hip, hiptime, steps, every, navg, fd=True):
+def airfoil_adjoint_leg(hip, hiptime, steps, every, navg, fd=True, full_fd=True):
     """The adjoint of the large configuration (airfoil_case, FP32 sweeps +
     FP64 refine): J = mean C_D over the last navg of `steps` steps from the
     state two steps after the inflow, checkpoints every `every` steps (None:
@@ -580,7 +645,9 @@ def airfoil_adjoint_leg(hip, hiptime, steps, every, navg, fd=True):
     (virtualForceIntegrator.f90:41-75), so a row that starts or stops
     touching the body makes it jump; the chain rule gives the derivative
     between such jumps.  Under "aoa_markers", the same derivative with the
-    body forced by Lagrangian markers instead (marker_aoa), smooth in AOA."""
+    body forced by Lagrangian markers instead (marker_aoa), smooth in AOA.
+    full_fd = False keeps only the FP32-sweep forward's actuator difference
+    (for long horizons, where each FP64 forward costs minutes)."""
     g, bc, az0, ay0, configure = airfoil_case(hip)
     aoa, h_aoa = 8.0, 0.01
     s = hip.ProjectionSolver(g, bc)
@@ -681,6 +748,24 @@ def airfoil_adjoint_leg(hip, hiptime, steps, every, navg, fd=True):
         jp, _ = plain(eta0, [hm * x for x in d])
         jm, _ = plain(eta0, [-hm * x for x in d])
         fd_m = (jp - jm) / (2 * hm)
+        if not full_fd:
+            out = {"workload": (f"NACA 0012 at {aoa} deg, 512x384x128 clustered, Re 5000, LES Cs "
+                                f"0.1, dt 1e-4, FP32 sweeps + FP64 refine; J = mean C_D over the "
+                                f"last {navg} of {steps} steps from the state 2 steps after the "
+                                f"inflow; checkpoint every {every}"),
+                   "J": J, "J_plain_forward": J0, "sweeps_per_step": float(np.mean(ik)),
+                   **cost,
+                   "actuator": {"cells": ncells, "adjoint": ad,
+                                "fd_fp32_sweeps": {"step": hm, "fd": fd_m,
+                                                   "rel_err": abs(fd_m - ad) / abs(ad)
+                                                   if ad else None}}}
+    finally:
+        if not full_fd:
+            s.close()
+    if not full_fd:
+        out["aoa_markers"] = marker_aoa(hip, 3, 3)
+        return out
+    try:
         # and of the FP64 forward (the reference's arithmetic)
         configure(s, hip.SOR_REDBLACK)
         c0 = len(counts)
@@ -928,8 +1013,14 @@ def main():
                          "sqrt-spaced reverse pass for comparison)")
     ap.add_argument("--only-adjoint-256", action="store_true",
                     help="run the 256^3 cavity adjoint leg alone")
+    ap.add_argument("--only-c3-series", type=int, default=0, metavar="STEPS",
+                    help="only the airfoil configuration's C_D/C_L series over STEPS steps "
+                         "(lexicographic and FP32-pattern solvers)")
     ap.add_argument("--only-airfoil-c3", action="store_true",
                     help="run the moderate immersed-boundary airfoil leg alone")
+    ap.add_argument("--airfoil-adjoint-light", action="store_true",
+                    help="airfoil adjoint: only the FP32-sweep forward's actuator difference "
+                         "(no FP64 forwards, no cast-ETA angle differences)")
     ap.add_argument("--no-airfoil-adjoint-fd", action="store_true",
                     help="skip the finite differences of the large airfoil adjoint leg "
                          "(for long horizons)")
@@ -967,10 +1058,14 @@ def main():
     if a.only_airfoil_c3:
         print(json.dumps({"airfoil_c3_64x192x384": airfoil_c3_leg(hip, hiptime, 6, 2)}))
         return
+    if a.only_c3_series:
+        print(json.dumps({"airfoil_c3_series": c3_series_leg(hip, hiptime, a.only_c3_series)}))
+        return
     if a.only_airfoil_adjoint:
         print(json.dumps({"airfoil_adjoint_512x384x128":
                           airfoil_adjoint_leg(hip, hiptime, *aa_args,
-                                              fd=not a.no_airfoil_adjoint_fd)}))
+                                              fd=not a.no_airfoil_adjoint_fd,
+                                              full_fd=not a.airfoil_adjoint_light)}))
         return
 
     only = tuple(a.only.split(",")) if a.only else None

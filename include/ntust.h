@@ -425,7 +425,9 @@ int ntust_adjoint_eta_gradient(double *geta);
  * pass, reverse pass, its recomputation part, its adjoint part, in
  * milliseconds.  Afterwards the device holds the final forward state,
  * lambda the adjoint of the initial state and ntust_adjoint_gradient dJ/df.
- * Returns NTUST_E_STATE if a recomputed step's sweep count differs. */
+ * Returns NTUST_E_STATE if a recomputed step's sweep count differs.  With
+ * NTUST_PROGRESS set in the environment it writes one line per segment of
+ * each pass to stderr. */
 int ntust_adjoint_run(int nsteps, int checkpoint_every, int navg, double *J, int *ik,
                       double *ms);
 

@@ -2237,7 +2237,13 @@ int ntust_adjoint_run(int nsteps, int checkpoint_every, int navg, double *J, int
     // velocities are kept, so that segment needs no recomputation
     TRY(sync());
     auto t0 = now();
+    // progress lines on stderr, one per segment, when NTUST_PROGRESS is set
+    // in the environment (long horizons run for minutes inside this call)
+    const bool report = getenv("NTUST_PROGRESS") != nullptr;
     for (int n = 0; n < N; n++) {
+        if (report && n % K == 0 && n > 0)
+            fprintf(stderr, "ntust: adjoint forward, step %d of %d (%.1f s)\n", n, N,
+                    msec(t0, now()) / 1e3);
         if (n % K == 0) {
             double *d = slot(n / K);
             TRY(copy3(d, d + nh, d + 2 * nh, b.u, b.v, b.w));
@@ -2306,6 +2312,9 @@ int ntust_adjoint_run(int nsteps, int checkpoint_every, int navg, double *J, int
             TRY(sync());
             t_rec += msec(r0, now());
         }
+        if (report)
+            fprintf(stderr, "ntust: adjoint reverse, segment %d of %d (steps %d-%d, %.1f s)\n",
+                    nseg - sg, nseg, s0, s1 - 1, msec(t0, now()) / 1e3);
         auto a0 = now();
         for (int n = s1 - 1; n >= s0; n--) {
             if (n >= N - navg && wke != 0.0) {

@@ -12,6 +12,8 @@ module ntust_hip
    integer(c_int), parameter :: NTUST_SOR_LEXICOGRAPHIC = 0, NTUST_SOR_REDBLACK = 1, &
                                 NTUST_SOR_MIXED = 2
    integer(c_int), parameter :: NTUST_QUICK_CENTRE = 0, NTUST_UPWIND = 1
+   integer(c_int), parameter :: NTUST_ARITH_EXACT = 0, NTUST_ARITH_FAST = 1, &
+                                NTUST_ARITH_FP32 = 2
    integer(c_int), parameter :: NTUST_SWEEP_TWO_PASS = 0, NTUST_SWEEP_FUSED = 1, &
                                 NTUST_SWEEP_GRAPH = 2, NTUST_SWEEP_TILE = 3
 
@@ -353,6 +355,44 @@ module ntust_hip
          real(c_double), value :: reserve_bytes
          integer(c_int), intent(out) :: every
          type(c_ptr), value :: bytes
+      end function
+
+      ! predictor and eddy-viscosity arithmetic of the resident step
+      ! (NTUST_ARITH_*; no counterpart in the reference)
+      integer(c_int) function ntust_set_arithmetic(mode) bind(C, name="ntust_set_arithmetic")
+         import :: c_int
+         integer(c_int), value :: mode
+      end function
+
+      ! Lagrangian markers of a body at rest (no counterpart in the
+      ! reference): n markers at (zm, ym) with weights wm, kernel widths
+      ! hx, hy, hz; Xn, Yn, Zn the node coordinates X(-1:nx+3) etc.
+      integer(c_int) function ntust_set_markers(n, zm, ym, wm, hx, hy, hz, Xn, Yn, Zn) &
+            bind(C, name="ntust_set_markers")
+         import :: c_int, c_double
+         integer(c_int), value :: n
+         real(c_double), intent(in) :: zm(*), ym(*), wm(*), Xn(*), Yn(*), Zn(*)
+         real(c_double), value :: hx, hy, hz
+      end function
+
+      integer(c_int) function ntust_marker_velocity(U) bind(C, name="ntust_marker_velocity")
+         import :: c_int, c_double
+         real(c_double), intent(out) :: U(*)
+      end function
+
+      integer(c_int) function ntust_adjoint_marker_gradient(gz, gy) &
+            bind(C, name="ntust_adjoint_marker_gradient")
+         import :: c_int, c_double
+         real(c_double), intent(out) :: gz(*), gy(*)
+      end function
+
+      ! force integral on device arrays, allocating nothing (for callers
+      ! that cannot allocate: graph capture, serving loops)
+      integer(c_int) function ntust_virtual_force_integrator_noalloc(FY, FZ, lx, coef) &
+            bind(C, name="ntust_virtual_force_integrator_noalloc")
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: FY, FZ, coef
+         real(c_double), value :: lx
       end function
    end interface
 
