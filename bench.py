@@ -988,7 +988,7 @@ def main():
                          "1 GiB each way), for the counter profiles")
     ap.add_argument("--adjoint-size", type=int, default=96)
     ap.add_argument("--adjoint-steps", type=int, default=128)
-    ap.add_argument("--fd-directions", type=int, default=2)
+    ap.add_argument("--fd-directions", type=int, default=8)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-128", action="store_true", help="skip the 128^3 leg")
     ap.add_argument("--no-256", action="store_true", help="skip the 256^3 adjoint leg")
