@@ -906,13 +906,13 @@ int one_step(const StepBufs &b, bool force, bool centre, int *ik, double *pm, do
     }
     // main.f90:213
     if (q.scheme == NTUST_QUICK_CENTRE && C.arith == NTUST_ARITH_FAST)
-        launch_predict_quick_fast<double>(g, qtables<double>(), b.u, b.v, b.w,
+        launch_predict_quick_march<double>(g, qtables<double>(), b.u, b.v, b.w,
+                                           q.les ? b.vis : nullptr, q.dt, q.nu, q.les, b.us,
+                                           b.vs, b.ws, 1, C.nz, C.st);
+    else if (q.scheme == NTUST_QUICK_CENTRE && C.arith == NTUST_ARITH_FP32)
+        launch_predict_quick_march<float>(g, qtables<float>(), b.u, b.v, b.w,
                                           q.les ? b.vis : nullptr, q.dt, q.nu, q.les, b.us,
                                           b.vs, b.ws, 1, C.nz, C.st);
-    else if (q.scheme == NTUST_QUICK_CENTRE && C.arith == NTUST_ARITH_FP32)
-        launch_predict_quick_fast<float>(g, qtables<float>(), b.u, b.v, b.w,
-                                         q.les ? b.vis : nullptr, q.dt, q.nu, q.les, b.us, b.vs,
-                                         b.ws, 1, C.nz, C.st);
     else if (q.scheme == NTUST_QUICK_CENTRE)
         launch_predict_quick(g, b.u, b.v, b.w, q.les ? b.vis : nullptr, q.dt, q.nu, q.les, b.us,
                              b.vs, b.ws, 1, C.nz, C.st);

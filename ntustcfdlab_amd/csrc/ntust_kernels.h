@@ -143,10 +143,10 @@ struct QTables { const QCoef<T> *t[3][2]; };
 template <typename T>
 struct RMetric { const T *riDx, *rDxs, *riDy, *rDys, *riDz, *rDzs; };
 template <typename T>
-void launch_predict_quick_fast(const KGrid &g, const QTables<T> &q, const double *u,
-                               const double *v, const double *w, const double *visc, double dt,
-                               double nu, int les, double *us, double *vs, double *ws, int k0,
-                               int k1, hipStream_t st);
+void launch_predict_quick_march(const KGrid &g, const QTables<T> &q, const double *u,
+                                const double *v, const double *w, const double *visc, double dt,
+                                double nu, int les, double *us, double *vs, double *ws, int k0,
+                                int k1, hipStream_t st);
 template <typename T>
 void launch_smagorinsky_fast(const KGrid &g, const RMetric<T> &r, const double *u,
                              const double *v, const double *w, const double *filt,

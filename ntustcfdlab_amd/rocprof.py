@@ -47,7 +47,7 @@ STEP_KERNELS = (
     "k_rb_sweep32<0, 0>", "k_rb_sweep32<1, 0>", "k_rb_sweep<0, 2>", "k_rb_sweep<1, 2>",
     "k_rb_sweep<0, 1>", "k_rb_sweep<1, 1>", "k_rb_sweep<0, 0>", "k_rb_sweep<1, 0>",
     "k_lex_sweep<0, false>", "k_lex_sweep<1, false>", "k_lex_sweep<0, true>",
-    "k_lex_sweep<1, true>", "k_predict_quick", "k_predict_quick_fast<float>",
+    "k_lex_sweep<1, true>", "k_predict_quick", "k_predict_quick_march<float>",
     "k_smagorinsky", "k_smagorinsky_fast<float>", "k_project", "k_rb_rhs",
     "k_rb_correct<0, true>", "k_rb_correct<1, true>", "k_copy_shell", "k_force_z")
 
