@@ -570,6 +570,76 @@ def c3_series_leg(hip, hiptime, steps, chunk=100, every=10):
     return out
 
 
+def c2_series_leg(hip, hiptime, steps, size=128, chunk=50):
+    """The LES cavity (Re 1000, QUICK, Smagorinsky Cs 0.1, dt 5e-4, zeta
+    1e-10, itmax 3000, seeded U(+-1e-3) noise) on `size`^3 for `steps`
+    steps from the same state with GS.f90's lexicographic SOR in FP64 (the
+    CPU restatement's bits), red-black FP64 and the FP32 pattern (predictor
+    increment, nu_t and sweeps in FP32 inside FP64 defect correction).
+    Every `chunk` steps the states of the two red-black solvers are compared
+    with GS.f90's order: velocity and pressure, relative, max norm.  One
+    progress line per chunk."""
+    from ntustcfdlab_amd.grid import Grid, bc_codes
+    w = workload(size)
+    g = Grid(w["nx"], w["ny"], w["nz"])
+    s = hip.ProjectionSolver(g, bc_codes())
+    out = {"workload": (f"LES cavity Re 1000, {size}^3 uniform, QUICK, Smagorinsky Cs 0.1, "
+                        f"dt 5e-4, SOR omega 1.5, zeta 1e-10, itmax 3000, seeded U(+-1e-3) "
+                        f"noise; {steps} steps"), "steps": steps, "every": chunk,
+           "solvers": {}}
+    ref = {}
+    try:
+        def configure(sor):
+            s.configure(dt=w["dt"], nu=1.0 / w["Re"], Cs=w["Cs"], LES=w["LES"],
+                        scheme=w["scheme"], omega=w["omega"], zeta=w["zeta"],
+                        itmax=w["itmax"], sor=sor, lx=1.0, force_every=1)
+        configure(hip.SOR_REDBLACK)
+        state = initial_state(g, w, s)
+        for name, sor, arith in (("lex", hip.SOR_LEXICOGRAPHIC, hip.ARITH_EXACT),
+                                 ("fp64", hip.SOR_REDBLACK, hip.ARITH_EXACT),
+                                 ("mixed", hip.SOR_MIXED, hip.ARITH_FP32)):
+            configure(sor)
+            s.set_arithmetic(arith)
+            s.upload(*[x.copy(order="F") for x in state])
+            iks, ms, errs = [], 0.0, []
+            for c0 in range(0, steps, chunk):
+                n = min(chunk, steps - c0)
+                e0, e1 = hiptime.Event(), hiptime.Event()
+                hiptime.device_synchronize()
+                e0.record(s.stream())
+                ik, pm, cd, cl, rc = s.step(n)
+                e1.record(s.stream())
+                ms += e0.elapsed_ms(e1)
+                iks += [int(x) for x in ik]
+                st = [g.halo() for _ in range(4)]
+                s.download(*st)
+                at = c0 + n
+                if name == "lex":
+                    ref[at] = st
+                    msg = ""
+                else:
+                    r = ref[at]
+                    vel = max(np.abs(x).max() for x in r[:3])
+                    ev = max(np.abs(a - b).max() for a, b in zip(st[:3], r[:3])) / vel
+                    ep = float(np.abs(st[3] - r[3]).max() / np.abs(r[3]).max())
+                    errs.append([at, float(ev), ep])
+                    msg = f", velocity {ev:.2e}, p {ep:.2e} from GS.f90's order"
+                print(f"#   C2 {size}^3 {name}: step {at}/{steps}, {ms / at:.1f} ms/step{msg}",
+                      file=sys.stderr, flush=True)
+            rec = {"ms_per_step": round(ms / steps, 3),
+                   "value": round(g.cells * steps / (ms / 1e3) / 1e6, 3),
+                   "sweeps_per_step": round(float(np.mean(iks)), 1),
+                   "sweeps_first_last": [iks[0], iks[-1]]}
+            if errs:
+                rec["vs_oracle"] = errs
+                rec["max_rel_velocity_err"] = max(e[1] for e in errs)
+                rec["max_rel_pressure_err"] = max(e[2] for e in errs)
+            out["solvers"][name] = rec
+    finally:
+        s.close()
+    return out
+
+
 def actuator_strip(g, aoa, seed):
     from ntustcfdlab_amd.configs import actuator_strip as strip
     return strip(g, aoa, seed)
@@ -1013,6 +1083,9 @@ def main():
                          "sqrt-spaced reverse pass for comparison)")
     ap.add_argument("--only-adjoint-256", action="store_true",
                     help="run the 256^3 cavity adjoint leg alone")
+    ap.add_argument("--only-c2-series", type=int, default=0, metavar="STEPS",
+                    help="the 128^3 LES cavity over STEPS steps: the red-black solvers "
+                         "against GS.f90's order every 50 steps (own JSON line)")
     ap.add_argument("--only-c3-series", type=int, default=0, metavar="STEPS",
                     help="only the airfoil configuration's C_D/C_L series over STEPS steps "
                          "(lexicographic and FP32-pattern solvers)")
@@ -1060,6 +1133,10 @@ def main():
         return
     if a.only_c3_series:
         print(json.dumps({"airfoil_c3_series": c3_series_leg(hip, hiptime, a.only_c3_series)}))
+        return
+    if a.only_c2_series:
+        print(json.dumps({"les_cavity_128_series": c2_series_leg(hip, hiptime,
+                                                                 a.only_c2_series)}))
         return
     if a.only_airfoil_adjoint:
         print(json.dumps({"airfoil_adjoint_512x384x128":
