@@ -931,7 +931,7 @@ def timed_run(s, hip, hiptime, state, sor, steps, warmup):
 SOLVER_NAMES = ("fp64", "mixed", "lex")
 
 
-def measure(hip, hiptime, size, steps, warmup, sweep, only, calibrate):
+def measure(hip, hiptime, size, steps, warmup, only, calibrate):
     """The three solvers on the LES cavity of `size`^3 from the same state:
     red-black FP64 ("fp64"), the FP32 pattern -- predictor, eddy viscosity
     and SOR sweeps in FP32 inside FP64 defect correction -- ("mixed"), and
@@ -941,8 +941,6 @@ def measure(hip, hiptime, size, steps, warmup, sweep, only, calibrate):
     w = workload(size)
     g = Grid(w["nx"], w["ny"], w["nz"])
     s = hip.ProjectionSolver(g, bc_codes())
-    s.set_sweep_variant({"fused": hip.SWEEP_FUSED, "two-pass": hip.SWEEP_TWO_PASS,
-                         "graph": hip.SWEEP_GRAPH, "tile": hip.SWEEP_TILE}[sweep])
     if calibrate:
         s.prof_copy_calibration()
 
@@ -1068,10 +1066,6 @@ def main():
                     help="run the airfoil ray-casting leg alone and print its JSON")
     ap.add_argument("--only-krylov", action="store_true",
                     help="run the BiCGSTAB leg alone and print its JSON")
-    ap.add_argument("--sweep", choices=["two-pass", "graph", "fused", "tile"], default="two-pass",
-                    help="red-black sweep launches (bit-identical results): two-pass "
-                         "kernels from the stream, the same replayed from a HIP graph, "
-                         "the z-streaming one-launch sweep, or the 3-D-tile one")
     ap.add_argument("--only-adjoint", action="store_true",
                     help="run the adjoint leg alone and print its JSON")
     ap.add_argument("--airfoil-adjoint-steps", type=int, default=8,
@@ -1146,7 +1140,7 @@ def main():
         return
 
     only = tuple(a.only.split(",")) if a.only else None
-    head_run = measure(hip, hiptime, a.size, a.steps, a.warmup, a.sweep, only, a.calibrate)
+    head_run = measure(hip, hiptime, a.size, a.steps, a.warmup, only, a.calibrate)
     g, w, state, sweeps = head_run["grid"], head_run["workload"], head_run["state"], None
     res = head_run["summary"]
     head = res["head"]
@@ -1172,8 +1166,7 @@ def main():
     }
     if a.size != 128 and not a.no_128 and not a.only:
         print("# 128^3 LES cavity", file=sys.stderr, flush=True)
-        small = measure(hip, hiptime, 128, a.steps, min(a.warmup, 2), a.sweep, None,
-                        False)["summary"]
+        small = measure(hip, hiptime, 128, a.steps, min(a.warmup, 2), None, False)["summary"]
         out["les_cavity_128"] = {
             "workload": "the same LES cavity on uniform 128^3 (2.1 M cells)",
             "steps": a.steps, "warmup": min(a.warmup, 2), "solvers": small["solvers"],

@@ -49,18 +49,6 @@ enum {
     NTUST_SOR_MIXED = 2           /* red-black, FP32 sweeps inside FP64 defect correction */
 };
 
-/* red-black sweep kernels (ntust_set_sweep_variant); all give the same bits */
-enum {
-    NTUST_SWEEP_TWO_PASS = 0,     /* one launch per colour, in place, from the
-                                     stream (default) */
-    NTUST_SWEEP_FUSED = 1,        /* one launch per sweep on LDS tiles */
-    NTUST_SWEEP_GRAPH = 2,        /* the two-pass launches replayed from a captured
-                                     HIP graph, 32 sweeps (FP64) or one FP32
-                                     segment per replay */
-    NTUST_SWEEP_TILE = 3          /* one launch per sweep on 3-D tiles (red halo
-                                     recomputed in LDS, no plane loop) */
-};
-
 /* arithmetic of the resident step's QUICK predictor and eddy viscosity
  * (ntust_set_arithmetic); the pressure solve, projection, forcing and force
  * integral are unaffected */
@@ -480,9 +468,6 @@ int ntust_set_arithmetic(int mode);
  * memory counters against.  Allocates and frees the two buffers. */
 int ntust_prof_copy(double nbytes, double *ms);
 
-/* Selects the red-black sweep kernel (NTUST_SWEEP_*); not a reference
- * setting, the iterates are bit-identical either way. */
-int ntust_set_sweep_variant(int variant);
 
 #ifdef __cplusplus
 }

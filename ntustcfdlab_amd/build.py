@@ -21,7 +21,7 @@ LIB = PKG / "libntust_hip.so"
 ORACLE_DIR = ROOT / "oracle"
 
 HIP_SOURCES = ["k_les.hip", "k_predict.hip", "k_sor.hip", "k_sor_lex.hip", "k_sor_mixed.hip",
-               "k_sor_fused.hip", "k_sor_tile.hip", "k_project.hip", "k_fast.hip", "k_marker.hip", "k_output.hip", "k_ray.hip", "k_krylov.hip", "k_bc.hip",
+               "k_project.hip", "k_fast.hip", "k_marker.hip", "k_output.hip", "k_ray.hip", "k_krylov.hip", "k_bc.hip",
                "k_adjoint.hip", "ntust_api.hip"]
 
 # -ffp-contract=off: no fused multiply-add unless the source writes fma(),

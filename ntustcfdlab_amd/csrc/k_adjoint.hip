@@ -1078,7 +1078,7 @@ void launch_adj_pressure(const KGrid &g, const SplitGrid &s, const double *rx, c
         // the first segment's residual; each segment's end computes the next
         // one's (k_adj_seg_black, k_adj_seg_red)
         launch_rb_residual(g, s, rx, ry, rz, z0, z1, mx->bzero, mx->bzero, mx->r0, mx->r1,
-                           mx->e1, nullptr, nullptr, st);
+                           mx->e1, nullptr, st);
         for (long h0 = 0; h0 < nh; h0 += mx->seg) {
             const long h1 = h0 + mx->seg < nh ? h0 + mx->seg : nh;
             for (long h = h0; h < h1; h++) {

@@ -169,7 +169,7 @@ k_rb_sweep32(KGrid g, SplitGrid s, FMetric fm, float *__restrict__ e_own,
 //
 // Colour 1 is shifted: its correction is e1' + s1 with s1 the scaled
 // residual the r1 plane holds; its launch leaves s1 = r1/aP1 in that plane
-// and e1' = -s1 (also in e_copy when not null).  Colour 0 adds S(s1) to its
+// and e1' = -s1.  Colour 0 adds S(s1) to its
 // residual, read as -e1' from the e1 plane, and leaves e0 = 0.
 //
 // Thread layout of the sweeps (RB_KPT cells along k, other colour's W/E
@@ -181,7 +181,7 @@ k_rb_correct(KGrid g, SplitGrid s, const double *__restrict__ rx, const double *
              const double *__restrict__ rz, double *__restrict__ p_own,
              const double *__restrict__ p_oth, float *__restrict__ e_own,
              const float *__restrict__ e_oth, const double *__restrict__ b_own,
-             float *__restrict__ r_own, float *__restrict__ e_copy, const int *dirty)
+             float *__restrict__ r_own, const int *dirty)
 {
     constexpr int KPT = RB_KPT;
     // FLY: the other colour is formed as p + e; SSUM: the residual gains
@@ -297,7 +297,6 @@ k_rb_correct(KGrid g, SplitGrid s, const double *__restrict__ rx, const double *
                 const float sv = (float)(res / aP);
                 r_own[at[r]] = sv;
                 e_own[at[r]] = -sv;
-                if (e_copy) e_copy[at[r]] = -sv;
             } else {
                 if (ABSORB) e_own[at[r]] = 0.0f;
                 r_own[at[r]] = (float)res;
@@ -341,16 +340,16 @@ void launch_rb_sweep32_spec(const KGrid &g, const SplitGrid &s, const FMetric &f
 void launch_rb_residual(const KGrid &g, const SplitGrid &s, const double *rx, const double *ry,
                         const double *rz, const double *p0, const double *p1, const double *b0,
                         const double *b1, float *r0, float *r1, float *e1_shift,
-                        float *e1_copy, const int *dirty, hipStream_t st)
+                        const int *dirty, hipStream_t st)
 {
     const int nq = (g.nx + 1) / 2;
     dim3 blk(64, 4), grd((nq + 63) / 64, (g.ny + 3) / 4, (g.nz + RB_KPT - 1) / RB_KPT);
     // p is only read when ABSORB is false
     double *q0 = const_cast<double *>(p0), *q1 = const_cast<double *>(p1);
     hipLaunchKernelGGL((k_rb_correct<1, false>), grd, blk, 0, st, g, s, rx, ry, rz, q1, p0,
-                       e1_shift, nullptr, b1, r1, e1_copy, dirty);
+                       e1_shift, nullptr, b1, r1, dirty);
     hipLaunchKernelGGL((k_rb_correct<0, false>), grd, blk, 0, st, g, s, rx, ry, rz, q0, p1,
-                       nullptr, e1_shift, b0, r0, nullptr, dirty);
+                       nullptr, e1_shift, b0, r0, dirty);
 }
 
 // absorb + residual of a segment of the shifted sweeps (k_rb_correct,
@@ -363,9 +362,9 @@ void launch_rb_correct(const KGrid &g, const SplitGrid &s, const double *rx, con
     const int nq = (g.nx + 1) / 2;
     dim3 blk(64, 4), grd((nq + 63) / 64, (g.ny + 3) / 4, (g.nz + RB_KPT - 1) / RB_KPT);
     hipLaunchKernelGGL((k_rb_correct<1, true>), grd, blk, 0, st, g, s, rx, ry, rz, p1, p0,
-                       e1, e0, b1, r1, nullptr, dirty);
+                       e1, e0, b1, r1, dirty);
     hipLaunchKernelGGL((k_rb_correct<0, true>), grd, blk, 0, st, g, s, rx, ry, rz, p0, p1,
-                       e0, e1, b0, r0, nullptr, dirty);
+                       e0, e1, b0, r0, dirty);
 }
 
 }  // namespace ntust

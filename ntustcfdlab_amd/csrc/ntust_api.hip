@@ -101,13 +101,6 @@ struct Ctx {
     // previous solve's sweep count
     int lex_epoch = 0;
     int lex_hint = 0;
-    // red-black sweep kernel: one launch per sweep (fused) or per colour,
-    // the latter from the stream or replayed from a captured graph
-    int sweep_variant = NTUST_SWEEP_TWO_PASS;
-    struct Graph {
-        hipGraphExec_t exec = nullptr;
-        std::vector<char> key;   // bytes of the captured launch arguments
-    } graph64, graph32;
     // body force added to the predictor (device halo arrays, "body_*")
     bool body = false;
     double *fu = nullptr, *fv = nullptr, *fw = nullptr;
@@ -177,30 +170,6 @@ int d2h(double *h, const double *d, size_t n)
     } while (0)
 
 int sync() { HIPCHK(hipStreamSynchronize(C.st)); HIPCHK(hipGetLastError()); return 0; }
-
-// Captures the launches `body` makes on the library stream into an
-// executable graph, unless the graph already holds launches with the same
-// arguments (key).  The launches must read everything that changes between
-// replays from device memory.
-template <typename F>
-int capture(Ctx::Graph &G, const void *key, size_t klen, F body)
-{
-    if (G.exec && G.key.size() == klen && memcmp(G.key.data(), key, klen) == 0) return 0;
-    if (G.exec) {
-        HIPCHK(hipGraphExecDestroy(G.exec));
-        G.exec = nullptr;
-    }
-    hipGraph_t gr = nullptr;
-    HIPCHK(hipStreamBeginCapture(C.st, hipStreamCaptureModeThreadLocal));
-    body();
-    const hipError_t e = hipStreamEndCapture(C.st, &gr);
-    if (e != hipSuccess)
-        return fail(NTUST_E_HIP, "graph capture of the sweeps: %s", hipGetErrorString(e));
-    HIPCHK(hipGraphInstantiate(&G.exec, gr, nullptr, nullptr, 0));
-    HIPCHK(hipGraphDestroy(gr));
-    G.key.assign((const char *)key, (const char *)key + klen);
-    return 0;
-}
 
 // (Cs*delta)**2 with delta = (iDx*iDy*iDz)**(1./3.), smagorinsky.f90:20-22:
 // libm pow on the host, the exponent the single-precision literal 1./3.
@@ -435,32 +404,13 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
         return fail(NTUST_E_ARG, "unsupported SOR mode %d", mode);
 
     const size_t np = (size_t)C.sg.plane;
-    // one launch per sweep: the z-streaming or the 3-D-tile kernel, both
-    // reading one copy of the colour planes and writing the other
-    const bool tile = C.sweep_variant == NTUST_SWEEP_TILE;
-    const bool fused = C.sweep_variant == NTUST_SWEEP_FUSED || tile;
     NEED(P0, "sor_p0", np, true);
     NEED(P1, "sor_p1", np, true);
     NEED(B0, "sor_b0", np, true);
     NEED(B1, "sor_b1", np, true);
-    // the fused FP64 sweep reads one copy of the planes and writes the
-    // other: copy 0 = (P0, P1), copy 1 = (Q0, Q1); sweep t writes copy t&1
-    double *Q0 = nullptr, *Q1 = nullptr;
-    if (fused && mode == NTUST_SOR_REDBLACK) {
-        NEED(a, "sor_q0", np, true);
-        NEED(b, "sor_q1", np, true);
-        Q0 = a;
-        Q1 = b;
-    }
     const long tmax = (long)fmin(itmax, 1.0e8) > 0 ? (long)fmin(itmax, 1.0e8) : 0;
-    // graph replay runs whole replays: up to one replay of sweeps past tmax
-    // is launched (they do not run, but read the buckets before them)
-    const bool graphs = C.sweep_variant == NTUST_SWEEP_GRAPH;
-    const long per_replay = mode == NTUST_SOR_MIXED ? C.mix_segment : 32;
-    const size_t nslot = (size_t)slot_index(tmax + (graphs ? per_replay : 0) + 2, 0);
-    NEED(slots, "sor_slots", nslot, false);
+    NEED(slots, "sor_slots", (size_t)slot_index(tmax + 2, 0), false);
     launch_rb_gather(g, C.sg, p, P0, P1, C.st);
-    if (Q0) launch_rb_gather(g, C.sg, p, Q0, Q1, C.st);   // ghosts in both copies
     launch_rb_rhs(g, C.sg, us, vs, ws, dt, B0, B1, C.st);
     // sweep 0: 1.0 (pChangeMax_ before the first sweep), sweep 1: 0, stop
     // and sweep count cleared; each sweep clears the buckets of the next
@@ -477,16 +427,6 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
         NEED(r0d, "mix_r0", nf, true);
         NEED(r1d, "mix_r1", nf, true);
         float *E0 = (float *)e0d, *E1 = (float *)e1d, *R0 = (float *)r0d, *R1 = (float *)r1d;
-        // second copy of e for the fused sweeps (ghost entries stay zero)
-        float *F0 = nullptr, *F1 = nullptr;
-        if (fused) {
-            NEED(f0d, "mix_f0", nf, true);
-            NEED(f1d, "mix_f1", nf, true);
-            F0 = (float *)f0d;
-            F1 = (float *)f1d;
-            HIPCHK(hipMemsetAsync(F0, 0, np * sizeof(float), C.st));
-            HIPCHK(hipMemsetAsync(F1, 0, np * sizeof(float), C.st));
-        }
         const long seg = C.mix_segment;
         const long nseg = (tmax + seg - 1) / seg + 1;
         NEED(dd, "mix_dirty", (size_t)nseg, true);
@@ -494,13 +434,13 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
         HIPCHK(hipMemsetAsync(dirty, 0, nseg * sizeof(int), C.st));
         HIPCHK(hipMemsetAsync(E0, 0, np * sizeof(float), C.st));
         HIPCHK(hipMemsetAsync(E1, 0, np * sizeof(float), C.st));
-        // colour 1 runs shifted (-s1 in both copies of e for the fused sweeps)
-        launch_rb_residual(g, C.sg, rx, ry, rz, P0, P1, B0, B1, R0, R1, E1, F1, nullptr, C.st);
+        // colour 1 runs shifted (e1' = -s1)
+        launch_rb_residual(g, C.sg, rx, ry, rz, P0, P1, B0, B1, R0, R1, E1, nullptr, C.st);
         FMetric fm{C.fmetric[0] + 1, C.fmetric[1] + 1, C.fmetric[2] + 1,
                    C.fmetric[3] + 1, C.fmetric[4] + 1, C.fmetric[5] + 1};
         long c = 0;
         long want = C.ik_hint > seg ? C.ik_hint - seg / 2 : 2 * seg;
-        const long nb1 = !fused && !graphs && 1.0 > zeta ? spec_batch(tmax, seg) : 0;
+        const long nb1 = 1.0 > zeta ? spec_batch(tmax, seg) : 0;
         if (nb1 > 0) {
             // sweeps 1..upto in segments counted from sweep 1, each followed
             // by the FP64 correction, as the gated loop below runs them; all
@@ -536,7 +476,7 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
                 HIPCHK(hipMemsetAsync(E0, 0, np * sizeof(float), C.st));
                 HIPCHK(hipMemsetAsync(E1, 0, np * sizeof(float), C.st));
                 launch_rb_residual(g, C.sg, rx, ry, rz, P0, P1, B0, B1, R0, R1, E1, nullptr,
-                                   nullptr, C.st);
+                                   C.st);
                 return 0;
             };
             launch_slots_clear(slots, 1, (int)nb1 + 1, C.st);
@@ -587,75 +527,19 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
             }
             want = 2 * seg;
         }
-        if (graphs) {
-            // one segment of sweeps per replay, the sweep index kept on the
-            // device; one dirty flag, reset by the segment's first sweep
-            struct {   // 8-byte members only: no padding bytes in the key
-                long nx, ny, nz;
-                const void *e0, *e1, *r0, *r1, *fm0, *slots, *ctl, *dirty;
-                double omega, zeta, itmax;
-                long seg;
-            } key = {g.nx, g.ny, g.nz, E0, E1, R0, R1, fm.iDx, slots, C.ctl, dirty,
-                     omega, zeta, itmax, seg};
-            TRY(capture(C.graph32, &key, sizeof key, [&] {
-                for (long b = 0; b < seg; b++)
-                    launch_rb_sweep32(g, C.sg, fm, E0, E1, R0, R1, (float)omega, zeta, itmax,
-                                      0, slots, C.ctl, dirty, b == 0 ? 1 : 0, C.st);
-            }));
-            while (t <= tmax) {
-                const long nrep = want / seg > 1 ? want / seg : 1;
-                for (long r = 0; r < nrep; r++) {
-                    if (r == 0) HIPCHK(hipEventRecord(C.ev0, C.st));
-                    HIPCHK(hipGraphLaunch(C.graph32.exec, C.st));
-                    if (r == 0) HIPCHK(hipEventRecord(C.ev1, C.st));
-                    launch_rb_correct(g, C.sg, rx, ry, rz, P0, P1, E0, E1, B0, B1, R0, R1,
-                                      dirty, C.st);
-                }
-                HIPCHK(hipGetLastError());
-                HIPCHK(hipMemcpyAsync(&h, C.ctl, sizeof h, hipMemcpyDeviceToHost, C.st));
-                TRY(sync());
-                if (h.ik >= t + seg - 1) {   // every sweep of the timed replay ran
-                    float ms = 0.f;
-                    HIPCHK(hipEventElapsedTime(&ms, C.ev0, C.ev1));
-                    C.prof_ms += ms;
-                    C.prof_launches += 2 * seg;
-                }
-                if (h.stop) break;
-                t += nrep * seg;
-                want = 2 * seg;
-            }
-            t = tmax + 1;
-        }
         while (t <= tmax) {
             long queued = 0;
             bool timed = false;
             do {
                 const long n = seg < tmax - t + 1 ? seg : tmax - t + 1;
                 if (!timed) HIPCHK(hipEventRecord(C.ev0, C.st));
-                for (long b = 0; b < n; b++) {
-                    const long tt = t + b;
-                    if (fused) {
-                        const bool rd = (tt - 1) & 1, wr = tt & 1;
-                        (tile ? launch_rb_sweep32_tile : launch_rb_sweep32_fused)(
-                            g, C.sg, fm, rd ? F0 : E0, rd ? F1 : E1, wr ? F0 : E0,
-                            wr ? F1 : E1, R0, R1, (float)omega, zeta, itmax, (int)tt, slots,
-                            C.ctl, dirty + c, C.st);
-                    } else {
-                        launch_rb_sweep32(g, C.sg, fm, E0, E1, R0, R1, (float)omega, zeta,
-                                          itmax, (int)tt, slots, C.ctl, dirty + c,
-                                          b == 0 ? 1 : 0, C.st);
-                    }
-                }
+                for (long b = 0; b < n; b++)
+                    launch_rb_sweep32(g, C.sg, fm, E0, E1, R0, R1, (float)omega, zeta, itmax,
+                                      (int)(t + b), slots, C.ctl, dirty + c, b == 0 ? 1 : 0,
+                                      C.st);
                 if (!timed) HIPCHK(hipEventRecord(C.ev1, C.st));
-                if (fused) {
-                    launch_rb_absorb_pp(g, C.sg, P0, P1, E0, E1, F0, F1, R1, dirty + c, C.ctl,
-                                        C.st);
-                    launch_rb_residual(g, C.sg, rx, ry, rz, P0, P1, B0, B1, R0, R1, E1, F1,
-                                       dirty + c, C.st);
-                } else {
-                    launch_rb_correct(g, C.sg, rx, ry, rz, P0, P1, E0, E1, B0, B1, R0, R1,
-                                      dirty + c, C.st);
-                }
+                launch_rb_correct(g, C.sg, rx, ry, rz, P0, P1, E0, E1, B0, B1, R0, R1,
+                                  dirty + c, C.st);
                 HIPCHK(hipGetLastError());
                 if (!timed) { timed = true; batch = n; }
                 t += n;
@@ -668,14 +552,14 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
                 float ms = 0.f;
                 HIPCHK(hipEventElapsedTime(&ms, C.ev0, C.ev1));
                 C.prof_ms += ms;
-                C.prof_launches += fused ? batch : 2 * batch;
+                C.prof_launches += 2 * batch;
             }
             if (h.stop) break;
             want = 2 * seg;
         }
         t = tmax + 1;   // the red-black FP64 loop below does not run
     }
-    const long nb64 = !fused && !graphs && mode == NTUST_SOR_REDBLACK && 1.0 > zeta
+    const long nb64 = mode == NTUST_SOR_REDBLACK && 1.0 > zeta
                           ? spec_batch(tmax, 8)
                           : 0;
     if (nb64 > 0) {
@@ -735,56 +619,12 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
         }
         batch = 16;
     }
-    if (graphs && t <= tmax) {
-        // per_replay sweeps per replay, the sweep index kept on the device
-        struct {   // 8-byte members only: no padding bytes in the key
-            long nx, ny, nz;
-            const void *p0, *p1, *b0, *b1, *rx, *slots, *ctl;
-            double omega, zeta, itmax;
-            long n;
-        } key = {g.nx, g.ny, g.nz, P0, P1, B0, B1, rx, slots, C.ctl, omega, zeta, itmax,
-                 per_replay};
-        TRY(capture(C.graph64, &key, sizeof key, [&] {
-            for (long b = 0; b < per_replay; b++)
-                launch_rb_sweep(g, C.sg, rx, ry, rz, P0, P1, B0, B1, omega, zeta, itmax, 0,
-                                slots, C.ctl, C.st);
-        }));
-        long want = C.ik_hint > per_replay ? C.ik_hint - per_replay / 2 : per_replay;
-        while (t <= tmax) {
-            const long nrep = want / per_replay > 1 ? want / per_replay : 1;
-            HIPCHK(hipEventRecord(C.ev0, C.st));
-            HIPCHK(hipGraphLaunch(C.graph64.exec, C.st));
-            HIPCHK(hipEventRecord(C.ev1, C.st));
-            for (long r = 1; r < nrep; r++) HIPCHK(hipGraphLaunch(C.graph64.exec, C.st));
-            HIPCHK(hipMemcpyAsync(&h, C.ctl, sizeof h, hipMemcpyDeviceToHost, C.st));
-            TRY(sync());
-            if (h.ik >= t + per_replay - 1) {   // every sweep of the timed replay ran
-                float ms = 0.f;
-                HIPCHK(hipEventElapsedTime(&ms, C.ev0, C.ev1));
-                C.prof_ms += ms;
-                C.prof_launches += 2 * per_replay;
-            }
-            if (h.stop) break;
-            t += nrep * per_replay;
-            want = 2 * per_replay;
-        }
-        t = tmax + 1;
-    }
     while (t <= tmax) {
         const long nb = batch < tmax - t + 1 ? batch : tmax - t + 1;
         HIPCHK(hipEventRecord(C.ev0, C.st));
-        for (long b = 0; b < nb; b++) {
-            const long tt = t + b;
-            if (fused) {
-                const bool rd = (tt - 1) & 1, wr = tt & 1;
-                (tile ? launch_rb_sweep_tile : launch_rb_sweep_fused)(
-                    g, C.sg, rx, ry, rz, rd ? Q0 : P0, rd ? Q1 : P1, wr ? Q0 : P0,
-                    wr ? Q1 : P1, B0, B1, omega, zeta, itmax, (int)tt, slots, C.ctl, C.st);
-            } else {
-                launch_rb_sweep(g, C.sg, rx, ry, rz, P0, P1, B0, B1, omega, zeta, itmax,
-                                (int)tt, slots, C.ctl, C.st);
-            }
-        }
+        for (long b = 0; b < nb; b++)
+            launch_rb_sweep(g, C.sg, rx, ry, rz, P0, P1, B0, B1, omega, zeta, itmax,
+                            (int)(t + b), slots, C.ctl, C.st);
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(C.ev1, C.st));
         HIPCHK(hipMemcpyAsync(&h, C.ctl, sizeof h, hipMemcpyDeviceToHost, C.st));
@@ -793,7 +633,7 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
             float ms = 0.f;
             HIPCHK(hipEventElapsedTime(&ms, C.ev0, C.ev1));
             C.prof_ms += ms;
-            C.prof_launches += fused ? nb : 2 * nb;
+            C.prof_launches += 2 * nb;
         }
         if (h.stop) break;
         t += nb;
@@ -803,9 +643,7 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
     double bk[SLOT_BUCKETS * SLOT_STRIDE];
     HIPCHK(hipMemcpyAsync(bk, slots + slot_index(h.ik, 0), sizeof bk, hipMemcpyDeviceToHost,
                           C.st));
-    // the fused FP64 sweeps leave the result in copy ik&1
-    const bool in_q = Q0 && (h.ik & 1);
-    launch_rb_scatter(g, C.sg, in_q ? Q0 : P0, in_q ? Q1 : P1, p, pre, C.st);
+    launch_rb_scatter(g, C.sg, P0, P1, p, pre, C.st);
     HIPCHK(hipGetLastError());
     TRY(sync());
     double m = 0.0;
@@ -1159,8 +997,6 @@ int ntust_finalize(void)
     hipError_t first = hipSuccess;
     auto keep = [&first](hipError_t e) { if (first == hipSuccess) first = e; };
     if (C.st) keep(hipStreamSynchronize(C.st));
-    if (C.graph64.exec) keep(hipGraphExecDestroy(C.graph64.exec));
-    if (C.graph32.exec) keep(hipGraphExecDestroy(C.graph32.exec));
     for (auto &b : C.bufs) keep(hipFree(b.ptr));
     C.bufs.clear();
     for (int m = 0; m < 6; m++)
@@ -2363,15 +2199,6 @@ int ntust_set_arithmetic(int mode)
     return 0;
 }
 
-int ntust_set_sweep_variant(int variant)
-{
-    if (variant != NTUST_SWEEP_TWO_PASS && variant != NTUST_SWEEP_FUSED &&
-        variant != NTUST_SWEEP_GRAPH && variant != NTUST_SWEEP_TILE)
-        return fail(NTUST_E_ARG, "unknown sweep variant %d", variant);
-    C.sweep_variant = variant;
-    return 0;
-}
-
 int ntust_prof_kernel(double *bytes_per_cell)
 {
     // algorithmic bytes per grid cell of one timed launch: per sweep, read
@@ -2389,11 +2216,8 @@ int ntust_prof_kernel(double *bytes_per_cell)
         return 6;
     }
     const bool mixed = C.prm.sor == NTUST_SOR_MIXED;
-    const bool tile = C.sweep_variant == NTUST_SWEEP_TILE;
-    const bool fused = C.sweep_variant == NTUST_SWEEP_FUSED || tile;
-    if (bytes_per_cell) *bytes_per_cell = (mixed ? 10.0 : 24.0) / (fused ? 1.0 : 2.0);
-    if (tile) return mixed ? 5 : 4;
-    return mixed ? (fused ? 3 : 2) : (fused ? 1 : 0);
+    if (bytes_per_cell) *bytes_per_cell = (mixed ? 10.0 : 24.0) / 2.0;
+    return mixed ? 2 : 0;
 }
 
 int ntust_prof_read(double *ms, long long *launches)

@@ -246,44 +246,16 @@ void launch_rb_sweep_spec(const KGrid &g, const SplitGrid &s, const double *rx, 
                           const double *rz, double *p0, double *p1, const double *b0,
                           const double *b1, double omega, int t, double *slots, double zeta,
                           int *flags, hipStream_t st);
-// one launch per sweep on 3-D tiles (k_sor_tile.hip); arguments as the
-// z-streaming fused sweeps
-void launch_rb_sweep_tile(const KGrid &g, const SplitGrid &s, const double *rx, const double *ry,
-                          const double *rz, const double *in0, const double *in1, double *out0,
-                          double *out1, const double *b0, const double *b1, double omega,
-                          double zeta, double itmax, int t, double *slots, SorControl *ctl,
-                          hipStream_t st);
-void launch_rb_sweep32_tile(const KGrid &g, const SplitGrid &s, const FMetric &fm,
-                            const float *in0, const float *in1, float *out0, float *out1,
-                            const float *r0, const float *r1, float omega, double zeta,
-                            double itmax, int t, double *slots, SorControl *ctl, int *dirty,
-                            hipStream_t st);
 void launch_rb_sweep32_spec(const KGrid &g, const SplitGrid &s, const FMetric &fm, float *e0,
                             float *e1, const float *r0, const float *r1, float omega, int t,
                             double *slots, double zeta, int *flags, hipStream_t st);
 void launch_slots_clear(double *slots, int t0, int n, hipStream_t st);
-// one launch per sweep (red then black on LDS tiles), reading copy "in" of
-// the colour planes and writing copy "out"
-void launch_rb_sweep_fused(const KGrid &g, const SplitGrid &s, const double *rx,
-                           const double *ry, const double *rz, const double *in0,
-                           const double *in1, double *out0, double *out1, const double *b0,
-                           const double *b1, double omega, double zeta, double itmax, int t,
-                           double *slots, SorControl *ctl, hipStream_t st);
-void launch_rb_sweep32_fused(const KGrid &g, const SplitGrid &s, const FMetric &fm,
-                             const float *in0, const float *in1, float *out0, float *out1,
-                             const float *r0, const float *r1, float omega, double zeta,
-                             double itmax, int t, double *slots, SorControl *ctl, int *dirty,
-                             hipStream_t st);
-void launch_rb_absorb_pp(const KGrid &g, const SplitGrid &s, double *p0, double *p1,
-                         float *ea0, float *ea1, float *eb0, float *eb1, const float *s1,
-                         const int *dirty, const SorControl *ctl, hipStream_t st);
 // the residual for the shifted colour-1 sweeps: r1 holds s1 = r1/aP1,
-// e1_shift (and e1_copy, the second copy of the fused sweeps, when not
-// null) = -s1, r0 gains S(s1)
+// e1_shift = -s1, r0 gains S(s1)
 void launch_rb_residual(const KGrid &g, const SplitGrid &s, const double *rx, const double *ry,
                         const double *rz, const double *p0, const double *p1, const double *b0,
                         const double *b1, float *r0, float *r1, float *e1_shift,
-                        float *e1_copy, const int *dirty, hipStream_t st);
+                        const int *dirty, hipStream_t st);
 // absorb followed by the residual, in two colour launches (same bits),
 // for the shifted colour-1 sweeps
 void launch_rb_correct(const KGrid &g, const SplitGrid &s, const double *rx, const double *ry,

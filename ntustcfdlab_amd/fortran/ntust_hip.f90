@@ -14,8 +14,6 @@ module ntust_hip
    integer(c_int), parameter :: NTUST_QUICK_CENTRE = 0, NTUST_UPWIND = 1
    integer(c_int), parameter :: NTUST_ARITH_EXACT = 0, NTUST_ARITH_FAST = 1, &
                                 NTUST_ARITH_FP32 = 2
-   integer(c_int), parameter :: NTUST_SWEEP_TWO_PASS = 0, NTUST_SWEEP_FUSED = 1, &
-                                NTUST_SWEEP_GRAPH = 2, NTUST_SWEEP_TILE = 3
 
    interface
       integer(c_int) function ntust_init(nx, ny, nz, iDx, Dxs, iDy, Dys, iDz, Dzs) &
@@ -150,12 +148,6 @@ module ntust_hip
          integer(c_int), value :: nsteps
          integer(c_int), intent(out) :: ik(*)
          real(c_double), intent(out) :: pchange(*), cdrag(*), clift(*)
-      end function
-
-      integer(c_int) function ntust_set_sweep_variant(variant) &
-            bind(C, name="ntust_set_sweep_variant")
-         import :: c_int
-         integer(c_int), value :: variant
       end function
 
       ! airfoil geometry: ETA is type(c_ptr) (c_loc(ETA), or c_null_ptr for

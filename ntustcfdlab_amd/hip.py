@@ -28,13 +28,8 @@ ARITH_EXACT, ARITH_FAST, ARITH_FP32 = 0, 1, 2
 WARN_ITMAX = 1
 E_STATE = -5
 E_IO = -6
-SWEEP_TWO_PASS, SWEEP_FUSED, SWEEP_GRAPH, SWEEP_TILE = 0, 1, 2, 3
 PROF_KERNELS = {0: ("k_rb_sweep<C>", "two-pass FP64 half-sweep"),
-                1: ("k_rb_sweep_fused<double>", "fused FP64 sweep"),
                 2: ("k_rb_sweep32<C>", "two-pass FP32 half-sweep"),
-                3: ("k_rb_sweep_fused<float>", "fused FP32 sweep"),
-                4: ("k_rb_sweep_tile<double>", "3-D-tile FP64 sweep"),
-                5: ("k_rb_sweep_tile<float>", "3-D-tile FP32 sweep"),
                 6: ("k_lex_sweep<C>", "lexicographic FP64 step: one colour, pipelined sweeps")}
 
 _D = ct.POINTER(ct.c_double)
@@ -95,7 +90,6 @@ def load_library() -> ct.CDLL:
         "ntust_adjoint_objective": [d, d, d, i],
         "ntust_adjoint_eta_gradient": [_D],
         "ntust_adjoint_checkpoint_plan": [i, d, _I, _D],
-        "ntust_set_sweep_variant": [i],
         "ntust_poisson_coeffs": [_D, _I],
         "ntust_poisson_rhs": [_D, _D, _D, d, _D],
         "ntust_pressure_field": [_D, _D, _D],
@@ -292,11 +286,6 @@ class ProjectionSolver:
 
     def prof_reset(self):
         self.lib.ntust_prof_reset()
-
-    def set_sweep_variant(self, variant: int):
-        """SWEEP_TWO_PASS (default), SWEEP_GRAPH, SWEEP_FUSED or SWEEP_TILE; the
-        iterates are bit-identical."""
-        _check(self.lib.ntust_set_sweep_variant(int(variant)))
 
     def prof_kernel(self):
         """(kernel id, algorithmic bytes per grid cell of one timed launch)."""
