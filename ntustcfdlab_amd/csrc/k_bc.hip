@@ -162,31 +162,65 @@ void launch_copy_full(long n, const double *src, double *dst0, double *dst1, dou
 // Device-resident step: the whole-array copies only matter outside the
 // interior box 1..n of each axis (u* and u2 are rewritten inside it before
 // anything reads them), so copy that shell of u, v, w into the u* and u2
-// arrays.  Thread per (i, j) column; interior columns copy k in {-1,0,n+1,n+2}.
+// arrays: the four end planes k = -1, 0, nz+1, nz+2 whole, and in every
+// plane k = 1..nz the frame of columns outside 1..nx x 1..ny (rows j = -1,
+// 0, ny+1, ny+2 whole, then columns i = -1, 0, nx+1, nx+2 of rows 1..ny).
+// One thread per shell cell.
+__device__ __forceinline__ void copy6(const KGrid &g, int i, int j, int k, const double *u,
+                                      const double *v, const double *w, double *a0, double *a1,
+                                      double *a2, double *b0, double *b1, double *b2)
+{
+    const long o = hix(g, i, j, k);
+    const double x = u[o], y = v[o], z = w[o];
+    a0[o] = x; a1[o] = y; a2[o] = z;
+    b0[o] = x; b1[o] = y; b2[o] = z;
+}
+
 __global__ void __launch_bounds__(256)
-k_copy_shell(KGrid g, const double *__restrict__ u, const double *__restrict__ v,
+k_copy_caps(KGrid g, const double *__restrict__ u, const double *__restrict__ v,
+            const double *__restrict__ w, double *a0, double *a1, double *a2, double *b0,
+            double *b1, double *b2)
+{
+    const int ex = g.nx + 4;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)ex * (g.ny + 4)) return;
+    const int i = (int)(t % ex) - 1, j = (int)(t / ex) - 1;
+    const int c = blockIdx.y;
+    const int k = c < 2 ? c - 1 : g.nz + c - 1;
+    copy6(g, i, j, k, u, v, w, a0, a1, a2, b0, b1, b2);
+}
+
+__global__ void __launch_bounds__(256)
+k_copy_frame(KGrid g, const double *__restrict__ u, const double *__restrict__ v,
              const double *__restrict__ w, double *a0, double *a1, double *a2, double *b0,
              double *b1, double *b2)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x - 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y - 1;
-    if (i > g.nx + 2 || j > g.ny + 2) return;
-    const bool inner = (i >= 1 && i <= g.nx && j >= 1 && j <= g.ny);
-    for (int k = -1; k <= g.nz + 2; k++) {
-        if (inner && k == 1) k = g.nz + 1;
-        const long o = hix(g, i, j, k);
-        const double x = u[o], y = v[o], z = w[o];
-        a0[o] = x; a1[o] = y; a2[o] = z;
-        b0[o] = x; b1[o] = y; b2[o] = z;
+    const int ex = g.nx + 4;
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= 4 * ex + 4 * g.ny) return;
+    int i, j;
+    if (f < 4 * ex) {
+        const int r = f / ex;
+        j = r < 2 ? r - 1 : g.ny + r - 1;
+        i = f % ex - 1;
+    } else {
+        const int h = f - 4 * ex, c = h % 4;
+        j = h / 4 + 1;
+        i = c < 2 ? c - 1 : g.nx + c - 1;
     }
+    copy6(g, i, j, (int)blockIdx.y + 1, u, v, w, a0, a1, a2, b0, b1, b2);
 }
 
 void launch_copy_shell(const KGrid &g, const double *u, const double *v, const double *w,
                        double *a0, double *a1, double *a2, double *b0, double *b1,
                        double *b2, hipStream_t st)
 {
-    dim3 blk(64, 4), grd((g.nx + 4 + 63) / 64, (g.ny + 4 + 3) / 4);
-    hipLaunchKernelGGL(k_copy_shell, grd, blk, 0, st, g, u, v, w, a0, a1, a2, b0, b1, b2);
+    const long plane = (long)(g.nx + 4) * (g.ny + 4);
+    const int frame = 4 * (g.nx + 4) + 4 * g.ny;
+    hipLaunchKernelGGL(k_copy_caps, dim3((unsigned)((plane + 255) / 256), 4), dim3(256), 0, st,
+                       g, u, v, w, a0, a1, a2, b0, b1, b2);
+    hipLaunchKernelGGL(k_copy_frame, dim3((unsigned)((frame + 255) / 256), g.nz), dim3(256), 0,
+                       st, g, u, v, w, a0, a1, a2, b0, b1, b2);
 }
 
 }  // namespace ntust

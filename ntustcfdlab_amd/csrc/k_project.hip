@@ -125,7 +125,12 @@ void launch_pack_interior(const KGrid &g, const double *a, double *x, hipStream_
 // ---- virtualForceIntegrator ------------------------------------------
 
 // scratch layouts: FXz/FYz (nx+2)*(ny+2) indexed (i, j) for i,j in 0..n+1,
-// FXy/FYy (nx+2); the border entries stay zero.
+// FXy/FYy (nx+2); the border entries stay zero.  Each sum keeps the
+// reference's order; the loads of the next FORCE_CHUNK samples are issued
+// together ahead of the sequential additions, so a line's sum pays one
+// memory latency per chunk instead of one per sample.
+constexpr int FORCE_CHUNK = 16;
+
 __global__ void __launch_bounds__(256)
 k_force_z(KGrid g, const double *__restrict__ FY, const double *__restrict__ FZ,
           double *__restrict__ FXz, double *__restrict__ FYz)
@@ -134,17 +139,30 @@ k_force_z(KGrid g, const double *__restrict__ FY, const double *__restrict__ FZ,
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
     if (i > g.nx || j > g.ny) return;
     const double *Dzs = g.Dzs;
-    long o = hix(g, i, j, 0);
+    const long o = hix(g, i, j, 0);
     double zm = FZ[o], z0 = FZ[o + g.sxy];
     double ym = FY[o], y0 = FY[o + g.sxy];
     double fx = 0.0, fy = 0.0;
-    for (int k = 1; k <= g.nz; k++) {
-        o += g.sxy;
-        const double zp = FZ[o + g.sxy], yp = FY[o + g.sxy];
-        fx = fx + (zm * Dzs[k - 1] + 4.0 * z0 * Dzs[k] + zp * Dzs[k + 1]) / 6.0;
-        fy = fy + (ym * Dzs[k - 1] + 4.0 * y0 * Dzs[k] + yp * Dzs[k + 1]) / 6.0;
-        zm = z0; z0 = zp;
-        ym = y0; y0 = yp;
+    for (int k0 = 1; k0 <= g.nz; k0 += FORCE_CHUNK) {
+        // samples k0+1 .. k0+FORCE_CHUNK (plane nz+1 is the last one read)
+        double zc[FORCE_CHUNK], yc[FORCE_CHUNK];
+#pragma unroll
+        for (int c = 0; c < FORCE_CHUNK; c++) {
+            const long oc = o + g.sxy * (long)min(k0 + 1 + c, g.nz + 1);
+            zc[c] = FZ[oc];
+            yc[c] = FY[oc];
+        }
+#pragma unroll
+        for (int c = 0; c < FORCE_CHUNK; c++) {
+            const int k = k0 + c;
+            if (k <= g.nz) {
+                const double zp = zc[c], yp = yc[c];
+                fx = fx + (zm * Dzs[k - 1] + 4.0 * z0 * Dzs[k] + zp * Dzs[k + 1]) / 6.0;
+                fy = fy + (ym * Dzs[k - 1] + 4.0 * y0 * Dzs[k] + yp * Dzs[k + 1]) / 6.0;
+                zm = z0; z0 = zp;
+                ym = y0; y0 = yp;
+            }
+        }
     }
     const long s = (long)i + (long)(g.nx + 2) * j;
     FXz[s] = fx;
@@ -160,11 +178,26 @@ k_force_y(KGrid g, const double *__restrict__ FXz, const double *__restrict__ FY
     const long px = g.nx + 2;
     const double *Dys = g.Dys;
     double fx = 0.0, fy = 0.0;
-    for (int j = 1; j <= g.ny; j++) {
-        const double xm = FXz[i + px * (j - 1)], x0 = FXz[i + px * j], xp = FXz[i + px * (j + 1)];
-        const double ym = FYz[i + px * (j - 1)], y0 = FYz[i + px * j], yp = FYz[i + px * (j + 1)];
-        if (x0 != 0.0) fx = fx + (xm * Dys[j - 1] + 4.0 * x0 * Dys[j] + xp * Dys[j + 1]) / 6.0;
-        if (y0 != 0.0) fy = fy + (ym * Dys[j - 1] + 4.0 * y0 * Dys[j] + yp * Dys[j + 1]) / 6.0;
+    double xm = FXz[i], x0 = FXz[i + px], ym = FYz[i], y0 = FYz[i + px];
+    for (int j0 = 1; j0 <= g.ny; j0 += FORCE_CHUNK) {
+        double xc[FORCE_CHUNK], yc[FORCE_CHUNK];
+#pragma unroll
+        for (int c = 0; c < FORCE_CHUNK; c++) {
+            const long q = i + px * (long)min(j0 + 1 + c, g.ny + 1);
+            xc[c] = FXz[q];
+            yc[c] = FYz[q];
+        }
+#pragma unroll
+        for (int c = 0; c < FORCE_CHUNK; c++) {
+            const int j = j0 + c;
+            if (j <= g.ny) {
+                const double xp = xc[c], yp = yc[c];
+                if (x0 != 0.0) fx = fx + (xm * Dys[j - 1] + 4.0 * x0 * Dys[j] + xp * Dys[j + 1]) / 6.0;
+                if (y0 != 0.0) fy = fy + (ym * Dys[j - 1] + 4.0 * y0 * Dys[j] + yp * Dys[j + 1]) / 6.0;
+                xm = x0; x0 = xp;
+                ym = y0; y0 = yp;
+            }
+        }
     }
     FXy[i] = fx;
     FYy[i] = fy;
@@ -176,11 +209,26 @@ __global__ void k_force_x(KGrid g, const double *__restrict__ FXy,
     if (threadIdx.x != 0) return;
     const double *Dxs = g.Dxs;
     double tx = 0.0, ty = 0.0;
-    for (int i = 1; i <= g.nx; i++) {
-        if (FXy[i] != 0.0)
-            tx = tx + (FXy[i - 1] * Dxs[i - 1] + 4.0 * FXy[i] * Dxs[i] + FXy[i + 1] * Dxs[i + 1]) / 6.0;
-        if (FYy[i] != 0.0)
-            ty = ty + (FYy[i - 1] * Dxs[i - 1] + 4.0 * FYy[i] * Dxs[i] + FYy[i + 1] * Dxs[i + 1]) / 6.0;
+    double xm = FXy[0], x0 = FXy[1], ym = FYy[0], y0 = FYy[1];
+    for (int i0 = 1; i0 <= g.nx; i0 += FORCE_CHUNK) {
+        double xc[FORCE_CHUNK], yc[FORCE_CHUNK];
+#pragma unroll
+        for (int c = 0; c < FORCE_CHUNK; c++) {
+            const int q = min(i0 + 1 + c, g.nx + 1);
+            xc[c] = FXy[q];
+            yc[c] = FYy[q];
+        }
+#pragma unroll
+        for (int c = 0; c < FORCE_CHUNK; c++) {
+            const int i = i0 + c;
+            if (i <= g.nx) {
+                const double xp = xc[c], yp = yc[c];
+                if (x0 != 0.0) tx = tx + (xm * Dxs[i - 1] + 4.0 * x0 * Dxs[i] + xp * Dxs[i + 1]) / 6.0;
+                if (y0 != 0.0) ty = ty + (ym * Dxs[i - 1] + 4.0 * y0 * Dxs[i] + yp * Dxs[i + 1]) / 6.0;
+                xm = x0; x0 = xp;
+                ym = y0; y0 = yp;
+            }
+        }
     }
     coef[0] = (-2.0) * tx / lx;   // cDrag, virtualForceIntegrator.f90:80
     coef[1] = (-2.0) * ty / lx;   // cLift, :81
