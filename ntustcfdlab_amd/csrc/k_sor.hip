@@ -30,8 +30,6 @@ __global__ void k_sor_init(double *slots, SorControl *ctl)
     if (b == 0) {
         ctl->stop = 0;
         ctl->ik = 0;
-        ctl->tr = 1;
-        ctl->tb = 0;
     }
 }
 
@@ -139,7 +137,7 @@ k_rb_sweep(KGrid g, SplitGrid s, const double *__restrict__ rx,
     const int tid = threadIdx.x + threadIdx.y * blockDim.x;
     const int blin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
     const int lane = tid & 63;
-    const int t = SPEC ? targ : sweep_index<C>(targ, ctl);
+    const int t = targ;
     double gate = 0.0;
     int stopped = 0;
     if (!SPEC) {
@@ -221,10 +219,7 @@ k_rb_sweep(KGrid g, SplitGrid s, const double *__restrict__ rx,
             if (blin == 0 && tid == 0) ctl->stop = 1;
             return;
         }
-        if (blin == 0) {
-            if (C == 0) clear_next_buckets(slots, t, tid);
-            if (tid == 0) pass_sweep_index<C>(t, ctl);
-        }
+        if (blin == 0 && C == 0) clear_next_buckets(slots, t, tid);
     }
 #pragma unroll
     for (int r = 0; r < KPT; r++)

@@ -47,7 +47,7 @@ k_rb_sweep32(KGrid g, SplitGrid s, FMetric fm, float *__restrict__ e_own,
     const int tid = threadIdx.x + threadIdx.y * blockDim.x;
     const int blin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
     const int lane = tid & 63;
-    const int t = SPEC ? targ : sweep_index<C>(targ, ctl);
+    const int t = targ;
     double gate = 0.0;
     int stopped = 0;
     if (!SPEC) {
@@ -123,10 +123,7 @@ k_rb_sweep32(KGrid g, SplitGrid s, FMetric fm, float *__restrict__ e_own,
             }
             return;
         }
-        if (blin == 0) {
-            if (C == 0) clear_next_buckets(slots, t, tid);
-            if (tid == 0) pass_sweep_index<C>(t, ctl);
-        }
+        if (blin == 0 && C == 0) clear_next_buckets(slots, t, tid);
     }
 #pragma unroll
     for (int r = 0; r < RB_KPT; r++)

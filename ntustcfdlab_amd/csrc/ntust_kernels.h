@@ -182,23 +182,7 @@ struct FMetric {
 struct SorControl {
     int stop;            // set by the first sweep that does not run
     int ik;              // sweeps run
-    int tr, tb;          // sweep index for the next red / black half-sweep
-                         // when the host does not pass it (graph replay)
 };
-// the sweep index of a two-pass half-sweep: the argument when > 0, else the
-// one the previous half-sweep left in ctl (graph replay: every replay of the
-// captured launches runs the next sweeps)
-template <int C>
-__device__ __forceinline__ int sweep_index(int t, const SorControl *ctl)
-{
-    return t > 0 ? t : load_flag(C == 0 ? &ctl->tr : &ctl->tb);
-}
-template <int C>
-__device__ __forceinline__ void pass_sweep_index(int t, SorControl *ctl)
-{
-    if (C == 0) ctl->tb = t;
-    else ctl->tr = t + 1;
-}
 // buckets of sweep 0 (1.0 in bucket 0, GS.f90:9) and sweep 1 (0), control
 // cleared; every running sweep t clears the buckets of sweep t+1
 void launch_sor_init(double *slots, SorControl *ctl, hipStream_t st);
@@ -231,7 +215,7 @@ void launch_rb_sweep(const KGrid &g, const SplitGrid &s, const double *rx, const
                      const double *rz, double *p0, double *p1, const double *b0,
                      const double *b1, double omega, double zeta, double itmax, int t,
                      double *slots, SorControl *ctl, hipStream_t st);
-// t = 0: take the sweep index from ctl; first: the first sweep of a segment
+// sweep t (from 1); first: the first sweep of a segment
 // (clears the segment's dirty flag when it does not run)
 void launch_rb_sweep32(const KGrid &g, const SplitGrid &s, const FMetric &fm, float *e0,
                        float *e1, const float *r0, const float *r1, float omega, double zeta,
