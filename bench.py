@@ -395,10 +395,32 @@ def airfoil_case(hip):
         configs.configure(s, prm, sor, force_every)
     return g, bc, az0, ay0, configure
 
+def oracle_errors(finals, cds, cls, names=("fp64", "mixed"), ref="lex"):
+    """Differences of the named solvers' final states (u, v, w, p) and last
+    C_D, C_L from the reference solver's (GS.f90's order, the oracle's
+    bits): velocity over the largest |u|, |v|, |w|, pressure over the
+    largest |p|, coefficients relative to their value."""
+    r = finals[ref]
+    vel = max(np.abs(x).max() for x in r[:3])
+    pm = np.abs(r[3]).max()
+    out = {}
+    for n in names:
+        f = finals[n]
+        out[f"{n}_vs_oracle_rel_velocity_err"] = float(
+            max(np.abs(a - b).max() for a, b in zip(f[:3], r[:3])) / vel)
+        out[f"{n}_vs_oracle_rel_pressure_err"] = float(np.abs(f[3] - r[3]).max() / pm)
+        out[f"{n}_vs_oracle_rel_cD_err"] = float(abs(cds[n] - cds[ref]) / abs(cds[ref]))
+        out[f"{n}_vs_oracle_rel_cL_err"] = float(abs(cls[n] - cls[ref]) / abs(cls[ref]))
+    return out
+
+
 def airfoil_les_leg(hip, hiptime, steps, warmup):
     """The large airfoil configuration (airfoil_case) at 8 degrees, forward:
-    ETA cast on the device, both solvers from the same inflow state; C_D,
-    C_L of the last step from the force integral."""
+    ETA cast on the device, three solvers from the same inflow state --
+    red-black FP64, the FP32 pattern (predictor increment, nu_t and sweeps
+    in FP32 inside FP64 defect correction) and GS.f90's order in FP64 (the
+    oracle's bits) -- with C_D, C_L of the last step from the force
+    integral and each red-black solver's differences from GS.f90's order."""
     g, bc, az0, ay0, configure = airfoil_case(hip)
     s = hip.ProjectionSolver(g, bc)
     configure(s, hip.SOR_REDBLACK)
@@ -411,11 +433,14 @@ def airfoil_les_leg(hip, hiptime, steps, warmup):
     out = {"workload": "NACA 0012 at 8 deg, Re 5000, LES Cs 0.1, dt 1e-4, 512x384x128 clustered "
                        "at the airfoil, inflow w = 1 along z", "steps": steps, "warmup": warmup,
            "solvers": {}}
-    finals = {}
+    finals, cds, cls = {}, {}, {}
     try:
-        for name, sor in (("fp64", hip.SOR_REDBLACK), ("mixed", hip.SOR_MIXED)):
+        for name, sor, arith in (("fp64", hip.SOR_REDBLACK, hip.ARITH_EXACT),
+                                 ("mixed", hip.SOR_MIXED, hip.ARITH_FP32),
+                                 ("lex", hip.SOR_LEXICOGRAPHIC, hip.ARITH_EXACT)):
             print(f"# airfoil LES {name}", file=sys.stderr, flush=True)
             configure(s, sor)
+            s.set_arithmetic(arith)
             s.upload(*state)
             if warmup:
                 s.step(warmup)
@@ -426,19 +451,21 @@ def airfoil_les_leg(hip, hiptime, steps, warmup):
             e1.record(s.stream())
             ms = e0.elapsed_ms(e1)
             hiptime.device_synchronize()
-            final = [g.halo() for _ in range(3)]
+            final = [g.halo() for _ in range(4)]
             s.download(*final)
             finals[name] = final
+            cds[name], cls[name] = float(cd[-1]), float(cl[-1])
             out["solvers"][name] = {"value": round(g.cells * steps / (ms / 1e3) / 1e6, 3),
                                     "ms_per_step": round(ms / steps, 3),
                                     "sweeps_per_step": round(float(np.mean(ik)), 1),
-                                    "cD": float(cd[-1]), "cL": float(cl[-1])}
+                                    "cD": cds[name], "cL": cls[name]}
     finally:
         s.close()
     ref = finals["fp64"]
-    vel = max(np.abs(x).max() for x in ref)
+    vel = max(np.abs(x).max() for x in ref[:3])
     out["mixed_vs_fp64_rel_velocity_err"] = float(
-        max(np.abs(x - y).max() for x, y in zip(finals["mixed"], ref)) / vel)
+        max(np.abs(x - y).max() for x, y in zip(finals["mixed"][:3], ref[:3])) / vel)
+    out.update(oracle_errors(finals, cds, cls))
     out["mixed_speedup_vs_fp64"] = round(out["solvers"]["mixed"]["value"] /
                                          out["solvers"]["fp64"]["value"], 3)
     out["solid_cells"] = float(eta[2:-2, 2:-2, 2:-2].sum())
@@ -477,10 +504,14 @@ def airfoil_c3_leg(hip, hiptime, steps, warmup):
         s.boundary_conditions(u, v, w, p)
         eta = g.halo()
         s.ray_casting(az0, ay0, 5.0, zc, yc, ETA=eta)
-        for name, sor in (("fp64", hip.SOR_REDBLACK), ("mixed", hip.SOR_MIXED)):
+        finals = {}
+        for name, sor, arith in (("fp64", hip.SOR_REDBLACK, hip.ARITH_EXACT),
+                                 ("mixed", hip.SOR_MIXED, hip.ARITH_FP32),
+                                 ("lex", hip.SOR_LEXICOGRAPHIC, hip.ARITH_EXACT)):
             print(f"# airfoil C3 {name}", file=sys.stderr, flush=True)
             s.configure(dt=2e-4, nu=1.0 / 1000.0, Cs=0.1, LES=0, scheme=hip.QUICK_CENTRE,
                         omega=1.5, zeta=1e-10, itmax=3000.0, sor=sor, lx=3.95, force_every=1)
+            s.set_arithmetic(arith)
             s.upload(u.copy(order="F"), v.copy(order="F"), w.copy(order="F"), p.copy(order="F"),
                      eta)
             if warmup:
@@ -492,6 +523,9 @@ def airfoil_c3_leg(hip, hiptime, steps, warmup):
             e1.record(s.stream())
             ms = e0.elapsed_ms(e1)
             series[name] = (cd.copy(), cl.copy())
+            final = [g.halo() for _ in range(4)]
+            s.download(*final)
+            finals[name] = final
             out["solvers"][name] = {"value": round(g.cells * steps / (ms / 1e3) / 1e6, 3),
                                     "ms_per_step": round(ms / steps, 3),
                                     "sweeps_per_step": round(float(np.mean(ik)), 1),
@@ -501,6 +535,8 @@ def airfoil_c3_leg(hip, hiptime, steps, warmup):
     (d64, l64), (dm, lm) = series["fp64"], series["mixed"]
     out["mixed_vs_fp64_max_rel_diff_cD"] = float(np.max(np.abs(dm - d64)) / np.max(np.abs(d64)))
     out["mixed_vs_fp64_max_rel_diff_cL"] = float(np.max(np.abs(lm - l64)) / np.max(np.abs(l64)))
+    out.update(oracle_errors(finals, {n: float(series[n][0][-1]) for n in series},
+                             {n: float(series[n][1][-1]) for n in series}))
     out["solid_cells"] = float(eta[2:-2, 2:-2, 2:-2].sum())
     return out
 
