@@ -39,8 +39,11 @@ def run(g, arith, les, state, steps=1, zeta=0.0, itmax=40.0, sor=SOR_REDBLACK, d
 
 
 @pytest.mark.parametrize("les", [0, 1])
-@pytest.mark.parametrize("shape", [(14, 12, 10), (33, 9, 17)])
+@pytest.mark.parametrize("shape", [(14, 12, 10), (33, 9, 17), (5, 2, 3), (7, 3, 2), (70, 6, 37)])
 def test_fast_modes_against_exact_step(shape, les, hip_lib):
+    """(5, 2, 3), (7, 3, 2), (70, 6, 37): rows and planes that leave the
+    eddy viscosity's 4-row, 32-plane thread segments and the predictor's
+    64-wide rows partly empty."""
     g = stretched_grid(*shape)
     u, v, w = noise_ic(g, 0.3, seed=sum(shape) + les)
     p = g.halo()
