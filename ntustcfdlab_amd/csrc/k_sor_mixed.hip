@@ -5,8 +5,9 @@
 // from p0 + e equals p0 plus one sweep on the correction equation
 //     aP e = sum_nb a_nb e_nb + r0,   r0 = sum_nb a_nb p0_nb - b - aP p0
 // from e.  So the sweeps run in FP32 on e (half the bytes of an FP64 sweep),
-// and every m sweeps the FP64 state absorbs the correction, p0 += e, and the
-// FP64 residual r0 is recomputed with the FP64 operator of the plain path.
+// and every m sweeps (m = 256) the FP64 state absorbs the correction,
+// p0 += e, and the FP64 residual r0 is recomputed with the FP64 operator of
+// the plain path.
 // In exact arithmetic the iterates are those of the FP64 red-black solver;
 // the FP32 rounding of e is bounded by one segment of m sweeps, because the
 // residual and the state stay FP64.  The stopping rule is GS.f90:11 applied

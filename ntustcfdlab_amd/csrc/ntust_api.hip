@@ -72,7 +72,17 @@ struct Ctx {
     double *metric[6] = {};            // device, base (element -1)
     double *recip[3] = {};             // 1/Dxs, 1/Dys, 1/Dzs, base
     float *fmetric[6] = {};            // FP32 iDx, iDy, iDz, 1/Dxs, 1/Dys, 1/Dzs, base
-    long mix_segment = 64;             // FP32 sweeps between FP64 corrections
+    // FP32 sweeps between FP64 corrections of the forward solve: the first
+    // segment of a solve, which carries the solve's initial correction
+    // (the whole solution from a cold start), is mix_first_segment long,
+    // the later ones, which carry only the change since the last fold,
+    // mix_segment.  The gated loop checks the stop on the host every
+    // gate_quantum sweeps, independently of the segments.  The adjoint's
+    // transposed FP32 sweeps fold every adj_mix_segment sweeps.
+    long mix_first_segment = 64;
+    long mix_segment = 256;
+    long gate_quantum = 64;
+    long adj_mix_segment = 64;
     // arithmetic of the resident step's predictor and eddy viscosity
     // (NTUST_ARITH_*), with their tables (base pointers of device arrays)
     int arith = NTUST_ARITH_EXACT;
@@ -427,8 +437,19 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
         NEED(r0d, "mix_r0", nf, true);
         NEED(r1d, "mix_r1", nf, true);
         float *E0 = (float *)e0d, *E1 = (float *)e1d, *R0 = (float *)r0d, *R1 = (float *)r1d;
-        const long seg = C.mix_segment;
-        const long nseg = (tmax + seg - 1) / seg + 1;
+        // segment ends: seg1, seg1 + seg, seg1 + 2 seg, ..., and tmax
+        const long seg = C.mix_segment, seg1 = C.mix_first_segment;
+        auto seg_end = [&](long tt) {
+            const long e = tt <= seg1 ? seg1 : seg1 + ((tt - seg1 - 1) / seg + 1) * seg;
+            return e < tmax ? e : tmax;
+        };
+        auto seg_first = [&](long tt) {
+            return tt == 1 || (tt > seg1 && (tt - seg1 - 1) % seg == 0);
+        };
+        auto seg_floor = [&](long x) {   // the last segment end <= x (0: none)
+            return x < seg1 ? 0L : seg1 + (x - seg1) / seg * seg;
+        };
+        const long nseg = (tmax + seg - 1) / seg + 2;
         NEED(dd, "mix_dirty", (size_t)nseg, true);
         int *dirty = (int *)dd;
         HIPCHK(hipMemsetAsync(dirty, 0, nseg * sizeof(int), C.st));
@@ -438,9 +459,11 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
         launch_rb_residual(g, C.sg, rx, ry, rz, P0, P1, B0, B1, R0, R1, E1, nullptr, C.st);
         FMetric fm{C.fmetric[0] + 1, C.fmetric[1] + 1, C.fmetric[2] + 1,
                    C.fmetric[3] + 1, C.fmetric[4] + 1, C.fmetric[5] + 1};
-        long c = 0;
-        long want = C.ik_hint > seg ? C.ik_hint - seg / 2 : 2 * seg;
-        const long nb1 = 1.0 > zeta ? spec_batch(tmax, seg) : 0;
+        long c = 0;   // the gated loop's segment (its dirty flag)
+        const long gq = C.gate_quantum;
+        long want = C.ik_hint > gq ? C.ik_hint - gq / 2 : 2 * gq;
+        long nb1 = 1.0 > zeta ? seg_floor(spec_batch(tmax, 1)) : 0;
+        if (nb1 < 2 * gq) nb1 = 0;
         if (nb1 > 0) {
             // sweeps 1..upto in segments counted from sweep 1, each followed
             // by the FP64 correction, as the gated loop below runs them; all
@@ -453,8 +476,9 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
             int nev = 0;
             long tlaunch = 0;
             auto run = [&](long from, long upto, bool timed) {
-                for (long tt = from; tt <= upto; tt += seg) {
-                    const long n = seg < upto - tt + 1 ? seg : upto - tt + 1;
+                for (long tt = from; tt <= upto;) {
+                    const long se = seg_end(tt) < upto ? seg_end(tt) : upto;
+                    const long n = se - tt + 1;
                     const bool ev = timed && tt > 1 && nev < Ctx::NSEV;
                     if (ev) HIPCHK(hipEventRecord(C.sev[2 * nev], C.st));
                     for (long b = 0; b < n; b++)
@@ -468,6 +492,7 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
                     }
                     launch_rb_correct(g, C.sg, rx, ry, rz, P0, P1, E0, E1, B0, B1, R0, R1,
                                       nullptr, C.st);
+                    tt = se + 1;
                 }
                 return 0;
             };
@@ -501,8 +526,10 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
             t = q >= 1 ? tmax + 1 : nb1 + 1;
             // after a solve that ran to itmax, the tail to itmax speculative
             // too: from a copy of the FP64 planes, so that a stop inside it
-            // replays from there
-            const long hi = t <= tmax ? spec_tail(tmax) : 0;
+            // replays from there (a tail cut short of itmax ends on a
+            // segment end)
+            long hi = t <= tmax ? spec_tail(tmax) : 0;
+            if (hi < tmax) hi = seg_floor(hi);
             if (hi > nb1) {
                 NEED(k0d, "mix_ck0", np, false);
                 NEED(k1d, "mix_ck1", np, false);
@@ -525,26 +552,36 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
                 h.ik = (int)(q >= 1 ? q : hi);
                 t = q >= 1 ? tmax + 1 : hi + 1;
             }
-            want = 2 * seg;
+            want = 2 * gq;
         }
+        // Each segment is closed by the FP64 correction (the speculative
+        // batch ends on a segment end).  The host checks the stop every
+        // `want` sweeps, which may cut a segment in parts: the later parts
+        // keep the segment's dirty flag, and a stop inside a segment still
+        // closes it with the correction (the sweeps after the stop return
+        // at once, so the result is that of the whole segment).
         while (t <= tmax) {
             long queued = 0;
             bool timed = false;
             do {
-                const long n = seg < tmax - t + 1 ? seg : tmax - t + 1;
+                const long se = seg_end(t);
+                const bool first = seg_first(t);
+                const long n = se - t + 1 < want - queued ? se - t + 1 : want - queued;
                 if (!timed) HIPCHK(hipEventRecord(C.ev0, C.st));
                 for (long b = 0; b < n; b++)
                     launch_rb_sweep32(g, C.sg, fm, E0, E1, R0, R1, (float)omega, zeta, itmax,
-                                      (int)(t + b), slots, C.ctl, dirty + c, b == 0 ? 1 : 0,
-                                      C.st);
+                                      (int)(t + b), slots, C.ctl, dirty + c,
+                                      b == 0 && first ? 1 : 0, C.st);
                 if (!timed) HIPCHK(hipEventRecord(C.ev1, C.st));
-                launch_rb_correct(g, C.sg, rx, ry, rz, P0, P1, E0, E1, B0, B1, R0, R1,
-                                  dirty + c, C.st);
+                if (t + n - 1 == se) {
+                    launch_rb_correct(g, C.sg, rx, ry, rz, P0, P1, E0, E1, B0, B1, R0, R1,
+                                      dirty + c, C.st);
+                    c++;
+                }
                 HIPCHK(hipGetLastError());
                 if (!timed) { timed = true; batch = n; }
                 t += n;
                 queued += n;
-                c++;
             } while (queued < want && t <= tmax);
             HIPCHK(hipMemcpyAsync(&h, C.ctl, sizeof h, hipMemcpyDeviceToHost, C.st));
             TRY(sync());
@@ -554,8 +591,15 @@ int solve_pressure(const double *us, const double *vs, const double *ws, double 
                 C.prof_ms += ms;
                 C.prof_launches += 2 * batch;
             }
-            if (h.stop) break;
-            want = 2 * seg;
+            if (h.stop) {
+                if (t - 1 != seg_end(t - 1)) {
+                    launch_rb_correct(g, C.sg, rx, ry, rz, P0, P1, E0, E1, B0, B1, R0, R1,
+                                      dirty + c, C.st);
+                    HIPCHK(hipGetLastError());
+                }
+                break;
+            }
+            want = 2 * gq;
         }
         t = tmax + 1;   // the red-black FP64 loop below does not run
     }
@@ -859,7 +903,7 @@ int adjoint_of_step(const AdjBufs &a, const double *eta, const double *filt, con
     launch_adj_project(g, a.lu, a.lv, a.lw, eta, q.dt, F, a.lus, a.lvs, a.lws, a.lp, a.lu, a.lv,
                        a.lw, C.st);
     // the FP32-sweep solver's adjoint runs its transposed sweeps in FP32
-    // too (FP64 folds every mix_segment sweeps); the FP64 solver's in FP64
+    // too (FP64 folds every adj_mix_segment sweeps); the FP64 solver's in FP64
     AdjMixed mx{};
     const bool mixed = q.sor == NTUST_SOR_MIXED;
     if (mixed) {
@@ -875,7 +919,7 @@ int adjoint_of_step(const AdjBufs &a, const double *eta, const double *filt, con
         mx.bzero = bz;
         mx.fm = FMetric{C.fmetric[0] + 1, C.fmetric[1] + 1, C.fmetric[2] + 1,
                         C.fmetric[3] + 1, C.fmetric[4] + 1, C.fmetric[5] + 1};
-        mx.seg = 2 * C.mix_segment;
+        mx.seg = 2 * C.adj_mix_segment;
     }
     if (sweeps > 0)
         launch_adj_pressure(g, C.sg, rx, ry, rz, q.omega, sweeps, a.lp, a.lb, a.z0, a.z1, a.Z0,

@@ -55,12 +55,14 @@ def assert_same(a, b):
 @pytest.mark.parametrize("mode", [SOR_REDBLACK, SOR_MIXED])
 def test_speculative_equals_gated(shape, mode, hip_lib):
     """Two itmax-capped solves (the second runs its first batch and tail
-    speculatively), then a solve that the stopping rule ends."""
+    speculatively: 7/8 of 800 sweeps, for the FP32-sweep solver rounded
+    down to its last segment end, 576, then the tail), then a solve that
+    the stopping rule ends."""
     g = stretched_grid(*shape)
     rng = np.random.default_rng(sum(shape))
     us, vs, ws = (rand_halo(g, rng, 1e-2) for _ in range(3))
     p0 = rand_halo(g, rng, 1e-3)
-    calls = [(0.0, 400.0), (0.0, 400.0), (1e-7, 400.0)]
+    calls = [(0.0, 800.0), (0.0, 800.0), (1e-7, 800.0)]
     seq = sequence(g, us, vs, ws, p0, calls, mode)
     for (zeta, itmax), a in zip(calls, seq):
         assert_same(a, gated(g, us, vs, ws, p0, zeta, itmax, mode))
