@@ -442,13 +442,12 @@ void *ntust_stream(void);
 int ntust_prof_reset(void);
 int ntust_prof_read(double *ms, long long *launches);
 
-/* The timed kernel: returns 0 two-pass FP64, 1 fused FP64, 2 two-pass FP32,
- * 3 fused FP32 sweep, 4 tile FP64 sweep, 5 tile FP32 sweep, 6 the
- * lexicographic step; *bytes_per_cell gets its algorithmic bytes per grid
- * cell and launch, averaged over a sweep (FP64: read p and b, write p;
- * FP32: read and write e, read r on the colour-0 cells, the colour-1 sweeps
- * being shifted; lexicographic: 24 B per cell update, over the timed
- * launches). */
+/* The timed kernel: returns 0 two-pass FP64 half-sweep, 2 two-pass FP32
+ * half-sweep, 6 the lexicographic step; *bytes_per_cell gets its compulsory
+ * bytes per grid cell and launch, averaged over a sweep (FP64: read p and b,
+ * write p; FP32: read and write e, read r on the colour-0 cells, the
+ * colour-1 sweeps being shifted; lexicographic: 24 B per cell update, over
+ * the timed launches). */
 int ntust_prof_kernel(double *bytes_per_cell);
 
 /* Arithmetic of the resident step's QUICK predictor and Smagorinsky
