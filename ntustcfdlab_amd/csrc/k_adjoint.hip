@@ -1067,9 +1067,9 @@ void launch_adj_pressure(const KGrid &g, const SplitGrid &s, const double *rx, c
     } else if (sweeps > 0) {
         // the first red half-sweep (self factor 1) in FP64, then the
         // sequence black, (red, black) x (sweeps-1) -- the last black not
-        // accumulating -- in FP32 segments of mx->seg half-sweeps (even:
-        // every segment starts with a black update, and only the last can
-        // end with one)
+        // accumulating -- in FP32 segments of mx->seg1 half-sweeps, then of
+        // mx->seg (both even: every segment starts with a black update, and
+        // only the last can end with one)
         hipLaunchKernelGGL((k_adj_sweep<0, true>), g64, blk, 0, st, g, s, rx, ry, rz, z0, z1, Z0,
                            1.0, omega);
         const int nq32 = (g.nx + 1) / 2;
@@ -1079,8 +1079,9 @@ void launch_adj_pressure(const KGrid &g, const SplitGrid &s, const double *rx, c
         // one's (k_adj_seg_black, k_adj_seg_red)
         launch_rb_residual(g, s, rx, ry, rz, z0, z1, mx->bzero, mx->bzero, mx->r0, mx->r1,
                            mx->e1, nullptr, st);
-        for (long h0 = 0; h0 < nh; h0 += mx->seg) {
-            const long h1 = h0 + mx->seg < nh ? h0 + mx->seg : nh;
+        for (long h0 = 0, h1; h0 < nh; h0 = h1) {
+            const long len = h0 == 0 ? mx->seg1 : mx->seg;
+            h1 = h0 + len < nh ? h0 + len : nh;
             for (long h = h0; h < h1; h++) {
                 if (h & 1)
                     hipLaunchKernelGGL((k_adj_sweep32<0, true>), g32, blk, 0, st, g, s, mx->fm,

@@ -78,11 +78,10 @@ struct Ctx {
     // the later ones, which carry only the change since the last fold,
     // mix_segment.  The gated loop checks the stop on the host every
     // gate_quantum sweeps, independently of the segments.  The adjoint's
-    // transposed FP32 sweeps fold every adj_mix_segment sweeps.
+    // transposed FP32 sweeps fold after the same segment lengths.
     long mix_first_segment = 64;
     long mix_segment = 256;
     long gate_quantum = 64;
-    long adj_mix_segment = 64;
     // arithmetic of the resident step's predictor and eddy viscosity
     // (NTUST_ARITH_*), with their tables (base pointers of device arrays)
     int arith = NTUST_ARITH_EXACT;
@@ -903,7 +902,8 @@ int adjoint_of_step(const AdjBufs &a, const double *eta, const double *filt, con
     launch_adj_project(g, a.lu, a.lv, a.lw, eta, q.dt, F, a.lus, a.lvs, a.lws, a.lp, a.lu, a.lv,
                        a.lw, C.st);
     // the FP32-sweep solver's adjoint runs its transposed sweeps in FP32
-    // too (FP64 folds every adj_mix_segment sweeps); the FP64 solver's in FP64
+    // too (FP64 folds after segments of the forward's lengths); the FP64
+    // solver's in FP64
     AdjMixed mx{};
     const bool mixed = q.sor == NTUST_SOR_MIXED;
     if (mixed) {
@@ -919,7 +919,8 @@ int adjoint_of_step(const AdjBufs &a, const double *eta, const double *filt, con
         mx.bzero = bz;
         mx.fm = FMetric{C.fmetric[0] + 1, C.fmetric[1] + 1, C.fmetric[2] + 1,
                         C.fmetric[3] + 1, C.fmetric[4] + 1, C.fmetric[5] + 1};
-        mx.seg = 2 * C.adj_mix_segment;
+        mx.seg1 = 2 * C.mix_first_segment;
+        mx.seg = 2 * C.mix_segment;
     }
     if (sweeps > 0)
         launch_adj_pressure(g, C.sg, rx, ry, rz, q.omega, sweeps, a.lp, a.lb, a.z0, a.z1, a.Z0,

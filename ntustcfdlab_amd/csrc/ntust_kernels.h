@@ -398,7 +398,7 @@ struct AdjMixed {
     FMetric fm;
     float *e0, *e1, *r0, *r1, *f0;   // f0: the red running sum
     const double *bzero;
-    long seg;
+    long seg1, seg;   // half-sweeps of the first segment and of the later ones (even)
 };
 // mx null: the FP64 transposed sweeps
 void launch_adj_pressure(const KGrid &g, const SplitGrid &s, const double *rx, const double *ry,

@@ -245,15 +245,17 @@ def test_body_force_step_matches_oracle(hip_lib):
         assert np.array_equal(a, r.f[n]), n
 
 
-@pytest.mark.parametrize("n,itmax", [(16, 70.0), (13, 70.0), (16, 65.0), (12, 1.0)])
+@pytest.mark.parametrize("n,itmax", [(16, 70.0), (13, 70.0), (16, 65.0), (12, 1.0),
+                                     (16, 400.0)])
 def test_mixed_forward_adjoint_matches_fp64(n, itmax, hip_lib):
     """With a fixed sweep count the FP32-sweep forward runs the same linear
     iteration, so its adjoint gradient (FP32 transposed sweeps with FP64
     folds) agrees with the FP64 one; n = 13 gives odd extents (13, 11, 9).
-    The transposed half-sweeps after the first run in segments of 128: 70
-    sweeps give a segment ending with a red update and a last one of 11
-    ending with the black update that does not count, 65 a last segment of
-    that black update alone, 1 sweep nothing but it."""
+    The transposed half-sweeps after the first run in segments of 128, then
+    of 512: 70 sweeps give a segment ending with a red update and a last one
+    of 11 ending with the black update that does not count, 65 a last
+    segment of that black update alone, 1 sweep nothing but it, 400 the
+    segments 128, 512 and a last one of 159."""
     g, state = cavity_setup(n, seed=5)
     eta = g.halo()
     out = {}
