@@ -111,3 +111,24 @@ def test_speculative_tail_replay(mode, hip_lib):
         assert 0.875 * 700 < q < 700                          # stopped inside the tail
     for (zeta, itmax), a in zip(calls, seq):
         assert_same(a, gated(g, us, vs, ws, p0, zeta, itmax, mode))
+
+
+@pytest.mark.parametrize("mode", [SOR_REDBLACK, SOR_MIXED])
+def test_speculative_tail_beyond_the_batch_limit(mode, hip_lib):
+    """itmax above the 65536 sweeps one speculative batch or tail may hold:
+    the second solve runs its batch, a tail cut at the limit (for the
+    FP32-sweep solver at the last segment end below it, 65344) and the
+    gated sweeps from there to itmax; both solves must equal a fresh
+    solver's gated solve bit for bit.  zeta < 0: no sweep's change is at or
+    below it, so every solve runs to itmax even where the tiny grid's
+    iteration reaches a fixed point."""
+    g = stretched_grid(7, 5, 3)
+    rng = np.random.default_rng(41)
+    us, vs, ws = (rand_halo(g, rng, 1e-2) for _ in range(3))
+    p0 = rand_halo(g, rng, 1e-3)
+    calls = [(-1.0, 70000.0), (-1.0, 70000.0)]
+    seq = sequence(g, us, vs, ws, p0, calls, mode)
+    assert [r[0] for r in seq] == [70000, 70000]
+    ref = gated(g, us, vs, ws, p0, -1.0, 70000.0, mode)
+    for a in seq:
+        assert_same(a, ref)
