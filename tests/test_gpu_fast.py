@@ -6,7 +6,7 @@ GPU tests pin bit for bit to the oracle:
     the state's magnitude, on stretched grids, with and without LES;
   * ARITH_FP32 (the increment u* - u and nu_t in FP32): the same step within
     1e-7 of the state's magnitude;
-  * the FP32 pattern of the north star -- FP32 predictor, eddy viscosity and
+  * the whole FP32 pattern -- FP32 predictor, eddy viscosity and
     SOR sweeps inside FP64 defect correction -- on the 128^3 LES cavity for
     three steps against GS.f90's own order in FP64 (the lexicographic
     solver, the oracle's bits): velocity within 1e-6."""

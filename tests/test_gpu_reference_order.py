@@ -12,7 +12,7 @@ pipelined over sweeps) is compared
     upwind, 100 steps) step by step, and the first step of the 128^3 LES cavity.
 The red-black FP64 and the FP32-sweep solvers are then measured against it
 (the reference's ordering) on the 128^3 LES cavity after three steps:
-relative velocity difference <= 1e-6 (BASELINE.json's bar for the FP32-sweep
+relative velocity difference <= 1e-6 (the accuracy bar of the FP32-sweep
 solver), with the pressure and force-coefficient differences printed."""
 import hashlib
 import json
